@@ -1,0 +1,172 @@
+"""Benchmark: 8-D double Gaussian peak on one MI355X, FP32-compensated vs plain FP64.
+
+Workload (the headline configuration): f(x) = 1/2 [g(x; 0.33) + g(x; 0.67)],
+g a normalised isotropic Gaussian with sigma = 0.05 on [0,1]^8, nbins = 1024,
+n_strat = 8 (16,777,216 hypercubes), 1e8 evaluations per iteration, alpha = 0.5,
+beta = 0.75.  A step is one VEGAS+ iteration (sampling pass over all hypercubes,
+map refinement, reallocation); --warmup iterations adapt the map and are left
+out of the average (skip), --steps iterations are timed.  Defaults 5 + 15 = the
+20 iterations of the workload.
+
+Prints one JSON line: value = evaluations/s of the FP32-compensated form over
+the timed iterations, with the plain FP64 form, time to 1e-4 relative error,
+errors against the closed form, the roofline of the sampling kernel and the
+CPU oracle timed on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def _log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def run_form(precision, itg, n_eval_it, warmup, steps, n_strat, nbins, seed, use_timing=True):
+    import numpy as np
+    from cuvegas_amd import Integrator
+    iters = warmup + steps
+    t0 = time.perf_counter()
+    st = Integrator(itg.source, itg.n_dim, n_eval_it, n_intervals=nbins, n_strat=n_strat,
+                    max_it=iters, skip=warmup, seed=seed, precision=precision)
+    setup_s = time.perf_counter() - t0
+    st.set_timing(use_timing)
+    st.run(warmup, use_graph=False)
+    t1 = time.perf_counter()
+    st.run(steps, use_graph=False)
+    wall_timed = time.perf_counter() - t1
+    r = st.result()
+    I_k, S2, ne = st.history()
+    fill_ms, iter_ms = st.timing()
+    st.close()
+    timed_evals = int(ne[warmup:].sum())
+    timed_ms = float(iter_ms[warmup:].sum())
+    # running weighted mean (iterations >= skip) -> time to 1e-4
+    t_to, it_to, cum = None, None, 0.0
+    for k in range(iters):
+        cum += iter_ms[k]
+        if k < warmup:
+            continue
+        sl = slice(warmup, k + 1)
+        w = 1.0 / S2[sl]
+        mean = float((I_k[sl] * w).sum() / w.sum())
+        K = k + 1 - warmup
+        chi2 = float((((I_k[sl] - mean) ** 2) * w).sum() / (K - 1)) if K > 1 else 0.0
+        if abs(mean - itg.exact) / itg.exact <= 1e-4 and chi2 < 2.0:
+            t_to, it_to = cum, k
+            break
+    return {
+        "precision": precision,
+        "evals_per_s": timed_evals / (timed_ms * 1e-3),
+        "ms_per_step": timed_ms / steps,
+        "wall_ms_per_step": wall_timed * 1e3 / steps,
+        "fill_ms_avg": float(np.mean(fill_ms[warmup:])),
+        "evals_per_fill_avg": float(np.mean(ne[warmup:])),
+        "result": r["result"], "error": r["error"], "chi2_dof": r["chi2_dof"],
+        "rel_err_vs_exact": abs(r["result"] - itg.exact) / itg.exact,
+        "pull": (r["result"] - itg.exact) / r["error"],
+        "time_to_1e-4_ms": t_to, "iteration_reached_1e-4": it_to,
+        "tot_nevals": int(r["tot_nevals"]), "compile_ms": r["compile_ms"],
+        "setup_s": setup_s,
+    }
+
+
+def cpu_baseline(itg, nbins, n_strat, n_eval_it, seed):
+    """The CPU oracle (FP64, 1 thread) on a bounded sample: the first iteration
+    of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    p, keep = orc.make_params(itg.n_dim, nbins, n_strat, n_eval_it, 1, orc.F_DGAUSS,
+                              list(itg.params), skip=0, seed=seed)
+    t0 = time.perf_counter()
+    r = orc.run(p, keep, n_strat ** itg.n_dim, snapshots=False)
+    dt = time.perf_counter() - t0
+    return {"value": r["tot_nevals"] / dt, "unit": "evals/s", "cores": 1, "kind": "port",
+            "sample": f"first iteration of the same workload ({r['tot_nevals']} evals, FP64, "
+                      f"{dt:.1f} s, 1 thread of {os.cpu_count()} host CPUs)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=15)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--neval", type=float, default=1e8, help="evaluations per iteration")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "sharding of the sampling pass is a design, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+    if args.steps < 1 or args.warmup < 0:
+        sys.exit("steps must be >= 1 and warmup >= 0")
+
+    from cuvegas_amd import integrands as I
+    from cuvegas_amd import lib
+    from cuvegas_amd.roofline import slots_per_eval, valu_peak
+    import ctypes as C
+
+    d, nbins, n_strat, seed = 8, 1024, 8, 0
+    n_eval_it = int(args.neval)
+    itg = I.double_gaussian(d, 0.33, 0.67, 0.05)
+
+    n_cu, clk = C.c_int(0), C.c_int(0)
+    arch = C.create_string_buffer(64)
+    rc = lib.cvg_device_info(C.byref(n_cu), C.byref(clk), arch, 64)
+    if rc != 0:
+        sys.exit("no HIP device: " + lib.cvg_last_error().decode())
+    clock_hz = clk.value * 1e3
+    _log("device", arch.value.decode(), n_cu.value, "CUs", clock_hz / 1e9, "GHz")
+
+    forms = {}
+    for prec in (["fp64", "fp32"] if args.only is None else [args.only]):
+        _log("running", prec)
+        forms[prec] = run_form(prec, itg, n_eval_it, args.warmup, args.steps, n_strat, nbins, seed)
+        _log(prec, json.dumps(forms[prec]))
+    head = forms.get("fp32") or forms["fp64"]
+
+    S = slots_per_eval(d, "double_gaussian")
+    peak = valu_peak(n_cu.value, clock_hz)
+    achieved = S * head["evals_per_fill_avg"] / (head["fill_ms_avg"] * 1e-3)
+    roof = {"bound": "valu", "kernel": "vg_fill",
+            "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tslot/s",
+            "frac": achieved / peak, "slots_per_eval": S, "traffic": None,
+            "fill_ms_avg": head["fill_ms_avg"], "evals_per_fill": head["evals_per_fill_avg"]}
+
+    cpu = None
+    if not args.no_cpu:
+        _log("cpu baseline")
+        cpu = cpu_baseline(itg, nbins, n_strat, n_eval_it, seed)
+
+    out = {
+        "metric": "integrand evals/s + time to 1e-4 rel. error, FP32-comp vs plain FP64, 1 GPU",
+        "value": head["evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if head is forms.get("fp32") else "f64",
+        "data": "synthetic (closed-form integrand, Philox4x32-10 seed 0)",
+        "config": {"workload": "8-D double Gaussian peak sigma=0.05, nbins=1024, n_strat=8 "
+                               "(16,777,216 cubes), 1e8 evals/iteration, alpha=0.5, beta=0.75, "
+                               f"{args.warmup}+{args.steps} iterations",
+                   "n_eval_it": n_eval_it, "exact": itg.exact},
+        "forms": forms,
+        "fp32_over_fp64": (forms["fp32"]["evals_per_s"] / forms["fp64"]["evals_per_s"]
+                           if "fp32" in forms and "fp64" in forms else None),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "device": {"arch": arch.value.decode(), "n_cu": n_cu.value, "clock_hz": clock_hz},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

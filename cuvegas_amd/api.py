@@ -1,0 +1,330 @@
+"""Python host API over the C ABI: programs, integrator states, integrate().
+
+``integrate`` mirrors the reference entry point
+(vegas_cuda/src/vegas_cuda_extension/python_wrapper.py:9-77): same arguments,
+same defaults, same ``(result, dict)`` return with the reference's 13 keys,
+plus extra keys (chi2_dof, per-iteration estimates, evals_per_s, precision).
+``device_function`` is a HIP C++ source string defining ``integrand`` (or the
+name of a library integrand, see cuvegas_amd.integrands), compiled by hipRTC;
+the reference takes a numba device function compiled to PTX.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import time
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import integrands as _ints
+from ._lib import (CVG_FP32C, CVG_FP64, CuvegasError, check, cvg_params, cvg_result, lib)
+
+PRECISIONS = {"fp64": CVG_FP64, "fp32": CVG_FP32C, "fp32c": CVG_FP32C}
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _uptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint))
+
+
+def _llptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_longlong))
+
+
+def default_n_strat(n_evals: int, max_it: int, n_dim: int) -> int:
+    """Reference default (python_wrapper.py:15-18): int((n_eval_it / 4)^(1/d))."""
+    n_eval_it = n_evals // max_it
+    return int(math.pow((n_eval_it / 4.0), 1.0 / n_dim))
+
+
+def make_params(n_dim: int, n_intervals: int, n_strat: int, max_it: int, skip: int,
+                n_eval_it: int, alpha: float, beta: float, seed: int, precision="fp64",
+                stream_id: int = 0, run_len: int = 0, domain: Optional[Sequence] = None,
+                n_theta: int = 0):
+    p = cvg_params()
+    p.n_dim = n_dim
+    p.n_intervals = n_intervals
+    p.n_strat = n_strat
+    p.max_it = max_it
+    p.skip = skip
+    p.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    p.n_eval_it = int(n_eval_it)
+    p.alpha = alpha
+    p.beta = beta
+    p.seed = int(seed) & 0xFFFFFFFF
+    p.stream_id = int(stream_id) & 0xFFFFFFFF
+    p.run_len = run_len
+    p.n_theta = n_theta
+    keep = None
+    if domain is not None:
+        keep = np.ascontiguousarray(domain, dtype=np.float64)
+        p.has_domain = 1
+        p.domain = _dptr(keep)
+    return p, keep
+
+
+class Program:
+    """hipRTC-compiled sampling kernel for one integrand source and shape."""
+
+    def __init__(self, source: str, params: cvg_params, options: Optional[str] = None):
+        h = C.c_void_p()
+        rc = lib.cvg_program_create(source.encode(), C.byref(params),
+                                    options.encode() if options else None, C.byref(h))
+        self._h = h
+        if rc < 0:
+            log = lib.cvg_program_log(h).decode(errors="replace") if h.value else ""
+            msg = lib.cvg_last_error().decode(errors="replace")
+            if h.value:
+                lib.cvg_program_destroy(h)
+                self._h = C.c_void_p()
+            raise CuvegasError(rc, msg + ("\n" + log if log and log not in msg else ""))
+        self.log = lib.cvg_program_log(h).decode(errors="replace")
+        self.code_size = lib.cvg_program_code_size(h)
+
+    def code(self) -> bytes:
+        """The gfx950 code object (ELF) of the sampling kernel."""
+        buf = C.create_string_buffer(int(self.code_size))
+        n = lib.cvg_program_get_code(self._h, buf, int(self.code_size))
+        return buf.raw[:n]
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h and self._h.value:
+            lib.cvg_program_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Integrator:
+    """One integral's device state and iteration loop."""
+
+    def __init__(self, source: str, n_dim: int, n_eval_it: int, *, n_intervals: int = 1024,
+                 n_strat: Optional[int] = None, max_it: int = 20, skip: int = 5,
+                 alpha: float = 0.5, beta: float = 0.75, seed: int = 0, precision="fp64",
+                 stream_id: int = 0, run_len: int = 0, domain: Optional[Sequence] = None,
+                 options: Optional[str] = None, theta: Optional[Sequence[float]] = None,
+                 program: Optional[Program] = None):
+        if n_strat is None:
+            n_strat = max(1, int(math.pow(n_eval_it / 4.0, 1.0 / n_dim)))
+        n_theta = 0 if theta is None else len(theta)
+        self.params, self._dom = make_params(n_dim, n_intervals, n_strat, max_it, skip, n_eval_it,
+                                             alpha, beta, seed, precision, stream_id, run_len,
+                                             domain, n_theta)
+        self.program = program or Program(source, self.params, options)
+        h = C.c_void_p()
+        check(lib.cvg_state_create(self.program.handle, C.byref(self.params), C.byref(h)))
+        self._h = h
+        self.n_dim, self.n_intervals, self.n_strat = n_dim, n_intervals, n_strat
+        self.n_cubes = int(lib.cvg_state_n_cubes(h))
+        self.max_it, self.skip = max_it, skip
+        if theta is not None:
+            self.set_theta(theta)
+
+    # -- loop --------------------------------------------------------------
+    def run(self, n_iter: int, use_graph: bool = True) -> "Integrator":
+        check(lib.cvg_state_run(self._h, int(n_iter), 1 if use_graph else 0))
+        return self
+
+    def result(self) -> dict:
+        r = cvg_result()
+        check(lib.cvg_state_result(self._h, C.byref(r)))
+        return {f: getattr(r, f) for f, _ in cvg_result._fields_}
+
+    def history(self):
+        n = self.max_it
+        I = np.zeros(n)
+        S2 = np.zeros(n)
+        ne = np.zeros(n, dtype=np.int64)
+        k = check(lib.cvg_state_history(self._h, _dptr(I), _dptr(S2), _llptr(ne), n))
+        return I[:k], S2[:k], ne[:k]
+
+    def set_timing(self, enable: bool = True) -> None:
+        check(lib.cvg_state_set_timing(self._h, 1 if enable else 0))
+
+    def timing(self):
+        """Per-iteration device times (ms) of eager runs: (fill kernel, whole iteration)."""
+        cap = 4096
+        f = np.zeros(cap)
+        t = np.zeros(cap)
+        k = check(lib.cvg_state_get_timing(self._h, _dptr(f), _dptr(t), cap))
+        k = min(k, cap)
+        return f[:k], t[:k]
+
+    # -- state I/O -----------------------------------------------------------
+    def get_map(self) -> np.ndarray:
+        x = np.zeros((self.n_dim, self.n_intervals + 1))
+        check(lib.cvg_state_get_map(self._h, _dptr(x)))
+        return x
+
+    def set_map(self, x_edges) -> None:
+        x = np.ascontiguousarray(x_edges, dtype=np.float64).reshape(self.n_dim, self.n_intervals + 1)
+        check(lib.cvg_state_set_map(self._h, _dptr(x)))
+
+    def get_nh(self) -> np.ndarray:
+        nh = np.zeros(self.n_cubes, dtype=np.int32)
+        check(lib.cvg_state_get_nh(self._h, _iptr(nh)))
+        return nh
+
+    def set_nh(self, nh) -> None:
+        a = np.ascontiguousarray(nh, dtype=np.int32)
+        check(lib.cvg_state_set_nh(self._h, _iptr(a)))
+
+    def get_dh(self) -> np.ndarray:
+        dh = np.zeros(self.n_cubes)
+        check(lib.cvg_state_get_dh(self._h, _dptr(dh)))
+        return dh
+
+    def get_cube_sums(self):
+        a = np.zeros(self.n_cubes)
+        b = np.zeros(self.n_cubes)
+        check(lib.cvg_state_get_cube_sums(self._h, _dptr(a), _dptr(b)))
+        return a, b
+
+    def get_bins(self):
+        w = np.zeros((self.n_dim, self.n_intervals))
+        c = np.zeros((self.n_dim, self.n_intervals), dtype=np.uint32)
+        check(lib.cvg_state_get_bins(self._h, _dptr(w), _uptr(c)))
+        return w, c
+
+    def set_bins(self, weights, counts) -> None:
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        c = np.ascontiguousarray(counts, dtype=np.uint32)
+        check(lib.cvg_state_set_bins(self._h, _dptr(w), _uptr(c)))
+
+    def set_iteration(self, it: int) -> None:
+        check(lib.cvg_state_set_iteration(self._h, int(it)))
+
+    def set_theta(self, theta) -> None:
+        t = np.ascontiguousarray(theta, dtype=np.float64)
+        check(lib.cvg_state_set_theta(self._h, _dptr(t), int(t.size)))
+
+    def set_stream(self, hip_stream: int) -> None:
+        check(lib.cvg_state_set_stream(self._h, C.c_void_p(hip_stream)))
+
+    # -- single steps ----------------------------------------------------------
+    def fill(self) -> None:
+        check(lib.cvg_state_fill(self._h))
+
+    def refine(self) -> None:
+        check(lib.cvg_state_refine(self._h))
+
+    def reallocate(self, dh, D: float) -> None:
+        a = np.ascontiguousarray(dh, dtype=np.float64)
+        check(lib.cvg_state_reallocate(self._h, _dptr(a), float(D)))
+
+    def dump(self, it: int, n: int):
+        cube = np.zeros(n, dtype=np.int64)
+        ids = np.zeros((n, self.n_dim), dtype=np.int32)
+        x = np.zeros((n, self.n_dim))
+        jf = np.zeros(n)
+        check(lib.cvg_state_dump(self._h, int(it), int(n), _llptr(cube), _iptr(ids), _dptr(x), _dptr(jf)))
+        return cube, ids, x, jf
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib.cvg_state_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def combine(I: np.ndarray, S2: np.ndarray, skip: int):
+    """Inverse-variance weighted mean of iterations >= skip, its error and chi2/dof
+    (reference cuda_wrapper.cu:758-766 and :869; chi2/dof is an addition)."""
+    res, sig, K = 0.0, 0.0, 0
+    for i in range(skip, len(I)):
+        res += I[i] / S2[i]
+        sig += 1.0 / S2[i]
+        K += 1
+    if K > 0:
+        res /= sig
+    err = 1.0 / math.sqrt(sig) if sig > 0 else math.inf
+    chi2 = sum((I[i] - res) ** 2 / S2[i] for i in range(skip, len(I)))
+    return res, err, (chi2 / (K - 1) if K > 1 else 0.0)
+
+
+def integrate(device_function, n_dim, domain=None, n_evals=1000000, max_it=20, skip=5,
+              batch_size=1048576, n_intervals=1024, n_strat=None, alpha=0.5, beta=0.75,
+              seed=None, multi_gpu=None, multi_cpu=None, gpu_num=-1, cpu_num=-1, gpu_thr=0.1,
+              *, precision="fp64", use_graph=True, run_len=0, options=None):
+    """VEGAS+ integral of ``device_function`` over ``domain`` (default [0,1]^n_dim).
+
+    Mirrors python_wrapper.integrate (python_wrapper.py:9-77).  ``n_evals`` is
+    the total over all iterations (n_evals // max_it per iteration).
+    Returns (result, info) where info holds the reference's keys error, n_strat,
+    n_cubes, n_intervals, tot_nevals, tot_time, avg_it_time, seed, multi_gpu,
+    multi_cpu, gpu_num, omp_threads_num, gpu_thr, plus chi2_dof, iterations,
+    evals_per_s, precision and exact (library integrands).
+    """
+    t0 = time.perf_counter()
+    if seed is None:  # python_wrapper.py:12-13
+        seed = int.from_bytes(os.urandom(4), "big")
+    if n_strat is None:  # python_wrapper.py:15-18
+        n_strat = default_n_strat(n_evals, max_it, n_dim)
+    exact = None
+    if isinstance(device_function, _ints.Integrand):
+        exact = device_function.exact
+        source = device_function.source
+    elif isinstance(device_function, str) and device_function in _ints.LIBRARY:
+        lib_int = _ints.by_name(device_function, n_dim)
+        exact, source = lib_int.exact, lib_int.source
+    elif isinstance(device_function, str):
+        source = device_function
+    else:
+        raise TypeError("device_function must be HIP C++ source defining `integrand`, "
+                        "an Integrand, or a library integrand name")
+    dom = None
+    if domain is not None:  # python_wrapper.py:20-46
+        start = np.array([float(d[0]) for d in domain])
+        rng = np.array([float(d[1]) - float(d[0]) for d in domain])
+        if len(start) != n_dim:
+            raise ValueError("domain must have n_dim intervals")
+        if np.any(start) or not np.all(rng == 1):
+            dom = np.concatenate([start, rng])
+    if multi_gpu is None:  # python_wrapper.py:63-69
+        multi_gpu = n_evals >= 1e10
+    if multi_cpu is None:
+        multi_cpu = (n_evals // max_it) >= 1e8
+    n_eval_it = n_evals // max_it
+    integ = Integrator(source, n_dim, n_eval_it, n_intervals=n_intervals, n_strat=n_strat,
+                       max_it=max_it, skip=skip, alpha=alpha, beta=beta, seed=seed,
+                       precision=precision, run_len=run_len, domain=dom, options=options)
+    try:
+        integ.run(max_it, use_graph=use_graph)
+        r = integ.result()
+        I, S2, ne = integ.history()
+    finally:
+        integ.close()
+    tot_ms = (time.perf_counter() - t0) * 1e3
+    info = {
+        "error": r["error"], "n_strat": n_strat, "n_cubes": r["n_cubes"],
+        "n_intervals": n_intervals, "tot_nevals": r["tot_nevals"], "tot_time": tot_ms,
+        "avg_it_time": r["loop_ms"] / max_it, "seed": seed, "multi_gpu": bool(multi_gpu),
+        "multi_cpu": bool(multi_cpu), "gpu_num": 1, "omp_threads_num": None, "gpu_thr": gpu_thr,
+        "chi2_dof": r["chi2_dof"], "iterations": list(zip(I.tolist(), np.sqrt(S2).tolist())),
+        "evals_per_s": r["tot_nevals"] / (r["loop_ms"] * 1e-3) if r["loop_ms"] > 0 else float("nan"),
+        "precision": precision, "compile_ms": r["compile_ms"], "setup_ms": r["setup_ms"],
+        "exact": exact,
+    }
+    return r["result"], info

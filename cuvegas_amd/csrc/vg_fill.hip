@@ -1,0 +1,282 @@
+// VEGAS+ sampling kernel, compiled at run time by hipRTC together with the
+// user's integrand (replaces the PTX link of cuda_kernel.cu with the numba
+// integrand, reference cuda_wrapper.cu:221-286 and python_wrapper.py:49-59).
+//
+// One launch evaluates every sample of one iteration (reference vegasFill,
+// cuda_kernel.cu:14-76) and produces
+//   * per-cube sums of J f and (J f)^2 in a fixed order, without atomics:
+//       level 0  each lane walks a run of VG_K consecutive samples (the
+//                sample -> cube map comes from the nh prefix sums, so there is
+//                no evals[] array) and sums each cube's samples in order;
+//       level 1  the 64 runs of a window (one wave) combine the pieces of a
+//                cube that crosses runs with a segmented Hillis-Steele scan;
+//       level 2  a cube that crosses windows leaves its window partials in
+//                win_first / win_last; the results kernel folds them;
+//   * per-dimension bin sums of (J f)^2 and hit counts, privatised in LDS and
+//     merged into global memory with one atomic per touched bin per block
+//     (the reference does 2d global atomics per sample, cuda_kernel.cu:60-65).
+//
+// Compile-time shape (hipRTC -D options): VG_NDIM, VG_NBINS, VG_NSTRAT, VG_K,
+// VG_BLOCK, VG_FP32 (0: FP64 sampling and evaluation; 1: FP32 sampling and
+// evaluation with FP64 accumulation), VG_LDS_BINS, VG_LDS_MAP, VG_HAS_DOMAIN.
+#include "vg_philox.h"
+
+#if VG_FP32
+typedef float vg_real;
+#else
+typedef double vg_real;
+#endif
+
+#include "vg_user.h"
+
+namespace vg_detail {
+// The integrand may be written in any of three forms:
+//   vg_real integrand(vg_real* x, const vg_real* theta)   parametrised
+//   vg_real integrand(vg_real* x)                          config.cuh style
+//   int integrand(vg_real* ret, vg_real* x)                extension ABI
+template <typename R>
+__device__ inline auto call(R* x, const R* th, int, int) -> decltype(integrand(x, th)) {
+    return integrand(x, th);
+}
+template <typename R>
+__device__ inline auto call(R* x, const R*, int, long) -> decltype(integrand(x)) {
+    return integrand(x);
+}
+template <typename R>
+__device__ inline R call(R* x, const R*, long, long) {
+    R r;
+    integrand(&r, x);
+    return r;
+}
+}  // namespace vg_detail
+
+constexpr int D = VG_NDIM;
+constexpr int NB = VG_NBINS;
+constexpr unsigned NS = VG_NSTRAT;
+constexpr int K = VG_K;
+constexpr int NG = (D + 3) / 4;
+constexpr int WPB = VG_BLOCK / 64;
+constexpr long long WS = 64LL * K;
+
+struct vg_sample {
+    double jf, jf2;
+    unsigned ids[D];
+    vg_real x[D];
+};
+
+// Evaluate global sample s (cube h) of iteration it.  map is the packed
+// (edge, width) table [D][NB] in the working precision.
+template <typename MapT>
+__device__ inline void vg_eval(vg_sample& o, long long s, int h, unsigned it, unsigned seed,
+                               unsigned stream, const MapT* __restrict__ map,
+                               const double* __restrict__ domain, double domain_vol,
+                               const vg_real* __restrict__ theta) {
+    unsigned words[4 * NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const vg_u4 v = vg_sample_words((unsigned long long)s, it, (unsigned)g, seed, stream);
+        words[4 * g + 0] = v.x;
+        words[4 * g + 1] = v.y;
+        words[4 * g + 2] = v.z;
+        words[4 * g + 3] = v.w;
+    }
+    unsigned tmp = (unsigned)h;
+    vg_real dxv[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        const unsigned q = tmp / NS;
+        const unsigned k = tmp - q * NS;
+        tmp = q;
+        const vg_binpos b = vg_bin_of(words[i], k * (unsigned)NB, (unsigned)NB, NS);
+#if VG_FP32
+        constexpr float inv_ns = 1.0f / (float)NS;
+        const float frac = ((float)b.rem + (float)(unsigned)(b.low >> 9) * 0x1p-24f) * inv_ns;
+        const float2 m = map[i * NB + b.id];
+        o.x[i] = fmaf(m.y, frac, m.x);
+#else
+        constexpr double inv_ns = 1.0 / (double)NS;
+        const double frac = ((double)b.rem + (double)b.low * 0x1p-33) * inv_ns;
+        const double2 m = map[i * NB + b.id];
+        o.x[i] = fma(m.y, frac, m.x);
+#endif
+        dxv[i] = m.y;
+        o.ids[i] = b.id;
+    }
+    vg_real jac = (vg_real)1;
+#pragma unroll
+    for (int i = 0; i < D; ++i) jac = jac * ((vg_real)NB * dxv[i]);
+    vg_real xe[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+#if VG_HAS_DOMAIN
+        xe[i] = o.x[i] * (vg_real)domain[D + i] + (vg_real)domain[i];
+#else
+        xe[i] = o.x[i];
+#endif
+    }
+    vg_real f = vg_detail::call<vg_real>(xe, theta, 0, 0);
+#if VG_HAS_DOMAIN
+    f = (vg_real)domain_vol * f;
+#endif
+    const vg_real jfr = f * jac;
+    o.jf = (double)jfr;
+    o.jf2 = o.jf * o.jf;
+}
+
+#if VG_FP32
+typedef float vg_bin_t;
+typedef float2 vg_map_t;
+#else
+typedef double vg_bin_t;
+typedef double2 vg_map_t;
+#endif
+
+extern "C" __global__ void __launch_bounds__(VG_BLOCK)
+vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
+        const long long* __restrict__ nh_sum,   // [C+1] exclusive prefix of n_h
+        const int* __restrict__ run_cube,       // cube holding the first sample of each run
+        const long long* __restrict__ s_total,  // samples this iteration
+        const int* __restrict__ it_p,           // iteration counter
+        double* __restrict__ JF, double* __restrict__ JF2,
+        double2* __restrict__ win_first, double2* __restrict__ win_last,
+        double* __restrict__ weights, unsigned* __restrict__ counts,
+        const double* __restrict__ domain, double domain_vol,
+        const vg_real* __restrict__ theta, unsigned seed, unsigned stream) {
+#if VG_LDS_BINS
+    __shared__ vg_bin_t lw[D * NB];
+    __shared__ unsigned lc[D * NB];
+    for (int i = threadIdx.x; i < D * NB; i += VG_BLOCK) { lw[i] = (vg_bin_t)0; lc[i] = 0u; }
+#endif
+#if VG_LDS_MAP
+    __shared__ vg_map_t lmap[D * NB];
+    for (int i = threadIdx.x; i < D * NB; i += VG_BLOCK) lmap[i] = gmap[i];
+    const vg_map_t* map = lmap;
+#else
+    const vg_map_t* map = gmap;
+#endif
+    __syncthreads();
+
+    const long long S = *s_total;
+    const unsigned it = (unsigned)*it_p;
+    const long long nwin = (S + WS - 1) / WS;
+    const int lane = threadIdx.x & 63;
+    const long long wave = (long long)blockIdx.x * WPB + (threadIdx.x >> 6);
+    const long long nwaves = (long long)gridDim.x * WPB;
+
+    for (long long w = wave; w < nwin; w += nwaves) {
+        const long long r = w * 64 + lane;
+        const long long rs = r * K;
+        const bool active = rs < S;
+        const long long re = active ? (rs + K < S ? rs + K : S) : rs;
+        int h = 0;
+        long long cstart = 0, cend = 0;
+        if (active) {
+            h = run_cube[r];
+            cstart = nh_sum[h];
+            cend = nh_sum[h + 1];
+        }
+        const long long startF = cstart;
+        const bool sb = active && cstart < rs;
+        bool hasF = false;
+        int hF = 0, npieces = 0;
+        double f1 = 0.0, f2 = 0.0, a1 = 0.0, a2 = 0.0;
+
+        for (int j = 0; j < K; ++j) {
+            const long long s = rs + j;
+            if (s < re) {
+                if (s == cend) {
+                    if (npieces == 0 && sb) {
+                        hasF = true; hF = h; f1 = a1; f2 = a2;
+                    } else {
+                        JF[h] = a1; JF2[h] = a2;
+                    }
+                    ++npieces;
+                    ++h;
+                    cstart = cend;
+                    cend = nh_sum[h + 1];
+                    a1 = 0.0; a2 = 0.0;
+                }
+                vg_sample o;
+                vg_eval(o, s, h, it, seed, stream, map, domain, domain_vol, theta);
+#pragma unroll
+                for (int i = 0; i < D; ++i) {
+#if VG_LDS_BINS
+                    atomicAdd(&lw[i * NB + o.ids[i]], (vg_bin_t)o.jf2);
+                    atomicAdd(&lc[i * NB + o.ids[i]], 1u);
+#else
+                    unsafeAtomicAdd(&weights[i * NB + o.ids[i]], o.jf2);
+                    atomicAdd(&counts[i * NB + o.ids[i]], 1u);
+#endif
+                }
+                a1 = a1 + o.jf;
+                a2 = a2 + o.jf2;
+            }
+        }
+
+        // level 1: segmented scan of each run's last piece across the window
+        const bool cont = cend > re;
+        int key = active ? h : -1;
+        double s1 = active ? a1 : 0.0, s2 = active ? a2 : 0.0;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int kp = __shfl_up(key, off, 64);
+            const double p1 = __shfl_up(s1, off, 64);
+            const double p2 = __shfl_up(s2, off, 64);
+            if (lane >= off && kp == key) { s1 = p1 + s1; s2 = p2 + s2; }
+        }
+        const double q1 = __shfl_up(s1, 1, 64), q2 = __shfl_up(s2, 1, 64);
+        const long long wstart = w * WS;
+        if (active) {
+            if (hasF) {
+                double t1 = f1, t2 = f2;
+                if (lane > 0) { t1 = q1 + t1; t2 = q2 + t2; }
+                if (startF < wstart) win_first[w] = make_double2(t1, t2);
+                else { JF[hF] = t1; JF2[hF] = t2; }
+            }
+            if (!cont) {
+                if (cstart < wstart) win_first[w] = make_double2(s1, s2);
+                else { JF[h] = s1; JF2[h] = s2; }
+            } else if (lane == 63) {
+                win_last[w] = make_double2(s1, s2);
+                if (cstart < wstart) win_first[w] = make_double2(s1, s2);
+            }
+        }
+    }
+
+#if VG_LDS_BINS
+    __syncthreads();
+    for (int i = threadIdx.x; i < D * NB; i += VG_BLOCK) {
+        const unsigned c = lc[i];
+        if (c) {
+            unsafeAtomicAdd(&weights[i], (double)lw[i]);
+            atomicAdd(&counts[i], c);
+        }
+    }
+#endif
+}
+
+// Sample dump: cube, bin ids, mapped point and J f of samples [0, n) of
+// iteration it under the current map and allocation (diagnostics and parity
+// tests of the sampling step).
+extern "C" __global__ void __launch_bounds__(256)
+vg_dump(const vg_map_t* __restrict__ gmap, const long long* __restrict__ nh_sum, int C,
+        long long n, unsigned it, const double* __restrict__ domain, double domain_vol,
+        const vg_real* __restrict__ theta, unsigned seed, unsigned stream,
+        long long* __restrict__ cube, int* __restrict__ ids, double* __restrict__ x,
+        double* __restrict__ jf) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n || s >= nh_sum[C]) return;
+    int lo = 0, hi = C;  // largest h with nh_sum[h] <= s
+    while (hi - lo > 1) {
+        const int mid = lo + (hi - lo) / 2;
+        if (nh_sum[mid] <= s) lo = mid; else hi = mid;
+    }
+    vg_sample o;
+    vg_eval(o, s, lo, it, seed, stream, gmap, domain, domain_vol, theta);
+    cube[s] = lo;
+    for (int i = 0; i < D; ++i) {
+        ids[s * D + i] = (int)o.ids[i];
+        x[s * D + i] = (double)o.x[i];
+    }
+    jf[s] = o.jf;
+}

@@ -1,0 +1,836 @@
+// Host driver and C ABI of the MI355X VEGAS+ integrator.
+//
+// Owns the hipRTC compile of the sampling kernel, the device state of one
+// integral and the device-resident iteration loop.  Mirrors the reference's
+// host driver (vegas_cuda/src/vegas_cuda_extension/cuda_wrapper.cu:290-872)
+// without its per-iteration host synchronisation: one iteration is seven
+// kernels on one stream, optionally captured once in a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/cuvegas.h"
+#include "vg_steps.h"
+
+#include "vg_embed.inc"  // kVgFillSrc, kVgPhiloxSrc (generated from vg_fill.hip, vg_philox.h)
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define VG_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(CVG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+constexpr int kLdsBytes = 160 * 1024;  // gfx950 LDS per workgroup
+constexpr int kFillBlock = 1024;
+constexpr int kDefaultRun = 16;
+
+bool checked_cubes(int ns, int d, long long* C) {
+    long long r = 1;
+    for (int i = 0; i < d; ++i) {
+        r *= ns;
+        if (r > 2147483647LL) return false;
+    }
+    *C = r;
+    return true;
+}
+
+int validate(const cvg_params* p) {
+    if (!p) return fail(CVG_ERR_INVALID, "params is NULL");
+    if (p->n_dim < 1 || p->n_dim > 64) return fail(CVG_ERR_INVALID, "n_dim must be in 1..64");
+    if (p->n_intervals < 2) return fail(CVG_ERR_INVALID, "n_intervals must be >= 2");
+    if (p->n_strat < 1) return fail(CVG_ERR_INVALID, "n_strat must be >= 1");
+    if ((unsigned long long)p->n_strat * (unsigned long long)p->n_intervals >= (1ull << 32))
+        return fail(CVG_ERR_INVALID, "n_strat * n_intervals must be < 2^32");
+    if (p->n_strat >= (1 << 24)) return fail(CVG_ERR_INVALID, "n_strat must be < 2^24");
+    long long C;
+    if (!checked_cubes(p->n_strat, p->n_dim, &C))
+        return fail(CVG_ERR_INVALID, "n_strat^n_dim must be < 2^31");
+    if (p->precision != CVG_FP64 && p->precision != CVG_FP32C)
+        return fail(CVG_ERR_INVALID, "precision must be CVG_FP64 or CVG_FP32C");
+    if (p->run_len < 0 || p->run_len > 4096) return fail(CVG_ERR_INVALID, "run_len must be in 0..4096");
+    if (p->has_domain && !p->domain) return fail(CVG_ERR_INVALID, "has_domain set but domain is NULL");
+    if (p->n_theta < 0) return fail(CVG_ERR_INVALID, "n_theta must be >= 0");
+    return CVG_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Program: the hipRTC-compiled sampling kernel for one (integrand, shape).
+// ---------------------------------------------------------------------------
+struct cvg_program {
+    int d = 0, nbins = 0, nstrat = 0, precision = 0, K = kDefaultRun, has_domain = 0;
+    int block = kFillBlock, lds_bins = 0, lds_map = 0;
+    std::string log;
+    std::vector<char> code;
+    double compile_ms = 0.0;
+    // module loaded on first use, on the device current at that time
+    int device = -1;
+    hipModule_t mod = nullptr;
+    hipFunction_t fill = nullptr, dump = nullptr;
+    int fill_blocks_per_cu = 0;
+
+    int load() {
+        int dev;
+        VG_HIP(hipGetDevice(&dev));
+        if (mod && dev == device) return CVG_OK;
+        if (mod) return fail(CVG_ERR_INVALID, "program was loaded on another device");
+        const double t0 = now_ms();
+        VG_HIP(hipModuleLoadData(&mod, code.data()));
+        VG_HIP(hipModuleGetFunction(&fill, mod, "vg_fill"));
+        VG_HIP(hipModuleGetFunction(&dump, mod, "vg_dump"));
+        VG_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&fill_blocks_per_cu, fill, block, 0));
+        if (fill_blocks_per_cu < 1) fill_blocks_per_cu = 1;
+        device = dev;
+        compile_ms += now_ms() - t0;
+        return CVG_OK;
+    }
+    ~cvg_program() {
+        if (mod) (void)hipModuleUnload(mod);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// State: device buffers of one integral.
+// ---------------------------------------------------------------------------
+template <typename T>
+static int dalloc(T** p, long long count) {
+    if (count < 1) count = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)count);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(CVG_ERR_ALLOC, std::string("hipMalloc of ") + std::to_string(sizeof(T) * count) +
+                                       " bytes: " + hipGetErrorString(e));
+    }
+    return CVG_OK;
+}
+
+struct cvg_state {
+    cvg_program* prog = nullptr;
+    cvg_params p{};
+    std::vector<double> domain_host;
+    long long C = 0, nb = 0, s_cap = 0, run_cap = 0, win_cap = 0;
+    int K = kDefaultRun;
+    double V = 0.0;
+    int nh0 = 2;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = -1;
+    // device buffers
+    double* x_edges = nullptr;
+    double2* map64 = nullptr;
+    float2* map32 = nullptr;
+    double* weights = nullptr;
+    unsigned* counts = nullptr;
+    double* refine_scratch = nullptr;
+    int* nh = nullptr;
+    long long* nh_sum = nullptr;
+    double* dh = nullptr;
+    double* JF = nullptr;
+    double* JF2 = nullptr;
+    int* run_cube = nullptr;
+    double2* win_first = nullptr;
+    double2* win_last = nullptr;
+    double* part = nullptr;
+    long long* bsum = nullptr;
+    long long* boff = nullptr;
+    long long* s_total = nullptr;
+    int* it_p = nullptr;
+    double* dsum = nullptr;
+    int* err = nullptr;
+    double* hist = nullptr;
+    double* domain = nullptr;
+    void* theta = nullptr;
+    int iterations_done = 0;
+    double setup_ms = 0.0, loop_ms = 0.0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // optional per-iteration timing (eager launches): fill and whole iteration
+    int timing = 0;
+    std::vector<hipEvent_t> tev;  // 3 events per iteration: start, after fill, end
+    std::vector<double> fill_ms, iter_ms;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+
+    ~cvg_state() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+        void* bufs[] = {x_edges, map64, map32, weights, counts, refine_scratch, nh, nh_sum, dh,
+                        JF, JF2, run_cube, win_first, win_last, part, bsum, boff, s_total,
+                        it_p, dsum, err, hist, domain, theta};
+        for (void* b : bufs)
+            if (b) (void)hipFree(b);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+
+    const void* map_for_fill() const {
+        return p.precision == CVG_FP32C ? (const void*)map32 : (const void*)map64;
+    }
+
+    int launch_fill() {
+        int ncu = 0;
+        VG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+        const unsigned grid = (unsigned)(ncu * prog->fill_blocks_per_cu);
+        const void* map = map_for_fill();
+        double vol = 1.0;
+        if (p.has_domain)
+            for (int i = 0; i < p.n_dim; ++i) vol *= domain_host[p.n_dim + i];
+        unsigned seed = p.seed, sid = p.stream_id;
+        void* args[] = {(void*)&map,       (void*)&nh_sum,  (void*)&run_cube, (void*)&s_total,
+                        (void*)&it_p,      (void*)&JF,      (void*)&JF2,      (void*)&win_first,
+                        (void*)&win_last,  (void*)&weights, (void*)&counts,   (void*)&domain,
+                        (void*)&vol,       (void*)&theta,   (void*)&seed,     (void*)&sid};
+        VG_HIP(hipModuleLaunchKernel(prog->fill, grid, 1, 1, (unsigned)prog->block, 1, 1, 0, stream,
+                                     args, nullptr));
+        return CVG_OK;
+    }
+
+    int launch_refine() {
+        VG_HIP(vg_launch_refine(stream, p.n_dim, p.n_intervals, p.alpha, x_edges, weights, counts,
+                                refine_scratch, map64, map32));
+        return CVG_OK;
+    }
+
+    int launch_results() {
+        VG_HIP(vg_launch_results(stream, (int)C, V, p.beta, nh, nh_sum, JF, JF2, win_first, win_last,
+                                 64LL * K, dh, part));
+        VG_HIP(vg_launch_finalize(stream, nb, part, s_total, it_p, p.max_it, hist, dsum, err));
+        return CVG_OK;
+    }
+
+    int launch_realloc(int mode, int advance) {
+        VG_HIP(vg_launch_alloc(stream, (int)C, mode, nh0, dh, dsum, p.n_eval_it, nh, bsum));
+        VG_HIP(vg_launch_scan(stream, (int)C, nh, bsum, boff, s_total, s_cap, it_p, advance, K,
+                              run_cap, nh_sum, run_cube, err));
+        return CVG_OK;
+    }
+
+    int launch_realloc_copy();
+
+    int enqueue_iteration(hipEvent_t e_start = nullptr, hipEvent_t e_fill = nullptr,
+                          hipEvent_t e_end = nullptr) {
+        int rc;
+        if (e_start) VG_HIP(hipEventRecord(e_start, stream));
+        if ((rc = launch_fill())) return rc;
+        if (e_fill) VG_HIP(hipEventRecord(e_fill, stream));
+        if ((rc = launch_refine())) return rc;
+        if ((rc = launch_results())) return rc;
+        if ((rc = launch_realloc(1, 1))) return rc;
+        if (e_end) VG_HIP(hipEventRecord(e_end, stream));
+        return CVG_OK;
+    }
+
+    int check_err() {
+        int h_err = 0;
+        VG_HIP(hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, stream));
+        VG_HIP(hipStreamSynchronize(stream));
+        if (h_err & VG_ERR_CAPACITY)
+            return fail(CVG_ERR_CAPACITY, "sample allocation exceeded the state's buffers");
+        if (h_err & VG_ERR_HIST)
+            return fail(CVG_ERR_CAPACITY, "more iterations than max_it were run on this state");
+        return CVG_OK;
+    }
+};
+
+// Layout (nh_sum, run_cube) from the n_h already on the device.
+int cvg_state::launch_realloc_copy() {
+    VG_HIP(vg_launch_layout(stream, (int)C, nh, bsum, boff, s_total, s_cap, it_p, K, run_cap, nh_sum,
+                            run_cube, err));
+    return CVG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* cvg_last_error(void) { return g_err.c_str(); }
+
+int cvg_device_info(int* n_cu, int* clock_khz, char* arch, int arch_len) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(CVG_ERR_NODEVICE, "no HIP device visible");
+    int dev;
+    VG_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    VG_HIP(hipGetDeviceProperties(&prop, dev));
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    if (clock_khz) *clock_khz = prop.clockRate;
+    if (arch && arch_len > 0) {
+        std::snprintf(arch, (size_t)arch_len, "%s", prop.gcnArchName);
+    }
+    return CVG_OK;
+}
+
+int cvg_program_create(const char* integrand_src, const cvg_params* shape, const char* options,
+                       cvg_program** out) {
+    if (!out) return fail(CVG_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!integrand_src) return fail(CVG_ERR_INVALID, "integrand source is NULL");
+    int rc = validate(shape);
+    if (rc) return rc;
+    auto prog = std::make_unique<cvg_program>();
+    prog->d = shape->n_dim;
+    prog->nbins = shape->n_intervals;
+    prog->nstrat = shape->n_strat;
+    prog->precision = shape->precision;
+    prog->K = shape->run_len > 0 ? shape->run_len : kDefaultRun;
+    prog->has_domain = shape->has_domain ? 1 : 0;
+    const long long dn = (long long)prog->d * prog->nbins;
+    const long long bin_bytes = dn * (prog->precision == CVG_FP32C ? 8 : 12);
+    const long long map_bytes = prog->precision == CVG_FP32C ? dn * 8 : 0;
+    prog->lds_bins = bin_bytes <= kLdsBytes;
+    prog->lds_map = prog->precision == CVG_FP32C && prog->lds_bins && bin_bytes + map_bytes <= kLdsBytes;
+
+    const double t0 = now_ms();
+    hiprtcProgram rtc;
+    const char* hdr_src[2] = {kVgPhiloxSrc, integrand_src};
+    const char* hdr_name[2] = {"vg_philox.h", "vg_user.h"};
+    if (hiprtcCreateProgram(&rtc, kVgFillSrc, "vg_fill.hip", 2, hdr_src, hdr_name) != HIPRTC_SUCCESS)
+        return fail(CVG_ERR_COMPILE, "hiprtcCreateProgram failed");
+    std::vector<std::string> opts = {
+        "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-DVG_NDIM=" + std::to_string(prog->d), "-DVG_NBINS=" + std::to_string(prog->nbins),
+        "-DVG_NSTRAT=" + std::to_string(prog->nstrat) + "u", "-DVG_K=" + std::to_string(prog->K),
+        "-DVG_BLOCK=" + std::to_string(prog->block),
+        "-DVG_FP32=" + std::to_string(prog->precision == CVG_FP32C ? 1 : 0),
+        "-DVG_LDS_BINS=" + std::to_string(prog->lds_bins),
+        "-DVG_LDS_MAP=" + std::to_string(prog->lds_map),
+        "-DVG_HAS_DOMAIN=" + std::to_string(prog->has_domain)};
+    if (options) {
+        std::istringstream ss(options);
+        std::string tok;
+        while (ss >> tok) opts.push_back(tok);
+    }
+    std::vector<const char*> copts;
+    for (auto& o : opts) copts.push_back(o.c_str());
+    hiprtcResult cr = hiprtcCompileProgram(rtc, (int)copts.size(), copts.data());
+    size_t log_size = 0;
+    hiprtcGetProgramLogSize(rtc, &log_size);
+    if (log_size > 1) {
+        prog->log.resize(log_size);
+        hiprtcGetProgramLog(rtc, &prog->log[0]);
+        while (!prog->log.empty() && prog->log.back() == '\0') prog->log.pop_back();
+    }
+    if (cr != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&rtc);
+        *out = prog.release();
+        return fail(CVG_ERR_COMPILE, std::string("hipRTC compile failed: ") + hiprtcGetErrorString(cr) +
+                                         "\n" + (*out)->log);
+    }
+    size_t code_size = 0;
+    hiprtcGetCodeSize(rtc, &code_size);
+    prog->code.resize(code_size);
+    hiprtcGetCode(rtc, prog->code.data());
+    hiprtcDestroyProgram(&rtc);
+    prog->compile_ms = now_ms() - t0;
+    *out = prog.release();
+    return CVG_OK;
+}
+
+const char* cvg_program_log(const cvg_program* prog) { return prog ? prog->log.c_str() : ""; }
+
+long long cvg_program_code_size(const cvg_program* prog) {
+    return prog ? (long long)prog->code.size() : 0;
+}
+
+long long cvg_program_get_code(const cvg_program* prog, void* buf, long long cap) {
+    if (!prog || !buf || cap <= 0) return 0;
+    const long long n = (long long)prog->code.size() < cap ? (long long)prog->code.size() : cap;
+    std::memcpy(buf, prog->code.data(), (size_t)n);
+    return n;
+}
+
+void cvg_program_destroy(cvg_program* prog) { delete prog; }
+
+int cvg_state_create(cvg_program* prog, const cvg_params* params, cvg_state** out) {
+    if (!out) return fail(CVG_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!prog || prog->code.empty()) return fail(CVG_ERR_INVALID, "program is NULL or not compiled");
+    int rc = validate(params);
+    if (rc) return rc;
+    const int K = params->run_len > 0 ? params->run_len : kDefaultRun;
+    if (params->n_dim != prog->d || params->n_intervals != prog->nbins || params->n_strat != prog->nstrat ||
+        params->precision != prog->precision || K != prog->K ||
+        (params->has_domain ? 1 : 0) != prog->has_domain)
+        return fail(CVG_ERR_INVALID, "params do not match the program's compiled shape");
+    if (params->max_it < 1) return fail(CVG_ERR_INVALID, "max_it must be >= 1");
+    if (params->n_eval_it < 1) return fail(CVG_ERR_INVALID, "n_eval_it must be >= 1");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(CVG_ERR_NODEVICE, "no HIP device visible");
+    if ((rc = prog->load())) return rc;
+
+    const double t0 = now_ms();
+    auto st = std::make_unique<cvg_state>();
+    st->prog = prog;
+    st->p = *params;
+    st->K = K;
+    VG_HIP(hipGetDevice(&st->device));
+    if (params->has_domain) {
+        st->domain_host.assign(params->domain, params->domain + 2 * params->n_dim);
+        st->p.domain = nullptr;
+    }
+    const int d = params->n_dim, n = params->n_intervals;
+    long long C = 0;
+    checked_cubes(params->n_strat, d, &C);
+    st->C = C;
+    st->nb = (C + 1023) / 1024;
+    st->V = std::pow(1.0 / params->n_strat, d);
+    // Reference: int nh_s = 1.0 / (double) n_cubes * n_eval_it; max(2, .)  (cuda_wrapper.cu:475-480)
+    {
+        const double v = 1.0 / (double)C * (double)params->n_eval_it;
+        int nh0 = v != v ? 0 : (v >= 2147483647.0 ? 2147483647 : (int)v);
+        st->nh0 = nh0 < 2 ? 2 : nh0;
+    }
+    // Sum of n_h <= N + 2C; a margin covers rounding of d_h / D.
+    st->s_cap = params->n_eval_it + 2 * C + 64LL * K + 1024;
+    if ((long long)st->nh0 * C > st->s_cap) st->s_cap = (long long)st->nh0 * C;
+    st->run_cap = (st->s_cap + K - 1) / K + 1;
+    st->win_cap = (st->s_cap + 64LL * K - 1) / (64LL * K) + 1;
+
+    if ((rc = dalloc(&st->x_edges, (long long)d * (n + 1)))) return rc;
+    if ((rc = dalloc(&st->map64, (long long)d * n))) return rc;
+    if ((rc = dalloc(&st->map32, (long long)d * n))) return rc;
+    if ((rc = dalloc(&st->weights, (long long)d * n))) return rc;
+    if ((rc = dalloc(&st->counts, (long long)d * n))) return rc;
+    if ((rc = dalloc(&st->refine_scratch, vg_refine_scratch_doubles(d, n)))) return rc;
+    if ((rc = dalloc(&st->nh, C))) return rc;
+    if ((rc = dalloc(&st->nh_sum, C + 1))) return rc;
+    if ((rc = dalloc(&st->dh, C))) return rc;
+    if ((rc = dalloc(&st->JF, C))) return rc;
+    if ((rc = dalloc(&st->JF2, C))) return rc;
+    if ((rc = dalloc(&st->run_cube, st->run_cap))) return rc;
+    if ((rc = dalloc(&st->win_first, st->win_cap))) return rc;
+    if ((rc = dalloc(&st->win_last, st->win_cap))) return rc;
+    if ((rc = dalloc(&st->part, 3 * st->nb))) return rc;
+    if ((rc = dalloc(&st->bsum, st->nb))) return rc;
+    if ((rc = dalloc(&st->boff, st->nb))) return rc;
+    if ((rc = dalloc(&st->s_total, 1))) return rc;
+    if ((rc = dalloc(&st->it_p, 1))) return rc;
+    if ((rc = dalloc(&st->dsum, 1))) return rc;
+    if ((rc = dalloc(&st->err, 1))) return rc;
+    if ((rc = dalloc(&st->hist, 4LL * params->max_it))) return rc;
+    if ((rc = dalloc(&st->domain, 2LL * d))) return rc;
+    if ((rc = dalloc((char**)&st->theta, 8LL * (params->n_theta > 0 ? params->n_theta : 1)))) return rc;
+
+    VG_HIP(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
+    st->own_stream = true;
+    VG_HIP(hipEventCreate(&st->ev0));
+    VG_HIP(hipEventCreate(&st->ev1));
+    hipStream_t s = st->stream;
+    if (params->has_domain)
+        VG_HIP(hipMemcpyAsync(st->domain, st->domain_host.data(), sizeof(double) * 2 * d,
+                              hipMemcpyHostToDevice, s));
+    VG_HIP(hipMemsetAsync(st->it_p, 0, sizeof(int), s));
+    VG_HIP(hipMemsetAsync(st->err, 0, sizeof(int), s));
+    VG_HIP(hipMemsetAsync(st->theta, 0, 8LL * (params->n_theta > 0 ? params->n_theta : 1), s));
+    VG_HIP(vg_launch_init_map(s, d, n, st->x_edges, st->map64, st->map32, st->weights, st->counts));
+    if ((rc = st->launch_realloc(0, 0))) return rc;
+    VG_HIP(hipStreamSynchronize(s));
+    st->setup_ms = now_ms() - t0;
+    *out = st.release();
+    return CVG_OK;
+}
+
+void cvg_state_destroy(cvg_state* st) {
+    if (st && st->stream) (void)hipStreamSynchronize(st->stream);
+    delete st;
+}
+
+long long cvg_state_n_cubes(const cvg_state* st) { return st ? st->C : 0; }
+
+int cvg_state_set_stream(cvg_state* st, void* hip_stream) {
+    if (!st) return fail(CVG_ERR_INVALID, "state is NULL");
+    VG_HIP(hipStreamSynchronize(st->stream));
+    if (st->graph_exec) { (void)hipGraphExecDestroy(st->graph_exec); st->graph_exec = nullptr; }
+    if (st->graph) { (void)hipGraphDestroy(st->graph); st->graph = nullptr; }
+    if (hip_stream) {
+        if (st->own_stream) VG_HIP(hipStreamDestroy(st->stream));
+        st->stream = (hipStream_t)hip_stream;
+        st->own_stream = false;
+    } else if (!st->own_stream) {
+        VG_HIP(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
+        st->own_stream = true;
+    }
+    return CVG_OK;
+}
+
+int cvg_state_set_theta(cvg_state* st, const double* theta, int n_theta) {
+    if (!st) return fail(CVG_ERR_INVALID, "state is NULL");
+    if (n_theta != st->p.n_theta || (n_theta > 0 && !theta))
+        return fail(CVG_ERR_INVALID, "theta size does not match n_theta");
+    if (n_theta == 0) return CVG_OK;
+    if (st->p.precision == CVG_FP32C) {
+        std::vector<float> f(theta, theta + n_theta);
+        VG_HIP(hipMemcpyAsync(st->theta, f.data(), sizeof(float) * n_theta, hipMemcpyHostToDevice, st->stream));
+        VG_HIP(hipStreamSynchronize(st->stream));
+    } else {
+        VG_HIP(hipMemcpyAsync(st->theta, theta, sizeof(double) * n_theta, hipMemcpyHostToDevice, st->stream));
+        VG_HIP(hipStreamSynchronize(st->stream));
+    }
+    return CVG_OK;
+}
+
+int cvg_state_run(cvg_state* st, int n_iter, int use_graph) {
+    if (!st) return fail(CVG_ERR_INVALID, "state is NULL");
+    if (n_iter < 0) return fail(CVG_ERR_INVALID, "n_iter must be >= 0");
+    if (st->iterations_done + n_iter > st->p.max_it)
+        return fail(CVG_ERR_CAPACITY, "more iterations than max_it requested");
+    if (n_iter == 0) return CVG_OK;
+    int rc;
+    if (use_graph && !st->graph_exec) {
+        VG_HIP(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal));
+        rc = st->enqueue_iteration();
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(st->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ce != hipSuccess) return fail(CVG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        st->graph = g;
+        VG_HIP(hipGraphInstantiate(&st->graph_exec, st->graph, nullptr, nullptr, 0));
+    }
+    const bool timed = st->timing && !use_graph;
+    if (timed) {
+        while ((int)st->tev.size() < 3 * n_iter) {
+            hipEvent_t e;
+            VG_HIP(hipEventCreate(&e));
+            st->tev.push_back(e);
+        }
+    }
+    VG_HIP(hipEventRecord(st->ev0, st->stream));
+    for (int i = 0; i < n_iter; ++i) {
+        if (use_graph) {
+            VG_HIP(hipGraphLaunch(st->graph_exec, st->stream));
+        } else if (timed) {
+            if ((rc = st->enqueue_iteration(st->tev[3 * i], st->tev[3 * i + 1], st->tev[3 * i + 2])))
+                return rc;
+        } else if ((rc = st->enqueue_iteration())) {
+            return rc;
+        }
+    }
+    VG_HIP(hipEventRecord(st->ev1, st->stream));
+    VG_HIP(hipEventSynchronize(st->ev1));
+    float ms = 0.0f;
+    VG_HIP(hipEventElapsedTime(&ms, st->ev0, st->ev1));
+    st->loop_ms += ms;
+    st->iterations_done += n_iter;
+    if (timed) {
+        for (int i = 0; i < n_iter; ++i) {
+            float a = 0.0f, b = 0.0f;
+            VG_HIP(hipEventElapsedTime(&a, st->tev[3 * i], st->tev[3 * i + 1]));
+            VG_HIP(hipEventElapsedTime(&b, st->tev[3 * i], st->tev[3 * i + 2]));
+            st->fill_ms.push_back(a);
+            st->iter_ms.push_back(b);
+        }
+    }
+    return st->check_err();
+}
+
+int cvg_state_set_timing(cvg_state* st, int enable) {
+    if (!st) return fail(CVG_ERR_INVALID, "state is NULL");
+    st->timing = enable ? 1 : 0;
+    return CVG_OK;
+}
+
+int cvg_state_get_timing(cvg_state* st, double* fill_ms, double* iter_ms, int cap) {
+    if (!st) return fail(CVG_ERR_INVALID, "state is NULL");
+    const int n = (int)st->fill_ms.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        if (fill_ms) fill_ms[i] = st->fill_ms[i];
+        if (iter_ms) iter_ms[i] = st->iter_ms[i];
+    }
+    return n;
+}
+
+int cvg_state_history(cvg_state* st, double* I, double* sigma2, long long* nevals, int cap) {
+    if (!st) return fail(CVG_ERR_INVALID, "state is NULL");
+    const int n = st->iterations_done;
+    std::vector<double> h(4 * (size_t)(n > 0 ? n : 1));
+    if (n > 0) {
+        VG_HIP(hipMemcpyAsync(h.data(), st->hist, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, st->stream));
+        VG_HIP(hipStreamSynchronize(st->stream));
+    }
+    for (int i = 0; i < n && i < cap; ++i) {
+        if (I) I[i] = h[4 * i + 0];
+        if (sigma2) sigma2[i] = h[4 * i + 1];
+        if (nevals) nevals[i] = (long long)h[4 * i + 3];
+    }
+    return n;
+}
+
+int cvg_state_result(cvg_state* st, cvg_result* out) {
+    if (!st || !out) return fail(CVG_ERR_INVALID, "state or out is NULL");
+    const int n = st->iterations_done;
+    std::vector<double> I(n > 0 ? n : 1), S2(n > 0 ? n : 1);
+    std::vector<long long> ne(n > 0 ? n : 1);
+    int rc = cvg_state_history(st, I.data(), S2.data(), ne.data(), n);
+    if (rc < 0) return rc;
+    // Reference combine (cuda_wrapper.cu:758-766, 869).
+    double res = 0.0, sig = 0.0;
+    int Kc = 0;
+    for (int i = st->p.skip; i < n; ++i) {
+        res += I[i] / S2[i];
+        sig += 1.0 / S2[i];
+        ++Kc;
+    }
+    if (Kc > 0) res /= sig;
+    double chi2 = 0.0;
+    for (int i = st->p.skip; i < n; ++i) {
+        const double dd = I[i] - res;
+        chi2 += dd * dd / S2[i];
+    }
+    long long tot = 0;
+    for (int i = 0; i < n; ++i) tot += ne[i];
+    out->result = res;
+    out->error = 1.0 / std::sqrt(sig);
+    out->chi2_dof = Kc > 1 ? chi2 / (Kc - 1) : 0.0;
+    out->tot_nevals = tot;
+    out->loop_ms = st->loop_ms;
+    out->compile_ms = st->prog->compile_ms;
+    out->setup_ms = st->setup_ms;
+    out->iterations = n;
+    out->n_cubes = (int)st->C;
+    out->n_strat = st->p.n_strat;
+    out->n_intervals = st->p.n_intervals;
+    return CVG_OK;
+}
+
+static int copy_d2h(cvg_state* st, void* dst, const void* src, size_t bytes) {
+    VG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st->stream));
+    VG_HIP(hipStreamSynchronize(st->stream));
+    return CVG_OK;
+}
+static int copy_h2d(cvg_state* st, void* dst, const void* src, size_t bytes) {
+    VG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st->stream));
+    VG_HIP(hipStreamSynchronize(st->stream));
+    return CVG_OK;
+}
+
+int cvg_state_get_map(cvg_state* st, double* x_edges) {
+    if (!st || !x_edges) return fail(CVG_ERR_INVALID, "NULL argument");
+    return copy_d2h(st, x_edges, st->x_edges, sizeof(double) * st->p.n_dim * (st->p.n_intervals + 1));
+}
+
+int cvg_state_set_map(cvg_state* st, const double* x_edges) {
+    if (!st || !x_edges) return fail(CVG_ERR_INVALID, "NULL argument");
+    const int d = st->p.n_dim, n = st->p.n_intervals;
+    for (int i = 0; i < d; ++i) {
+        const double* x = x_edges + (size_t)i * (n + 1);
+        for (int k = 0; k < n; ++k)
+            if (!(x[k + 1] > x[k])) return fail(CVG_ERR_INVALID, "map edges must be strictly increasing");
+    }
+    int rc = copy_h2d(st, st->x_edges, x_edges, sizeof(double) * d * (n + 1));
+    if (rc) return rc;
+    VG_HIP(vg_launch_tables(st->stream, d, n, st->x_edges, st->map64, st->map32));
+    VG_HIP(hipStreamSynchronize(st->stream));
+    return CVG_OK;
+}
+
+int cvg_state_get_nh(cvg_state* st, int* nh) {
+    if (!st || !nh) return fail(CVG_ERR_INVALID, "NULL argument");
+    return copy_d2h(st, nh, st->nh, sizeof(int) * st->C);
+}
+
+int cvg_state_set_nh(cvg_state* st, const int* nh) {
+    if (!st || !nh) return fail(CVG_ERR_INVALID, "NULL argument");
+    long long tot = 0;
+    for (long long h = 0; h < st->C; ++h) {
+        if (nh[h] < 1) return fail(CVG_ERR_INVALID, "n_h must be >= 1");
+        tot += nh[h];
+    }
+    if (tot > st->s_cap) return fail(CVG_ERR_CAPACITY, "sum of n_h exceeds the state's sample buffers");
+    int rc = copy_h2d(st, st->nh, nh, sizeof(int) * st->C);
+    if (rc) return rc;
+    if ((rc = st->launch_realloc_copy())) return rc;
+    VG_HIP(hipStreamSynchronize(st->stream));
+    return st->check_err();
+}
+
+int cvg_state_get_dh(cvg_state* st, double* dh) {
+    if (!st || !dh) return fail(CVG_ERR_INVALID, "NULL argument");
+    return copy_d2h(st, dh, st->dh, sizeof(double) * st->C);
+}
+
+int cvg_state_get_cube_sums(cvg_state* st, double* jf, double* jf2) {
+    if (!st || !jf || !jf2) return fail(CVG_ERR_INVALID, "NULL argument");
+    int rc = copy_d2h(st, jf, st->JF, sizeof(double) * st->C);
+    if (rc) return rc;
+    return copy_d2h(st, jf2, st->JF2, sizeof(double) * st->C);
+}
+
+int cvg_state_get_bins(cvg_state* st, double* weights, unsigned int* counts) {
+    if (!st || !weights || !counts) return fail(CVG_ERR_INVALID, "NULL argument");
+    const size_t dn = (size_t)st->p.n_dim * st->p.n_intervals;
+    int rc = copy_d2h(st, weights, st->weights, sizeof(double) * dn);
+    if (rc) return rc;
+    return copy_d2h(st, counts, st->counts, sizeof(unsigned) * dn);
+}
+
+int cvg_state_set_bins(cvg_state* st, const double* weights, const unsigned int* counts) {
+    if (!st || !weights || !counts) return fail(CVG_ERR_INVALID, "NULL argument");
+    const size_t dn = (size_t)st->p.n_dim * st->p.n_intervals;
+    int rc = copy_h2d(st, st->weights, weights, sizeof(double) * dn);
+    if (rc) return rc;
+    return copy_h2d(st, st->counts, counts, sizeof(unsigned) * dn);
+}
+
+int cvg_state_set_iteration(cvg_state* st, int it) {
+    if (!st || it < 0) return fail(CVG_ERR_INVALID, "bad argument");
+    return copy_h2d(st, st->it_p, &it, sizeof(int));
+}
+
+int cvg_state_fill(cvg_state* st) {
+    if (!st) return fail(CVG_ERR_INVALID, "state is NULL");
+    int rc = st->launch_fill();
+    if (rc) return rc;
+    VG_HIP(hipStreamSynchronize(st->stream));
+    return CVG_OK;
+}
+
+int cvg_state_refine(cvg_state* st) {
+    if (!st) return fail(CVG_ERR_INVALID, "state is NULL");
+    int rc = st->launch_refine();
+    if (rc) return rc;
+    VG_HIP(hipStreamSynchronize(st->stream));
+    return CVG_OK;
+}
+
+int cvg_state_reallocate(cvg_state* st, const double* dh, double D) {
+    if (!st || !dh) return fail(CVG_ERR_INVALID, "NULL argument");
+    int rc = copy_h2d(st, st->dh, dh, sizeof(double) * st->C);
+    if (rc) return rc;
+    if ((rc = copy_h2d(st, st->dsum, &D, sizeof(double)))) return rc;
+    if ((rc = st->launch_realloc(1, 0))) return rc;
+    VG_HIP(hipStreamSynchronize(st->stream));
+    return st->check_err();
+}
+
+int cvg_state_dump(cvg_state* st, int it, long long n, long long* cube, int* ids, double* x, double* jf) {
+    if (!st || !cube || !ids || !x || !jf || n < 0 || it < 0) return fail(CVG_ERR_INVALID, "bad argument");
+    if (n == 0) return CVG_OK;
+    const int d = st->p.n_dim;
+    long long* d_cube = nullptr;
+    int* d_ids = nullptr;
+    double *d_x = nullptr, *d_jf = nullptr;
+    int rc;
+    if ((rc = dalloc(&d_cube, n)) || (rc = dalloc(&d_ids, n * d)) || (rc = dalloc(&d_x, n * d)) ||
+        (rc = dalloc(&d_jf, n))) {
+        (void)hipFree(d_cube); (void)hipFree(d_ids); (void)hipFree(d_x); (void)hipFree(d_jf);
+        return rc;
+    }
+    const void* map = st->map_for_fill();
+    int C = (int)st->C;
+    unsigned itu = (unsigned)it, seed = st->p.seed, sid = st->p.stream_id;
+    double vol = 1.0;
+    if (st->p.has_domain)
+        for (int i = 0; i < d; ++i) vol *= st->domain_host[d + i];
+    void* args[] = {(void*)&map,   (void*)&st->nh_sum, (void*)&C,     (void*)&n,    (void*)&itu,
+                    (void*)&st->domain, (void*)&vol,   (void*)&st->theta, (void*)&seed, (void*)&sid,
+                    (void*)&d_cube, (void*)&d_ids,     (void*)&d_x,   (void*)&d_jf};
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    hipError_t e = hipModuleLaunchKernel(st->prog->dump, grid, 1, 1, 256, 1, 1, 0, st->stream, args, nullptr);
+    if (e == hipSuccess) e = hipMemcpyAsync(cube, d_cube, sizeof(long long) * n, hipMemcpyDeviceToHost, st->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ids, d_ids, sizeof(int) * n * d, hipMemcpyDeviceToHost, st->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(x, d_x, sizeof(double) * n * d, hipMemcpyDeviceToHost, st->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(jf, d_jf, sizeof(double) * n, hipMemcpyDeviceToHost, st->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+    (void)hipFree(d_cube); (void)hipFree(d_ids); (void)hipFree(d_x); (void)hipFree(d_jf);
+    if (e != hipSuccess) return fail(CVG_ERR_HIP, std::string("dump: ") + hipGetErrorString(e));
+    return CVG_OK;
+}
+
+int cvg_integrate_cuda(long long tot_eval, int max_it, int skip, long long max_batch_size,
+                       int n_intervals, int n_strat, int n_dim, double alpha, double beta,
+                       unsigned int seed, int multi_gpu, int multi_cpu, int gpu_num, int cpu_num,
+                       double gpu_thr, const char* integrand_src, const char* options,
+                       cvg_ref_result* out) {
+    (void)max_batch_size;
+    (void)cpu_num;
+    (void)gpu_num;
+    const double t0 = now_ms();
+    if (!out) return fail(CVG_ERR_INVALID, "out is NULL");
+    if (max_it < 1) return fail(CVG_ERR_INVALID, "max_it must be >= 1");
+    cvg_params p{};
+    p.n_dim = n_dim;
+    p.n_intervals = n_intervals;
+    p.n_strat = n_strat;
+    p.max_it = max_it;
+    p.skip = skip;
+    p.n_eval_it = tot_eval / max_it;  // cuda_wrapper.cu:331
+    p.alpha = alpha;
+    p.beta = beta;
+    p.seed = seed;
+    p.precision = CVG_FP64;
+    std::string rtc_opts;
+    if (options) {
+        std::istringstream ss(options);
+        std::string tok;
+        while (ss >> tok) {
+            if (tok == "fp32") p.precision = CVG_FP32C;
+            else if (tok == "fp64") p.precision = CVG_FP64;
+            else rtc_opts += tok + " ";
+        }
+    }
+    cvg_program* prog = nullptr;
+    int rc = cvg_program_create(integrand_src, &p, rtc_opts.c_str(), &prog);
+    if (rc) {
+        cvg_program_destroy(prog);
+        return rc;
+    }
+    cvg_state* st = nullptr;
+    rc = cvg_state_create(prog, &p, &st);
+    if (!rc) rc = cvg_state_run(st, max_it, 1);
+    cvg_result r{};
+    if (!rc) rc = cvg_state_result(st, &r);
+    cvg_state_destroy(st);
+    cvg_program_destroy(prog);
+    if (rc) return rc;
+    out->res = r.result;
+    out->err = r.error;
+    out->n_strat = n_strat;
+    out->n_cubes = r.n_cubes;
+    out->n_intervals = n_intervals;
+    out->tot_nevals = r.tot_nevals;
+    out->tot_time = now_ms() - t0;
+    out->avg_it_time = r.loop_ms / max_it;
+    out->seed = seed;
+    out->multi_gpu = multi_gpu;
+    out->multi_cpu = multi_cpu;
+    out->gpu_num = 1;
+    out->omp_threads_num = 0;
+    out->gpu_thr = gpu_thr;
+    return CVG_OK;
+}
+
+}  // extern "C"

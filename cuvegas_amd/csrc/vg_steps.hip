@@ -1,0 +1,433 @@
+// Between-fill steps of the VEGAS+ iteration, compiled ahead of time for gfx950.
+//
+// Every step runs on the device so an iteration needs no host round trip
+// (the reference synchronises the host 6+ times per iteration and launches
+// 3d+15 kernels / CUB calls, cuda_wrapper.cu:575-773).  Reductions use one
+// fixed order -- 1024-wide strided pairwise trees, repeated over partials --
+// so the stratification weights D = sum d_h and the integer allocation n_h are
+// reproducible bit for bit.
+#include <hip/hip_runtime.h>
+#include "vg_math.h"
+#include "vg_steps.h"
+
+#define VG_TREE 1024
+
+// In-place strided pairwise tree over t[0..1023] (block of VG_TREE threads).
+__device__ inline void vg_tree1024(double* t) {
+    for (int stride = VG_TREE / 2; stride >= 1; stride >>= 1) {
+        __syncthreads();
+        if ((int)threadIdx.x < stride) t[threadIdx.x] = t[threadIdx.x] + t[threadIdx.x + stride];
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Per-cube results (reference `results`, cuda_wrapper.cu:26-49).  Also folds
+// the window partials of cubes that crossed sampling windows and writes the
+// complete per-cube sums back to JF / JF2.  part[3b+q]: block partial of
+// q = 0: sum I_h, 1: sum sigma_h^2 / n_h, 2: sum d_h.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(VG_TREE)
+vg_results_kernel(int C, double V, double beta, const int* __restrict__ nh,
+                  const long long* __restrict__ nh_sum, double* __restrict__ JF,
+                  double* __restrict__ JF2, const double2* __restrict__ win_first,
+                  const double2* __restrict__ win_last, long long WS, double* __restrict__ dh,
+                  double* __restrict__ part) {
+    __shared__ double t0[VG_TREE], t1[VG_TREE], t2[VG_TREE];
+    const long long h = (long long)blockIdx.x * VG_TREE + threadIdx.x;
+    double sr = 0.0, ss = 0.0, dd = 0.0;
+    if (h < C) {
+        const int neval = nh[h];
+        const long long st = nh_sum[h], en = nh_sum[h + 1];
+        const long long wa = st / WS, wb = (en - 1) / WS;
+        double j1, j2;
+        if (wa == wb) {
+            j1 = JF[h];
+            j2 = JF2[h];
+        } else {
+            double2 v = win_last[wa];
+            j1 = v.x;
+            j2 = v.y;
+            for (long long w = wa + 1; w <= wb; ++w) {
+                v = win_first[w];
+                j1 = j1 + v.x;
+                j2 = j2 + v.y;
+            }
+            JF[h] = j1;
+            JF2[h] = j2;
+        }
+        const double Ih = j1 / neval * V;
+        const double Sig2 = (j2 / neval * V * V) - (Ih * Ih);
+        sr = Ih;
+        ss = Sig2 / neval;
+        const double u = V * j1 / neval;
+        double t = V * V * j2 / neval - u * u;
+        if (t < 0) t = 0;
+        dd = vg_pow(t, beta);
+        dh[h] = dd;
+    }
+    t0[threadIdx.x] = sr;
+    t1[threadIdx.x] = ss;
+    t2[threadIdx.x] = dd;
+    for (int stride = VG_TREE / 2; stride >= 1; stride >>= 1) {
+        __syncthreads();
+        if ((int)threadIdx.x < stride) {
+            t0[threadIdx.x] = t0[threadIdx.x] + t0[threadIdx.x + stride];
+            t1[threadIdx.x] = t1[threadIdx.x] + t1[threadIdx.x + stride];
+            t2[threadIdx.x] = t2[threadIdx.x] + t2[threadIdx.x + stride];
+        }
+    }
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x + 0] = t0[0];
+        part[3 * blockIdx.x + 1] = t1[0];
+        part[3 * blockIdx.x + 2] = t2[0];
+    }
+}
+
+// Remaining tree levels over the block partials, then the iteration record:
+// hist[4*it + {0,1,2,3}] = {I_k, sigma_k^2, D, samples}; *dsum = D.
+__global__ void __launch_bounds__(VG_TREE)
+vg_finalize_kernel(long long nparts, double* __restrict__ part, const long long* __restrict__ s_total,
+                   const int* __restrict__ it_p, int hist_cap, double* __restrict__ hist,
+                   double* __restrict__ dsum, int* __restrict__ err) {
+    __shared__ double t[VG_TREE];
+    long long cur = nparts;
+    while (cur > 1) {
+        const long long nch = (cur + VG_TREE - 1) / VG_TREE;
+        for (long long c = 0; c < nch; ++c) {
+            for (int q = 0; q < 3; ++q) {
+                const long long j = c * VG_TREE + threadIdx.x;
+                __syncthreads();
+                t[threadIdx.x] = j < cur ? part[3 * j + q] : 0.0;
+                vg_tree1024(t);
+                if (threadIdx.x == 0) part[3 * c + q] = t[0];
+            }
+        }
+        cur = nch;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int it = *it_p;
+        *dsum = part[2];
+        if (it < hist_cap) {
+            hist[4 * it + 0] = part[0];
+            hist[4 * it + 1] = part[1];
+            hist[4 * it + 2] = part[2];
+            hist[4 * it + 3] = (double)*s_total;
+        } else {
+            atomicOr(err, VG_ERR_HIST);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reallocation (reference `updateNh`, cuda_wrapper.cu:118-132):
+//   n_h = max(2, (int)(d_h / D * N))        (mode 1)
+//   n_h = nh_const                          (mode 0, initial allocation)
+// plus the per-block sums feeding the prefix sum.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(VG_TREE)
+vg_alloc_kernel(int C, int mode, int nh_const, const double* __restrict__ dh,
+                const double* __restrict__ dsum, long long N, int* __restrict__ nh,
+                long long* __restrict__ bsum) {
+    __shared__ long long t[VG_TREE];
+    const long long h = (long long)blockIdx.x * VG_TREE + threadIdx.x;
+    long long v = 0;
+    if (h < C) {
+        int n;
+        if (mode == 0) {
+            n = nh_const;
+        } else {
+            const double f = dh[h] / *dsum;
+            n = vg_sat_int(f * (double)N);
+            n = n < 2 ? 2 : n;
+        }
+        nh[h] = n;
+        v = n;
+    }
+    t[threadIdx.x] = v;
+    for (int stride = VG_TREE / 2; stride >= 1; stride >>= 1) {
+        __syncthreads();
+        if ((int)threadIdx.x < stride) t[threadIdx.x] += t[threadIdx.x + stride];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = t[0];
+}
+
+// Exclusive scan of the block sums (one block); total -> *s_total, capped at
+// s_cap (error flag set if the allocation exceeds the sample buffers).
+// Optionally advances the iteration counter (end of an iteration).
+__global__ void __launch_bounds__(VG_TREE)
+vg_scan_blocks_kernel(long long nb, const long long* __restrict__ bsum, long long* __restrict__ boff,
+                      long long* __restrict__ s_total, long long s_cap, int* __restrict__ it_p,
+                      int advance, int* __restrict__ err) {
+    __shared__ long long t[VG_TREE];
+    const long long per = (nb + VG_TREE - 1) / VG_TREE;
+    const long long b0 = threadIdx.x * per;
+    long long local = 0;
+    for (long long b = b0; b < b0 + per && b < nb; ++b) local += bsum[b];
+    t[threadIdx.x] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long acc = 0;
+        for (int i = 0; i < VG_TREE; ++i) { const long long x = t[i]; t[i] = acc; acc += x; }
+        if (acc > s_cap) { atomicOr(err, VG_ERR_CAPACITY); acc = s_cap; }
+        *s_total = acc;
+        if (advance) *it_p = *it_p + 1;
+    }
+    __syncthreads();
+    long long acc = t[threadIdx.x];
+    for (long long b = b0; b < b0 + per && b < nb; ++b) { boff[b] = acc; acc += bsum[b]; }
+}
+
+// Per-block local scan of n_h -> nh_sum (exclusive, nh_sum[C] = total), and
+// the cube holding the first sample of every run of K samples.
+__global__ void __launch_bounds__(VG_TREE)
+vg_scan_cubes_kernel(int C, const int* __restrict__ nh, const long long* __restrict__ boff,
+                     const long long* __restrict__ s_total, int K, long long run_cap,
+                     long long* __restrict__ nh_sum, int* __restrict__ run_cube) {
+    __shared__ long long t[VG_TREE + 1];
+    const long long base = (long long)blockIdx.x * VG_TREE;
+    const long long h = base + threadIdx.x;
+    const int cnt = (int)((C - base) < VG_TREE ? (C - base) : VG_TREE);
+    t[threadIdx.x] = h < C ? nh[h] : 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long acc = boff[blockIdx.x];
+        for (int i = 0; i < VG_TREE; ++i) { const long long x = t[i]; t[i] = acc; acc += x; }
+        t[VG_TREE] = acc;
+    }
+    __syncthreads();
+    const long long S = *s_total;
+    if (h < C) nh_sum[h] = t[threadIdx.x] < S ? t[threadIdx.x] : S;
+    if (h == C - 1) nh_sum[C] = S;
+    // runs whose first sample lies in this block's cubes
+    const long long lo = t[0], hi = t[cnt] < S ? t[cnt] : S;
+    const long long r0 = (lo + K - 1) / K, r1 = (hi + K - 1) / K;
+    for (long long r = r0 + threadIdx.x; r < r1 && r < run_cap; r += VG_TREE) {
+        const long long s = r * K;
+        int a = 0, b = cnt;  // largest a with t[a] <= s
+        while (b - a > 1) {
+            const int m = (a + b) >> 1;
+            if (t[m] <= s) a = m; else b = m;
+        }
+        run_cube[r] = (int)(base + a);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Map: uniform initialisation (cuda_wrapper.cu:464-473) and refinement
+// (normalizeWeights / smoothWeights / setMap / updateXEdges / updateDxEdges,
+// cuda_wrapper.cu:51-90,134-191 and the loop at :719-746).  One block per
+// dimension.  Also rebuilds the packed sampling tables and clears the bin
+// accumulators for the next fill.
+// ---------------------------------------------------------------------------
+
+__device__ void vg_write_tables(int n, const double* __restrict__ x, double2* __restrict__ map64,
+                                float2* __restrict__ map32) {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const double a = x[k], b = x[k + 1];
+        map64[k] = make_double2(a, b - a);
+        const float fa = (float)a, fb = (float)b;
+        map32[k] = make_float2(fa, fb - fa);
+    }
+}
+
+__global__ void __launch_bounds__(VG_TREE)
+vg_init_map_kernel(int n, double* __restrict__ x_edges, double2* __restrict__ map64,
+                   float2* __restrict__ map32, double* __restrict__ weights,
+                   unsigned* __restrict__ counts) {
+    const int dim = blockIdx.x;
+    double* x = x_edges + (long long)dim * (n + 1);
+    const double step = 1.0 / n;
+    for (int j = threadIdx.x; j <= n; j += blockDim.x) x[j] = j * step;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        weights[(long long)dim * n + k] = 0.0;
+        counts[(long long)dim * n + k] = 0u;
+    }
+    __syncthreads();
+    vg_write_tables(n, x, map64 + (long long)dim * n, map32 + (long long)dim * n);
+}
+
+// Strided-pairwise tree sum of a[0..m) (global), result broadcast; the same
+// order as the per-cube reductions.  part: scratch of (m+1023)/1024 entries.
+__device__ double vg_block_tree_sum(const double* __restrict__ a, int m, double* __restrict__ part,
+                                    double* t) {
+    int cur = m;
+    const double* src = a;
+    while (true) {
+        const int nch = (cur + VG_TREE - 1) / VG_TREE;
+        for (int c = 0; c < nch; ++c) {
+            const int j = c * VG_TREE + threadIdx.x;
+            __syncthreads();
+            t[threadIdx.x] = j < cur ? src[j] : 0.0;
+            vg_tree1024(t);
+            if (threadIdx.x == 0) part[c] = t[0];
+        }
+        __syncthreads();
+        if (nch == 1) break;
+        cur = nch;
+        src = part;
+    }
+    __syncthreads();
+    const double r = part[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(VG_TREE)
+vg_refine_kernel(int n, double alpha, double* __restrict__ x_edges, double* __restrict__ weights,
+                 unsigned* __restrict__ counts, double* __restrict__ scratch,
+                 double2* __restrict__ map64, float2* __restrict__ map32) {
+    __shared__ double t[VG_TREE];
+    const int dim = blockIdx.x;
+    const int E = n + 1;
+    double* x = x_edges + (long long)dim * E;
+    double* wb = weights + (long long)dim * n;
+    unsigned* cb = counts + (long long)dim * n;
+    // scratch per dim: w[n], d[n], cum[n], xo[n+1], part[n/1024 + 2]
+    double* base = scratch + (long long)dim * (4LL * n + 1 + n / VG_TREE + 2);
+    double* w = base;
+    double* d = w + n;
+    double* cum = d + n;
+    double* xo = cum + n;
+    double* part = xo + E;
+
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const unsigned c = cb[k];
+        w[k] = c != 0u ? wb[k] / (double)c : wb[k];
+        wb[k] = 0.0;
+        cb[k] = 0u;
+    }
+    for (int j = threadIdx.x; j < E; j += blockDim.x) xo[j] = x[j];
+    __syncthreads();
+    const double dsum = vg_block_tree_sum(w, n, part, t);
+    if (dsum > 0.0) {
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            double v;
+            if (k == 0) v = (7.0 * w[k] + w[k + 1]) / (8.0 * dsum);
+            else if (k == n - 1) v = (w[k - 1] + 7.0 * w[k]) / (8.0 * dsum);
+            else v = (w[k - 1] + 6.0 * w[k] + w[k + 1]) / (8.0 * dsum);
+            if (v != 0) v = vg_pow((v - 1.0) / vg_log(v), alpha);
+            d[k] = v;
+        }
+        __syncthreads();
+        const double S = vg_block_tree_sum(d, n, part, t);
+        if (threadIdx.x == 0) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) { acc = acc + d[k]; cum[k] = acc; }
+        }
+        __syncthreads();
+        const double delta = S / n;
+        for (int j = threadIdx.x; j < n - 1; j += blockDim.x) {
+            const int target = j + 1;
+            int lo = 0, hi = n;  // first k with (int)(cum_k / delta) >= target
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (vg_sat_int(cum[mid] / delta) >= target) hi = mid; else lo = mid + 1;
+            }
+            const int k = lo < n ? lo : n - 1;
+            const double a = target * delta - (k > 0 ? cum[k - 1] : 0.0);
+            x[j + 1] = xo[k] + a / d[k] * (xo[k + 1] - xo[k]);
+        }
+    }
+    __syncthreads();
+    vg_write_tables(n, x, map64 + (long long)dim * n, map32 + (long long)dim * n);
+}
+
+long long vg_refine_scratch_doubles(int d, int n) {
+    return (long long)d * (4LL * n + 1 + n / VG_TREE + 2);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch helpers (all on one stream, no synchronisation).
+// ---------------------------------------------------------------------------
+
+hipError_t vg_launch_results(hipStream_t st, int C, double V, double beta, const int* nh,
+                             const long long* nh_sum, double* JF, double* JF2,
+                             const double2* win_first, const double2* win_last, long long WS,
+                             double* dh, double* part) {
+    const unsigned nb = (unsigned)((C + VG_TREE - 1) / VG_TREE);
+    hipLaunchKernelGGL(vg_results_kernel, dim3(nb), dim3(VG_TREE), 0, st, C, V, beta, nh, nh_sum,
+                       JF, JF2, win_first, win_last, WS, dh, part);
+    return hipGetLastError();
+}
+
+hipError_t vg_launch_finalize(hipStream_t st, long long nparts, double* part, const long long* s_total,
+                              const int* it_p, int hist_cap, double* hist, double* dsum, int* err) {
+    hipLaunchKernelGGL(vg_finalize_kernel, dim3(1), dim3(VG_TREE), 0, st, nparts, part, s_total, it_p,
+                       hist_cap, hist, dsum, err);
+    return hipGetLastError();
+}
+
+hipError_t vg_launch_alloc(hipStream_t st, int C, int mode, int nh_const, const double* dh,
+                           const double* dsum, long long N, int* nh, long long* bsum) {
+    const unsigned nb = (unsigned)((C + VG_TREE - 1) / VG_TREE);
+    hipLaunchKernelGGL(vg_alloc_kernel, dim3(nb), dim3(VG_TREE), 0, st, C, mode, nh_const, dh, dsum,
+                       N, nh, bsum);
+    return hipGetLastError();
+}
+
+hipError_t vg_launch_scan(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
+                          long long* s_total, long long s_cap, int* it_p, int advance, int K,
+                          long long run_cap, long long* nh_sum, int* run_cube, int* err) {
+    const long long nb = (C + VG_TREE - 1) / VG_TREE;
+    hipLaunchKernelGGL(vg_scan_blocks_kernel, dim3(1), dim3(VG_TREE), 0, st, nb, bsum, boff, s_total,
+                       s_cap, it_p, advance, err);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(vg_scan_cubes_kernel, dim3((unsigned)nb), dim3(VG_TREE), 0, st, C, nh, boff,
+                       s_total, K, run_cap, nh_sum, run_cube);
+    return hipGetLastError();
+}
+
+hipError_t vg_launch_init_map(hipStream_t st, int d, int n, double* x_edges, double2* map64,
+                              float2* map32, double* weights, unsigned* counts) {
+    hipLaunchKernelGGL(vg_init_map_kernel, dim3(d), dim3(VG_TREE), 0, st, n, x_edges, map64, map32,
+                       weights, counts);
+    return hipGetLastError();
+}
+
+hipError_t vg_launch_refine(hipStream_t st, int d, int n, double alpha, double* x_edges,
+                            double* weights, unsigned* counts, double* scratch, double2* map64,
+                            float2* map32) {
+    hipLaunchKernelGGL(vg_refine_kernel, dim3(d), dim3(VG_TREE), 0, st, n, alpha, x_edges, weights,
+                       counts, scratch, map64, map32);
+    return hipGetLastError();
+}
+
+__global__ void vg_write_tables_kernel(int n, const double* __restrict__ x_edges,
+                                       double2* __restrict__ map64, float2* __restrict__ map32) {
+    const int dim = blockIdx.x;
+    vg_write_tables(n, x_edges + (long long)dim * (n + 1), map64 + (long long)dim * n,
+                    map32 + (long long)dim * n);
+}
+
+hipError_t vg_launch_tables(hipStream_t st, int d, int n, const double* x_edges, double2* map64,
+                            float2* map32) {
+    hipLaunchKernelGGL(vg_write_tables_kernel, dim3(d), dim3(256), 0, st, n, x_edges, map64, map32);
+    return hipGetLastError();
+}
+
+// Block sums of an existing n_h (no reallocation), then the prefix layout.
+__global__ void __launch_bounds__(VG_TREE)
+vg_bsum_kernel(int C, const int* __restrict__ nh, long long* __restrict__ bsum) {
+    __shared__ long long t[VG_TREE];
+    const long long h = (long long)blockIdx.x * VG_TREE + threadIdx.x;
+    t[threadIdx.x] = h < C ? nh[h] : 0;
+    for (int stride = VG_TREE / 2; stride >= 1; stride >>= 1) {
+        __syncthreads();
+        if ((int)threadIdx.x < stride) t[threadIdx.x] += t[threadIdx.x + stride];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = t[0];
+}
+
+hipError_t vg_launch_layout(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
+                            long long* s_total, long long s_cap, int* it_p, int K, long long run_cap,
+                            long long* nh_sum, int* run_cube, int* err) {
+    const unsigned nb = (unsigned)((C + VG_TREE - 1) / VG_TREE);
+    hipLaunchKernelGGL(vg_bsum_kernel, dim3(nb), dim3(VG_TREE), 0, st, C, nh, bsum);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return vg_launch_scan(st, C, nh, bsum, boff, s_total, s_cap, it_p, 0, K, run_cap, nh_sum,
+                          run_cube, err);
+}

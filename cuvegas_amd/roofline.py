@@ -1,0 +1,46 @@
+"""Algorithmic work per integrand evaluation and the VALU roofline of gfx950.
+
+The sampling kernel has no dense contraction (no MFMA) and moves O(cubes)
+bytes per iteration, not O(samples): it is bound by vector-ALU issue.  Work is
+counted in VALU lane-slots per evaluation with fixed rules (one slot per VALU
+operation, FP64 and FP32 alike since CDNA vector FP64 is full rate; a native
+transcendental 4 slots plus 1 for range scaling; an LDS atomic 1 slot):
+
+  S = 60 * ceil(d/4)   Philox4x32-10: 10 rounds x (2 mul_lo + 2 mul_hi + 2 xor3)
+                       per 4 random words
+    + 16 * d           uniform conversion 2, stratum/map/Jacobian 12, bin updates 2
+    + S_f              the integrand
+    + 10               f*J, (f*J)^2 and two compensated adds
+
+with S_f = 2d+7 (Gaussian peak), 4d+14 (double Gaussian), d+6 (oscillatory),
+d+9 (corner peak), K(2d+7) (K-component Gaussian mixture).
+
+Peak = CUs x 64 lanes x clock (lane-slots per second).
+"""
+from __future__ import annotations
+
+import math
+
+INTEGRAND_SLOTS = {
+    "gaussian_peak": lambda d, k=1: 2 * d + 7,
+    "double_gaussian": lambda d, k=1: 4 * d + 14,
+    "oscillatory": lambda d, k=1: d + 6,
+    "corner_peak": lambda d, k=1: d + 9,
+    "gaussian_mixture": lambda d, k=4: k * (2 * d + 7),
+}
+
+
+def slots_per_eval(d: int, family: str, components: int = 4) -> int:
+    """VALU lane-slots per integrand evaluation (the rules above)."""
+    s_f = INTEGRAND_SLOTS[family](d, components)
+    return 60 * math.ceil(d / 4) + 16 * d + s_f + 10
+
+
+def valu_peak(n_cu: int, clock_hz: float) -> float:
+    """Lane-slots per second: every SIMD issues one 64-lane VALU op per cycle per CU."""
+    return n_cu * 64 * clock_hz
+
+
+MI355X_CU = 256
+MI355X_CLOCK_HZ = 2.4e9
+MI355X_HBM_BPS = 8.0e12

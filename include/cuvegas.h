@@ -1,0 +1,186 @@
+/*
+ * cuvegas.h -- C ABI of the MI355X VEGAS+ integrator (libcuvegas_amd.so).
+ *
+ * Plain pointers and sizes only.  Every function returns CVG_OK (0) or a
+ * negative status; cvg_last_error() gives the message of the calling thread's
+ * last failure.  Nothing here calls exit() (the reference exits the process on
+ * any CUDA error: vegas_cuda/cuda-samples/Common/helper_cuda.h:582-595,
+ * vegas_cuda/src/vegas_cuda_extension/commons/common.cuh:39-45).
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   cvg_integrate_cuda   <- integrate(...) bound as integrate_cuda
+ *                           (vegas_cuda/src/vegas_cuda_extension/cuda_wrapper.cu:290,
+ *                            pybind at :874-876; 14-tuple at :869)
+ *   cvg_program_create   <- jit(): cuLink of cuda_kernel.ptx + integrand PTX
+ *                           (cuda_wrapper.cu:221-286; python_wrapper.py:49-59)
+ *   cvg_state_create     <- allocation and initialisation (cuda_wrapper.cu:348-543)
+ *   cvg_state_run        <- the iteration loop (cuda_wrapper.cu:575-773)
+ *   cvg_state_result     <- weighted average and error (cuda_wrapper.cu:750-766,869)
+ *   cvg_state_refine     <- map refinement (cuda_wrapper.cu:719-746)
+ *   cvg_state_reallocate <- stratification reallocation (cuda_wrapper.cu:729-730)
+ *   cvg_state_dump       <- no counterpart: diagnostics of the sampling step
+ *   cvg_state_get_..., cvg_state_set_... <- no counterpart: the reference keeps all state per call
+ *                           (cuda_wrapper.cu:394-450,779-851)
+ *
+ * Threading: handles are independent; a state must not be used from two
+ * threads at once.  All device work of a state is ordered on its stream.
+ */
+#ifndef CUVEGAS_H
+#define CUVEGAS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CVG_OK 0
+#define CVG_ERR_INVALID (-1)   /* bad argument or unsupported shape */
+#define CVG_ERR_COMPILE (-2)   /* hipRTC rejected the integrand; see cvg_program_log */
+#define CVG_ERR_HIP (-3)       /* HIP runtime error (message names the call) */
+#define CVG_ERR_ALLOC (-4)     /* device or host allocation failed */
+#define CVG_ERR_CAPACITY (-5)  /* allocation or iteration count exceeded the buffers */
+#define CVG_ERR_NODEVICE (-6)  /* no HIP device visible */
+
+#define CVG_FP64 0   /* FP64 sampling, map lookup and integrand */
+#define CVG_FP32C 1  /* FP32 sampling, map lookup and integrand; FP64 sums and map */
+
+typedef struct cvg_program cvg_program;
+typedef struct cvg_state cvg_state;
+
+typedef struct cvg_params {
+    int n_dim;            /* dimensions d (1..64) */
+    int n_intervals;      /* map bins per dimension (>= 2) */
+    int n_strat;          /* strata per dimension (>= 1), n_strat^d < 2^31 */
+    int max_it;           /* iterations to run (history capacity) */
+    int skip;             /* leading iterations left out of the weighted average */
+    int precision;        /* CVG_FP64 or CVG_FP32C */
+    long long n_eval_it;  /* sample budget per iteration */
+    double alpha;         /* map damping exponent */
+    double beta;          /* stratification exponent: n_h ~ (sigma_h^2)^beta */
+    unsigned int seed;    /* Philox key word 0 */
+    unsigned int stream_id; /* Philox key word 1 (separates independent integrals) */
+    int run_len;          /* samples per lane run; 0 selects 16 */
+    int has_domain;       /* non-zero: domain[] below is used */
+    const double* domain; /* 2*n_dim values: lower bounds, then ranges */
+    int n_theta;          /* parameters of a parametrised integrand (0 if none) */
+} cvg_params;
+
+/* Result of a run; timings are wall clock (ms). */
+typedef struct cvg_result {
+    double result;        /* inverse-variance weighted mean over iterations >= skip */
+    double error;         /* 1 / sqrt(sum 1/sigma_k^2) */
+    double chi2_dof;      /* sum (I_k - result)^2 / sigma_k^2 / (K - 1) */
+    long long tot_nevals; /* integrand evaluations over all iterations */
+    double loop_ms;       /* device time of the iteration loop (HIP events) */
+    double compile_ms;    /* hipRTC compile and module load */
+    double setup_ms;      /* allocation and initialisation */
+    int iterations;       /* iterations completed */
+    int n_cubes;
+    int n_strat;
+    int n_intervals;
+} cvg_result;
+
+/* The 14 values the reference's integrate_cuda returns (cuda_wrapper.cu:869). */
+typedef struct cvg_ref_result {
+    double res;
+    double err;
+    int n_strat;
+    int n_cubes;
+    int n_intervals;
+    long long tot_nevals;
+    double tot_time;      /* ms, whole call */
+    double avg_it_time;   /* ms, loop time / max_it */
+    unsigned int seed;
+    int multi_gpu;
+    int multi_cpu;
+    int gpu_num;
+    int omp_threads_num;
+    double gpu_thr;
+} cvg_ref_result;
+
+/* Last error message of the calling thread ("" if none). */
+const char* cvg_last_error(void);
+
+/* Device facts used for roofline figures. */
+int cvg_device_info(int* n_cu, int* clock_khz, char* arch, int arch_len);
+
+/* Compile an integrand source (HIP C++) for the given shape with hipRTC.
+ * shape supplies n_dim, n_intervals, n_strat, precision, run_len, has_domain.
+ * options: extra hipRTC options separated by spaces, or NULL.
+ * On CVG_ERR_COMPILE, *out is still set so cvg_program_log can be read. */
+int cvg_program_create(const char* integrand_src, const cvg_params* shape, const char* options,
+                       cvg_program** out);
+const char* cvg_program_log(const cvg_program* prog);
+/* Size of the compiled code object (bytes), 0 if compilation failed. */
+long long cvg_program_code_size(const cvg_program* prog);
+/* Copy the gfx950 code object (up to cap bytes) for caching or inspection;
+ * returns the bytes copied. */
+long long cvg_program_get_code(const cvg_program* prog, void* buf, long long cap);
+void cvg_program_destroy(cvg_program* prog);
+
+/* Integrator state: allocates the device buffers and initialises the uniform
+ * map and the initial allocation n_h = max(2, (int)(N / C)). */
+int cvg_state_create(cvg_program* prog, const cvg_params* params, cvg_state** out);
+void cvg_state_destroy(cvg_state* st);
+/* Use this HIP stream (hipStream_t) for all device work; NULL = own stream. */
+int cvg_state_set_stream(cvg_state* st, void* hip_stream);
+/* Parameters of a parametrised integrand (host array of n_theta doubles). */
+int cvg_state_set_theta(cvg_state* st, const double* theta, int n_theta);
+/* Enqueue n_iter iterations.  use_graph: capture one iteration in a hipGraph
+ * and replay it. */
+int cvg_state_run(cvg_state* st, int n_iter, int use_graph);
+/* Wait for the state's work and combine the iterations run so far. */
+int cvg_state_result(cvg_state* st, cvg_result* out);
+/* Per-iteration device timing of eager runs (use_graph = 0): HIP events
+ * around the sampling kernel and around the whole iteration, on the state's
+ * stream.  get_timing returns the number of timed iterations. */
+int cvg_state_set_timing(cvg_state* st, int enable);
+int cvg_state_get_timing(cvg_state* st, double* fill_ms, double* iter_ms, int cap);
+/* Per-iteration estimates: I_k, sigma_k^2 and samples; returns the count. */
+int cvg_state_history(cvg_state* st, double* I, double* sigma2, long long* nevals, int cap);
+/* Number of cubes of a state. */
+long long cvg_state_n_cubes(const cvg_state* st);
+
+/* State I/O (host buffers).  Map edges: n_dim * (n_intervals + 1). */
+int cvg_state_get_map(cvg_state* st, double* x_edges);
+int cvg_state_set_map(cvg_state* st, const double* x_edges);
+/* Allocation n_h per cube (n_cubes ints). set_nh also rebuilds the sample layout. */
+int cvg_state_get_nh(cvg_state* st, int* nh);
+int cvg_state_set_nh(cvg_state* st, const int* nh);
+/* d_h of the last iteration (n_cubes doubles). */
+int cvg_state_get_dh(cvg_state* st, double* dh);
+/* Per-cube sums of J f and (J f)^2 of the last iteration (n_cubes each). */
+int cvg_state_get_cube_sums(cvg_state* st, double* jf, double* jf2);
+/* Bin accumulators: weights (n_dim*n_intervals doubles), counts (uint32). */
+int cvg_state_get_bins(cvg_state* st, double* weights, unsigned int* counts);
+int cvg_state_set_bins(cvg_state* st, const double* weights, const unsigned int* counts);
+/* Iteration counter (selects the Philox counter block). */
+int cvg_state_set_iteration(cvg_state* st, int it);
+
+/* Single steps, for callers that drive their own loop. */
+int cvg_state_fill(cvg_state* st);      /* one sampling pass with the current map */
+int cvg_state_refine(cvg_state* st);    /* map refinement from the bin accumulators */
+/* Allocation from given d_h (host, n_cubes) and D; rebuilds the layout. */
+int cvg_state_reallocate(cvg_state* st, const double* dh, double D);
+
+/* Samples [0, n) of iteration it under the current map and allocation:
+ * cube index, bin ids (n*n_dim), point in the unit cube (n*n_dim), J f. */
+int cvg_state_dump(cvg_state* st, int it, long long n, long long* cube, int* ids, double* x,
+                   double* jf);
+
+/* One-shot call with the reference's argument list (cuda_wrapper.cu:290).
+ * integrand_src replaces the numba PTX; options replaces the kernel PTX path
+ * and may hold "fp32" (CVG_FP32C) and extra hipRTC options.  batch_size,
+ * multi_cpu, cpu_num and gpu_thr are accepted and reported back unchanged;
+ * the device path needs no sample batching or host-side cube mapping.
+ * multi_gpu runs on the current device only. */
+int cvg_integrate_cuda(long long tot_eval, int max_it, int skip, long long max_batch_size,
+                       int n_intervals, int n_strat, int n_dim, double alpha, double beta,
+                       unsigned int seed, int multi_gpu, int multi_cpu, int gpu_num, int cpu_num,
+                       double gpu_thr, const char* integrand_src, const char* options,
+                       cvg_ref_result* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUVEGAS_H */

@@ -1,0 +1,130 @@
+"""C ABI: the library loads, exports every declared symbol, validates its
+arguments and compiles integrands with hipRTC -- all without a GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    with open(os.path.join(ROOT, "include", "cuvegas.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cvg_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol():
+    from cuvegas_amd import _lib
+    names = _declared()
+    assert len(names) >= 30
+    raw = C.CDLL(_lib.LIB_PATH)
+    for n in names:
+        assert hasattr(raw, n), n
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+def _params(**kw):
+    from cuvegas_amd.api import make_params
+    base = dict(n_dim=3, n_intervals=64, n_strat=4, max_it=5, skip=1, n_eval_it=1000,
+                alpha=0.5, beta=0.75, seed=1)
+    base.update(kw)
+    return make_params(**base)[0]
+
+
+@pytest.mark.parametrize("bad", [dict(n_dim=0), dict(n_dim=65), dict(n_intervals=1),
+                                 dict(n_strat=0), dict(n_strat=2**16, n_dim=2),
+                                 dict(precision=7), dict(n_strat=70000, n_intervals=70000, n_dim=1)])
+def test_invalid_shapes_are_rejected(bad):
+    from cuvegas_amd._lib import CVG_ERR_INVALID, lib
+    h = C.c_void_p()
+    rc = lib.cvg_program_create(b"__device__ vg_real integrand(vg_real* x){return x[0];}",
+                                C.byref(_params(**bad)), None, C.byref(h))
+    assert rc == CVG_ERR_INVALID
+    assert lib.cvg_last_error()
+
+
+def test_compile_error_reports_log():
+    from cuvegas_amd import CuvegasError
+    from cuvegas_amd.api import Program
+    with pytest.raises(CuvegasError) as ei:
+        Program("__device__ vg_real integrand(vg_real* x) { return undefined_symbol(x); }", _params())
+    assert ei.value.code == -2
+    assert "undefined_symbol" in str(ei.value)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_library_integrands_compile(prec):
+    from cuvegas_amd import integrands as I
+    from cuvegas_amd.api import Program
+    for name in sorted(I.LIBRARY):
+        prog = Program(I.by_name(name, 3).source, _params(precision=prec))
+        assert prog.code_size > 1000, name
+        assert prog.code()[:4] == b"\x7fELF"
+
+
+def test_integrand_forms_compile():
+    """config.cuh form, extension status form and the parametrised form."""
+    from cuvegas_amd.api import Program
+    forms = [
+        "__host__ __device__ double integrand(double* x) { return x[0] * x[1]; }\nconst int n_dim = 3;",
+        'extern "C" __device__ int integrand(double* ret, double* x) { *ret = x[0]; return 0; }',
+        "__device__ vg_real integrand(vg_real* x, const vg_real* t) { return t[0] * x[0]; }",
+    ]
+    for src in forms:
+        assert Program(src, _params()).code_size > 0
+
+
+def test_state_needs_device():
+    from cuvegas_amd._lib import CVG_ERR_NODEVICE, lib
+    from cuvegas_amd.api import Program
+    ndev = C.c_int(0)
+    try:
+        import torch
+        if torch.cuda.device_count() > 0:
+            pytest.skip("a GPU is visible")
+    except ImportError:
+        pass
+    p = _params()
+    prog = Program("__device__ vg_real integrand(vg_real* x){return x[0];}", p)
+    h = C.c_void_p()
+    assert lib.cvg_state_create(prog.handle, C.byref(p), C.byref(h)) == CVG_ERR_NODEVICE
+    del ndev
+
+
+def test_reference_entry_points_importable():
+    from vegas_cuda_extension.cuda_extension import integrate_cuda  # noqa: F401
+    from vegas_cuda_extension.python_wrapper import integrate
+    import inspect
+    sig = inspect.signature(integrate)
+    names = list(sig.parameters)[:17]
+    assert names == ["device_function", "n_dim", "domain", "n_evals", "max_it", "skip", "batch_size",
+                     "n_intervals", "n_strat", "alpha", "beta", "seed", "multi_gpu", "multi_cpu",
+                     "gpu_num", "cpu_num", "gpu_thr"]
+    assert sig.parameters["n_evals"].default == 1000000
+    assert sig.parameters["max_it"].default == 20
+    assert sig.parameters["n_intervals"].default == 1024
+
+
+def test_default_n_strat_matches_reference_formula():
+    from cuvegas_amd.api import default_n_strat
+    assert default_n_strat(10**7, 10, 4) == 22
+    assert default_n_strat(2 * 10**9, 20, 8) == 8
+    assert default_n_strat(5 * 10**7, 10, 6) == 10
+
+
+def test_closed_forms_of_library():
+    import math
+    from cuvegas_amd import integrands as I
+    assert I.ridge(4).exact == pytest.approx(0.851317758241297887, rel=1e-12)
+    assert I.gaussian_peak(4).exact == pytest.approx(0.99999770678939702466, rel=1e-13)
+    assert I.double_gaussian(8).exact == pytest.approx(0.99999999983553688726, rel=1e-13)
+    assert I.gaussian(4).exact == pytest.approx(1.0, rel=1e-15)
+    # corner peak: inclusion-exclusion vs. 1-D quadrature in d = 1
+    assert I.corner_peak(1, [2.0]).exact == pytest.approx(1.0 / 3.0, rel=1e-14)  # int (1+2x)^-2
+    from scipy import integrate as si
+    q, _ = si.nquad(lambda x, y: (1 + 0.5 * x + 1.5 * y) ** -3, [[0, 1], [0, 1]])
+    assert I.corner_peak(2, [0.5, 1.5]).exact == pytest.approx(q, rel=1e-10)
+    assert I.oscillatory(1, 0.0, [1.0]).exact == pytest.approx(math.sin(1.0), rel=1e-14)

@@ -1,0 +1,265 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Bit-exact: sample -> cube and bin indices, mapped points, per-cube sums and
+d_h for polynomial integrands, the integer allocation n_h (from identical
+inputs, and end to end after the first iteration), map refinement from
+identical bin accumulators.  Bin counts are exact; bin weights are atomics
+(order-dependent rounding, relative 1e-12).  Integrals: pull against closed
+forms and agreement with the oracle within the combined errors.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def cv():
+    import cuvegas_amd
+    return cuvegas_amd
+
+
+ORC_ID = {"poly_peak": 2, "linear": 1, "one": 0, "gaussian_peak": 3, "double_gaussian": 4}
+
+
+def _orc_params(orc, itg, d, n, ns, N, iters, prec, K=16, seed=0, skip=1, domain=None):
+    params = list(itg.params)
+    return orc.make_params(d, n, ns, N, iters, ORC_ID[itg.name], params, skip=skip, K=K,
+                           seed=seed, precision=prec, domain=domain)
+
+
+def _integrator(cv, itg, d, n, ns, N, iters, prec, K=16, seed=0, skip=1, domain=None):
+    return cv.Integrator(itg.source, d, N, n_intervals=n, n_strat=ns, max_it=iters, skip=skip,
+                         seed=seed, precision=prec, run_len=K, domain=domain)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", ["poly_peak", "gaussian_peak"])
+def test_sample_dump_bit_exact(cv, orc, prec, name):
+    from cuvegas_amd import integrands as I
+    d, n, ns, N = 3, 16, 3, 3000
+    itg = I.poly_peak(d) if name == "poly_peak" else I.gaussian_peak(d, 0.45, 0.2)
+    _say("dump", name, prec)
+    st = _integrator(cv, itg, d, n, ns, N, 4, prec)
+    rng = np.random.default_rng(11)
+    for case in range(3):
+        if case > 0:
+            x = np.concatenate([np.zeros((d, 1)), np.sort(rng.random((d, n - 1)), axis=1),
+                                np.ones((d, 1))], axis=1)
+            st.set_map(x)
+            nh = rng.integers(2, 40, st.n_cubes).astype(np.int32)
+            st.set_nh(nh)
+        x_edges = st.get_map()
+        nh = st.get_nh()
+        S = int(nh.sum())
+        it = 3 * case
+        cube, ids, xs, jf = st.dump(it, S)
+        p, keep = _orc_params(orc, itg, d, n, ns, N, 4, 0 if prec == "fp64" else 1)
+        ocube, oids, ox, ojf = orc.sample_dump(p, keep, x_edges, nh, it, S)
+        assert np.array_equal(cube, ocube)
+        assert np.array_equal(ids, oids)
+        assert np.array_equal(xs, ox)
+        if name == "poly_peak":
+            assert np.array_equal(jf, ojf)
+        else:
+            tol = 1e-14 if prec == "fp64" else 2e-6
+            assert np.max(np.abs(jf - ojf) / np.maximum(np.abs(ojf), 1e-300)) < tol
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("layout", ["small_cubes", "big_cubes", "mixed"])
+@pytest.mark.parametrize("K", [16, 3])
+def test_first_iteration_bit_exact(cv, orc, prec, layout, K):
+    """One iteration of the polynomial integrand: per-cube sums (every layout:
+    cubes inside a run, across runs, across windows), d_h and the new n_h bit
+    for bit; bin counts exact, bin weights to 1e-12; refined map to 1e-12."""
+    from cuvegas_amd import integrands as I
+    d, n, alpha, beta = 3, 32, 0.5, 0.75
+    ns, N = {"small_cubes": (12, 5000), "big_cubes": (3, 200000), "mixed": (5, 100000)}[layout]
+    itg = I.poly_peak(d)
+    _say("first-iteration", prec, layout, K)
+    p, keep = _orc_params(orc, itg, d, n, ns, N, 1, 0 if prec == "fp64" else 1, K=K)
+    st = _integrator(cv, itg, d, n, ns, N, 2, prec, K=K)
+    if layout == "mixed":
+        rng = np.random.default_rng(3)
+        st.set_nh(np.where(rng.random(st.n_cubes) < 0.2, rng.integers(500, 3000, st.n_cubes), 2))
+    nh0 = st.get_nh()
+    x0 = st.get_map()
+    S = int(nh0.sum())
+    _, ids, _, ojf = orc.sample_dump(p, keep, x0, nh0, 0, S)
+    oJF, oJF2 = orc.cube_sums(nh0, ojf, ojf * ojf, K)
+    ow = np.zeros((d, n))
+    oc = np.zeros((d, n), dtype=np.uint32)
+    for i in range(d):
+        np.add.at(ow[i], ids[:, i], ojf * ojf)
+        oc[i] = np.bincount(ids[:, i], minlength=n)
+    # bins of one sampling pass
+    st.fill()
+    w, c = st.get_bins()
+    assert np.array_equal(c, oc)
+    # FP64 form: FP64 LDS partials; FP32 form: FP32 per-block partials merged in FP64
+    wtol, mtol = (1e-12, 1e-12) if prec == "fp64" else (5e-5, 1e-6)
+    assert np.max(np.abs(w - ow) / np.maximum(ow, 1e-300)) < wtol
+    # one full iteration on a fresh state with the same layout
+    st2 = _integrator(cv, itg, d, n, ns, N, 2, prec, K=K)
+    if layout == "mixed":
+        st2.set_nh(nh0)
+    st2.run(1, use_graph=False)
+    JF, JF2 = st2.get_cube_sums()
+    assert np.array_equal(JF, oJF)
+    assert np.array_equal(JF2, oJF2)
+    V = math.pow(1.0 / ns, d)
+    odh, sums = orc.results(V, beta, nh0, oJF, oJF2)
+    assert np.array_equal(st2.get_dh(), odh)
+    assert np.array_equal(st2.get_nh(), orc.alloc(odh, sums[2], N))
+    I_k, S2, ne = st2.history()
+    assert I_k[0] == sums[0] and S2[0] == sums[1] and ne[0] == S
+    assert np.max(np.abs(st2.get_map() - orc.refine(alpha, ow, oc, x0))) < mtol
+    if layout != "mixed":  # end to end against the oracle integrator from scratch
+        r = orc.run(p, keep, st.n_cubes)
+        assert np.array_equal(JF, r["JF_first"]) and np.array_equal(st2.get_nh(), r["nh_first"])
+
+
+def test_refine_pure_function_bit_exact(cv, orc):
+    """Identical bin accumulators in, identical map out."""
+    from cuvegas_amd import integrands as I
+    d, n = 4, 512
+    itg = I.poly_peak(d)
+    _say("refine")
+    st = _integrator(cv, itg, d, n, 2, 1000, 2, "fp64")
+    rng = np.random.default_rng(5)
+    x = np.concatenate([np.zeros((d, 1)), np.sort(rng.random((d, n - 1)), axis=1), np.ones((d, 1))], axis=1)
+    st.set_map(x)
+    for trial in range(3):
+        cnt = rng.integers(0, 30, (d, n)).astype(np.uint32)
+        wts = (rng.random((d, n)) ** 4 * 1e3) * cnt
+        if trial == 2:
+            wts[1] = 0.0
+            cnt[1] = 0
+        xin = st.get_map()
+        st.set_bins(wts, cnt)
+        st.refine()
+        got = st.get_map()
+        ref = orc.refine(0.5, wts, cnt, xin)
+        assert np.array_equal(got, ref), np.max(np.abs(got - ref))
+        w2, c2 = st.get_bins()
+        assert not w2.any() and not c2.any()  # cleared for the next fill
+
+
+def test_reallocate_pure_function_bit_exact(cv, orc):
+    from cuvegas_amd import integrands as I
+    d, ns, N = 4, 9, 250000
+    itg = I.poly_peak(d)
+    _say("reallocate")
+    st = _integrator(cv, itg, d, 64, ns, N, 2, "fp64")
+    rng = np.random.default_rng(9)
+    C = st.n_cubes
+    for dh in [rng.random(C) ** 6, np.ones(C), np.zeros(C), np.where(np.arange(C) == 5, 1.0, 0.0)]:
+        D = orc.tree_sum(dh)
+        st.reallocate(dh, D)
+        assert np.array_equal(st.get_nh(), orc.alloc(dh, D, N))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_integral_matches_oracle_and_closed_form(cv, orc, prec):
+    from cuvegas_amd import integrands as I
+    d, n, ns, N, iters = 4, 128, 5, 40000, 10
+    itg = I.gaussian_peak(d, 0.5, 0.1)
+    _say("integral", prec)
+    st = _integrator(cv, itg, d, n, ns, N, iters, prec, skip=3, seed=2)
+    st.run(iters, use_graph=True)
+    r = st.result()
+    p, keep = _orc_params(orc, itg, d, n, ns, N, iters, 0 if prec == "fp64" else 1, seed=2, skip=3)
+    o = orc.run(p, keep, st.n_cubes, snapshots=False)
+    assert abs(r["result"] - itg.exact) < 4 * r["error"]
+    assert abs(r["result"] - o["result"]) < 4 * math.hypot(r["error"], o["error"])
+    assert r["chi2_dof"] < 3.0
+    I_k, S2, ne = st.history()
+    assert ne.sum() == r["tot_nevals"]
+    assert ne[0] == o["hist_n"][0]
+    assert abs(I_k[0] - o["hist_I"][0]) < 1e-12 * abs(o["hist_I"][0]) + 1e-6 * (prec == "fp32")
+
+
+def test_graph_and_eager_agree(cv):
+    from cuvegas_amd import integrands as I
+    d, n, ns, N, iters = 5, 64, 4, 50000, 6
+    itg = I.double_gaussian(d, 0.3, 0.7, 0.1)
+    _say("graph-vs-eager")
+    out = []
+    for g in (True, False):
+        st = _integrator(cv, itg, d, n, ns, N, iters, "fp64", seed=4, skip=2)
+        st.run(iters, use_graph=g)
+        out.append((st.result(), st.history()[0]))
+    (ra, Ia), (rb, Ib) = out
+    assert Ia[0] == Ib[0]  # first iteration is deterministic
+    assert abs(ra["result"] - rb["result"]) < 3 * ra["error"]
+    assert abs(ra["result"] - itg.exact) < 4 * ra["error"]
+
+
+def test_integrate_dropin_and_domain(cv):
+    from vegas_cuda_extension.python_wrapper import integrate
+    from vegas_cuda_extension.cuda_extension import integrate_cuda
+    _say("integrate-dropin")
+    src = """
+__device__ double integrand(double* x) {
+    double f = 0.0;
+    for (int i = 0; i < VG_NDIM; ++i) f += x[i] * x[i];
+    return f;
+}"""
+    # domain [1,2] x [0,3] x [-1,1]: integral = vol * (sum of E[x_i^2])
+    dom = [[1, 2], [0, 3], [-1, 1]]
+    res, info = integrate(src, 3, domain=dom, n_evals=2_000_000, max_it=10, n_intervals=256, seed=1)
+    vol = 1 * 3 * 2
+    exact = vol * (7.0 / 3.0 + 3.0 + 1.0 / 3.0)
+    for k in ["error", "n_strat", "n_cubes", "n_intervals", "tot_nevals", "tot_time", "avg_it_time",
+              "seed", "multi_gpu", "multi_cpu", "gpu_num", "omp_threads_num", "gpu_thr"]:
+        assert k in info
+    assert abs(res - exact) < 4 * info["error"]
+    assert info["n_strat"] == int(math.pow(200000 / 4.0, 1.0 / 3))
+    t = integrate_cuda(2_000_000, 10, 5, 1048576, 256, 10, 4, 0.5, 0.75, 3, False, False, -1, -1, 0.1,
+                       "__device__ double integrand(double* x){ return x[0]*x[1]*x[2]*x[3]*16.0; }", "")
+    assert len(t) == 14 and abs(t[0] - 1.0) < 4 * t[1]
+
+
+def test_fp32_form_double_gaussian(cv):
+    from cuvegas_amd import integrands as I
+    d, n, ns, N, iters = 8, 256, 4, 2_000_000, 12
+    itg = I.double_gaussian(d, 0.33, 0.67, 0.05)
+    _say("fp32-double-gaussian")
+    for prec in ("fp32", "fp64"):
+        st = _integrator(cv, itg, d, n, ns, N, iters, prec, seed=5, skip=4)
+        st.run(iters)
+        r = st.result()
+        assert abs(r["result"] - itg.exact) < 5 * r["error"], (prec, r)
+        assert r["error"] / itg.exact < 5e-3
+
+
+def test_large_shape_properties(cv):
+    """Config-2 shape (8-D, 1024 bins, 8 strata, 1e7 per iteration): sizes and
+    invariants that hold at any size."""
+    from cuvegas_amd import integrands as I
+    d, n, ns, N, iters = 8, 1024, 8, 10_000_000, 3
+    itg = I.double_gaussian(d)
+    _say("large-shape")
+    st = _integrator(cv, itg, d, n, ns, N, iters, "fp32", skip=1)
+    st.run(iters)
+    r = st.result()
+    nh = st.get_nh()
+    I_k, S2, ne = st.history()
+    assert st.n_cubes == 8 ** 8
+    assert nh.min() >= 2 and nh.sum() <= N + 2 * st.n_cubes
+    assert ne[0] == 2 * 8 ** 8  # initial n_h = max(2, N / C) = 2
+    assert r["tot_nevals"] == ne.sum()
+    x = st.get_map()
+    assert np.all(np.diff(x, axis=1) > 0) and np.all(x[:, 0] == 0) and np.all(x[:, -1] == 1)
+    JF, JF2 = st.get_cube_sums()
+    assert np.all(JF2 >= 0)
+    assert np.isfinite(r["result"])

@@ -1,0 +1,232 @@
+"""The CPU oracle pinned against published vectors, libm, exact arithmetic and
+closed forms (no GPU needed)."""
+import json
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_philox_known_answers(orc):
+    with open(os.path.join(GOLD, "philox4x32_10_kat.json")) as f:
+        kat = json.load(f)["vectors"]
+    for v in kat:
+        ctr = [int(x, 16) for x in v["ctr"]]
+        key = [int(x, 16) for x in v["key"]]
+        out = orc.philox(ctr, key)
+        assert [f"{w:08x}" for w in out] == v["out"]
+
+
+def test_log_exp_pow_against_libm(orc):
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(1e-3, 10, 2000), 10.0 ** rng.uniform(-300, 300, 2000),
+                         [1e-310, 5e-324, 2.2250738585072014e-308, 0.5, 1.0, 2.0, 0.75,
+                          1.4142135623730951, 0.7071067811865476]])
+    for x in xs:
+        ref = math.log(x)
+        got = orc.log(x)
+        assert abs(got - ref) <= 4e-16 * max(abs(ref), 1e-300) + 1e-320, (x, got, ref)
+    ys = np.concatenate([rng.uniform(-700, 700, 3000), rng.uniform(-1, 1, 1000),
+                         [0.0, -745.0, 709.7, 1e-20, -1e-20]])
+    for y in ys:
+        ref = math.exp(y)
+        got = orc.exp(y)
+        assert abs(got - ref) <= 4e-16 * ref + 5e-324, (y, got, ref)
+    for _ in range(3000):
+        x = 10.0 ** rng.uniform(-250, 5)
+        a = rng.uniform(0.01, 2.0)
+        ref = x ** a
+        assert abs(orc.pow_(x, a) - ref) <= 1e-13 * ref + 1e-320, (x, a)
+    # special values
+    assert orc.pow_(0.0, 0.75) == 0.0
+    assert orc.pow_(1.0, 0.3) == 1.0
+    assert orc.pow_(7.0, 0.0) == 1.0
+    assert math.isnan(orc.pow_(-1.0, 0.5))
+    assert orc.log(1.0) == 0.0 and orc.exp(0.0) == 1.0
+    assert orc.log(0.0) == -math.inf and math.isnan(orc.log(-1.0))
+
+
+def test_sat_int(orc):
+    assert orc.sat_int(float("nan")) == 0
+    assert orc.sat_int(3.99) == 3 and orc.sat_int(-3.99) == -3
+    assert orc.sat_int(1e300) == 2**31 - 1 and orc.sat_int(-1e300) == -(2**31)
+
+
+@pytest.mark.parametrize("nbins,nstrat", [(1024, 8), (1024, 22), (512, 2), (4, 3), (7, 5), (1000, 1)])
+def test_bin_of_matches_exact_rational_floor(orc, nbins, nstrat):
+    """id = floor(y*n) and offset = y*n - id for y = (k + (2r+1)/2^33) / nstrat, exactly."""
+    rng = np.random.default_rng(nbins * 31 + nstrat)
+    base = list(rng.integers(0, 2**32, 300)) + [0, 1, 2**31 - 1, 2**31, 2**32 - 1, 2**32 - 2]
+    for k in sorted({0, nstrat - 1, nstrat // 2}):
+        # words straddling every bin boundary inside stratum k: u = j*nstrat/nbins - k
+        words = list(base)
+        for j in range(k * nbins // nstrat, (k + 1) * nbins // nstrat + 2):
+            u = Fraction(j * nstrat, nbins) - k
+            if 0 < u < 1:
+                w = math.floor(u * 2**32 - Fraction(1, 2))
+                words += [v for v in (w - 1, w, w + 1) if 0 <= v < 2**32]
+        for r in words[:2000]:
+            r = int(r)
+            y = (Fraction(k) + Fraction(2 * r + 1, 2**33)) / nstrat
+            yn = y * nbins
+            exp_id = math.floor(yn)
+            i, rem, low = orc.bin_of(r, k * nbins, nbins, nstrat)
+            assert i == exp_id
+            assert Fraction(rem) + Fraction(low, 2**33) == (yn - exp_id) * nstrat
+            assert 0 <= i < nbins
+
+
+def test_tree_sum_exact_and_close(orc):
+    rng = np.random.default_rng(3)
+    for m in [1, 5, 1023, 1024, 1025, 4096 + 17, 1024 * 1024 + 3]:
+        ints = rng.integers(-1000, 1000, m).astype(np.float64)
+        assert orc.tree_sum(ints) == float(ints.sum())
+    a = rng.standard_normal(200000) * 10.0 ** rng.uniform(-5, 5, 200000)
+    assert abs(orc.tree_sum(a) - math.fsum(a)) <= 1e-13 * np.abs(a).sum()
+    assert orc.tree_sum(np.zeros(0)) == 0.0
+
+
+@pytest.mark.parametrize("K", [1, 2, 3, 16, 64])
+def test_cube_sums_partition(orc, K):
+    """Every sample lands in exactly its cube: exact on integer data, for
+    layouts with cubes inside a run, across runs and across windows."""
+    rng = np.random.default_rng(K)
+    for trial in range(6):
+        C = int(rng.integers(1, 400))
+        if trial % 3 == 0:
+            nh = rng.integers(2, 6, C)
+        elif trial % 3 == 1:
+            nh = rng.integers(2, 3000, C)
+        else:
+            nh = np.where(rng.random(C) < 0.1, rng.integers(2000, 20000, C), 2)
+        S = int(nh.sum())
+        v1 = rng.integers(-50, 50, S).astype(np.float64)
+        v2 = v1 * v1
+        JF, JF2 = orc.cube_sums(nh, v1, v2, K)
+        starts = np.concatenate([[0], np.cumsum(nh)[:-1]])
+        ref1 = np.add.reduceat(v1, starts)
+        ref2 = np.add.reduceat(v2, starts)
+        assert np.array_equal(JF, ref1)
+        assert np.array_equal(JF2, ref2)
+        # floating data: within the error bound of any summation order
+        f1 = rng.standard_normal(S)
+        JFf, _ = orc.cube_sums(nh, f1, f1 * f1, K)
+        for h in rng.integers(0, C, 20):
+            seg = f1[starts[h]:starts[h] + nh[h]]
+            assert abs(JFf[h] - math.fsum(seg)) <= 4 * nh[h] * 1.2e-16 * np.abs(seg).sum() + 1e-300
+
+
+def test_results_constant_integrand(orc):
+    """Constant J f = c in a cube: I_h = V c, sigma_h^2 = 0, d_h = 0."""
+    C = 7
+    nh = np.full(C, 5, dtype=np.int32)
+    c = 3.0
+    JF = np.full(C, 5 * c)
+    JF2 = np.full(C, 5 * c * c)
+    V = 1.0 / 8
+    dh, sums = orc.results(V, 0.75, nh, JF, JF2)
+    assert np.all(dh == 0.0)
+    assert sums[0] == pytest.approx(C * V * c, rel=1e-15)
+    assert abs(sums[1]) < 1e-30
+    # two-point cube: values a, b -> sigma^2 = V^2 (a-b)^2 / 4
+    a, b = 1.0, 3.0
+    dh, sums = orc.results(0.5, 1.0, np.array([2], dtype=np.int32), np.array([a + b]),
+                           np.array([a * a + b * b]))
+    assert dh[0] == pytest.approx(0.25 * (a - b) ** 2 / 4, rel=1e-15)
+    assert sums[1] == pytest.approx(0.25 * (a - b) ** 2 / 4 / 2, rel=1e-15)
+
+
+def test_alloc_invariants(orc):
+    rng = np.random.default_rng(5)
+    C, N = 1000, 100000
+    dh = rng.random(C) ** 4
+    D = orc.tree_sum(dh)
+    nh = orc.alloc(dh, D, N)
+    assert nh.min() >= 2
+    assert nh.sum() <= N + 2 * C
+    expect = np.maximum(2, np.floor(dh / D * N)).astype(np.int64)
+    assert np.abs(nh - expect).max() <= 1  # floor of a correctly rounded quotient
+    uni = orc.alloc(np.ones(C), float(C), N)
+    assert np.all(uni == N // C)
+    one_hot = np.zeros(C)
+    one_hot[17] = 2.5
+    oh = orc.alloc(one_hot, 2.5, N)
+    assert oh[17] == N and np.all(np.delete(oh, 17) == 2)
+    # D = 0 (constant integrand): 0/0 -> NaN -> (int) 0 -> 2
+    assert np.all(orc.alloc(np.zeros(C), 0.0, N) == 2)
+
+
+def _inverse_cdf_edges(alpha, w, counts, x):
+    """Independent construction: damped smoothed weights define a piecewise
+    constant density on the old bins; new edges split its mass in n equal
+    parts (numpy interpolation of the cumulative mass)."""
+    n = w.size
+    ww = np.where(counts > 0, w / np.maximum(counts, 1), w)
+    s = ww.sum()
+    sm = np.empty(n)
+    sm[0] = (7 * ww[0] + ww[1]) / (8 * s)
+    sm[-1] = (ww[-2] + 7 * ww[-1]) / (8 * s)
+    sm[1:-1] = (ww[:-2] + 6 * ww[1:-1] + ww[2:]) / (8 * s)
+    d = np.where(sm != 0, ((sm - 1) / np.log(np.where(sm != 0, sm, 0.5))) ** alpha, 0.0)
+    cum = np.concatenate([[0.0], np.cumsum(d)])
+    targets = np.arange(1, n) * cum[-1] / n
+    return np.concatenate([[x[0]], np.interp(targets, cum, x), [x[-1]]])
+
+
+def test_refine_fixed_point_and_inverse_cdf(orc):
+    n = 64
+    x = np.linspace(0.0, 1.0, n + 1)
+    # uniform weights leave the map unchanged
+    out = orc.refine_dim(0.5, np.ones(n), np.ones(n, dtype=np.uint32), x)
+    assert np.max(np.abs(out - x)) <= 1e-15
+    assert out[0] == 0.0 and out[-1] == 1.0
+    # alpha = 0 damps every non-zero weight to 1: uniform bins again
+    rng = np.random.default_rng(2)
+    w = rng.random(n) + 0.1
+    out0 = orc.refine_dim(0.0, w, np.ones(n, dtype=np.uint32), x)
+    assert np.max(np.abs(out0 - x)) <= 1e-14
+    # general weights on a non-uniform map: equal-mass inverse CDF
+    xr = np.concatenate([[0.0], np.sort(rng.random(n - 1)), [1.0]])
+    for alpha in (0.5, 1.0, 1.5):
+        w = rng.random(n) ** 3 * 10
+        c = rng.integers(1, 50, n).astype(np.uint32)
+        out = orc.refine_dim(alpha, w * c, c, xr)
+        ref = _inverse_cdf_edges(alpha, w * c, c, xr)
+        assert np.max(np.abs(out - ref)) <= 1e-12
+        assert np.all(np.diff(out) > 0)
+    # zero weights: map unchanged (the reference would produce NaN)
+    assert np.array_equal(orc.refine_dim(0.5, np.zeros(n), np.zeros(n, dtype=np.uint32), xr), xr)
+
+
+def test_run_closed_form_gaussian(orc):
+    """4-D normalised Gaussian (sigma = 0.1): pull within 4 sigma, chi2/dof < 3."""
+    d, n, ns, N, iters = 4, 256, 6, 20000, 10
+    sigma, mu = 0.1, 0.5
+    norm = (1.0 / (sigma * math.sqrt(2 * math.pi))) ** d
+    p, keep = orc.make_params(d, n, ns, N, iters, orc.F_GAUSS, (mu, sigma, norm), seed=0)
+    r = orc.run(p, keep, ns ** d)
+    exact = (math.erf(0.5 / (sigma * math.sqrt(2)))) ** d
+    assert abs(r["result"] - exact) < 4 * r["error"]
+    assert r["chi2_dof"] < 3.0
+    assert r["error"] / exact < 2e-3
+    assert r["tot_nevals"] == r["hist_n"].sum()
+    assert np.all(r["nh"] >= 2)
+    assert np.all(np.diff(r["x_edges"], axis=1) > 0)
+
+
+def test_run_polynomial_fp32_and_domain(orc):
+    d, n, ns, N, iters = 3, 64, 5, 5000, 8
+    for prec in (0, 1):
+        p, keep = orc.make_params(d, n, ns, N, iters, orc.F_POLY, precision=prec, seed=7)
+        r = orc.run(p, keep, ns ** d)
+        assert abs(r["result"] - 1.0) < 4 * r["error"]
+    # linear over [1,3] x [0,2] x [-1,0]: integral = vol * mean
+    dom = np.array([1.0, 0.0, -1.0, 2.0, 2.0, 1.0])
+    p, keep = orc.make_params(d, n, ns, N, iters, orc.F_LINEAR, seed=3, domain=dom)
+    r = orc.run(p, keep, ns ** d)
+    exact = 4.0 * (2.0 + 1.0 - 0.5)
+    assert abs(r["result"] - exact) < 4 * r["error"] + 1e-12
