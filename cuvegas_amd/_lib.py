@@ -116,6 +116,9 @@ SIGNATURES = {
     "cvg_state_refine": (C.c_int, [_P]),
     "cvg_state_reallocate": (C.c_int, [_P, _PD, C.c_double]),
     "cvg_state_dump": (C.c_int, [_P, C.c_int, C.c_longlong, _PLL, _PI, _PD, _PD]),
+    "cvg_state_set_record": (C.c_int, [_P, C.c_int]),
+    "cvg_state_get_snapshot": (C.c_int, [_P, C.c_int, _PD, _PI]),
+    "cvg_state_backward": (C.c_int, [_P, _PD, _PD, _PD, _PD, _PD]),
     "cvg_integrate_cuda": (C.c_int, [C.c_longlong, C.c_int, C.c_int, C.c_longlong, C.c_int, C.c_int,
                                      C.c_int, C.c_double, C.c_double, C.c_uint, C.c_int, C.c_int,
                                      C.c_int, C.c_int, C.c_double, C.c_char_p, C.c_char_p,

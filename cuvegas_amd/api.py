@@ -237,6 +237,30 @@ class Integrator:
         check(lib.cvg_state_dump(self._h, int(it), int(n), _llptr(cube), _iptr(ids), _dptr(x), _dptr(jf)))
         return cube, ids, x, jf
 
+    # -- differentiation -------------------------------------------------------
+    def set_record(self, enable: bool = True) -> None:
+        """Keep the map and allocation of every iteration >= skip for backward()."""
+        check(lib.cvg_state_set_record(self._h, 1 if enable else 0))
+
+    def snapshot(self, it: int):
+        x = np.zeros((self.n_dim, self.n_intervals + 1))
+        nh = np.zeros(self.n_cubes, dtype=np.int32)
+        check(lib.cvg_state_get_snapshot(self._h, int(it), _dptr(x), _iptr(nh)))
+        return x, nh
+
+    def backward(self, theta=None):
+        """Replay the recorded iterations: returns (value, value_var, grad, grad_var)
+        of the frozen estimator at theta (default: the forward's parameters)."""
+        P = self.params.n_theta
+        g = np.zeros(P)
+        gv = np.zeros(P)
+        v = C.c_double()
+        vv = C.c_double()
+        th = None if theta is None else np.ascontiguousarray(theta, dtype=np.float64)
+        check(lib.cvg_state_backward(self._h, None if th is None else _dptr(th), _dptr(g), _dptr(gv),
+                                     C.byref(v), C.byref(vv)))
+        return v.value, vv.value, g, gv
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             lib.cvg_state_destroy(self._h)

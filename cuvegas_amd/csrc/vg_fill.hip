@@ -124,11 +124,15 @@ __device__ inline void vg_eval(vg_sample& o, long long s, int h, unsigned it, un
 }
 
 #if VG_FP32
-typedef float vg_bin_t;
 typedef float2 vg_map_t;
 #else
-typedef double vg_bin_t;
 typedef double2 vg_map_t;
+#endif
+// Per-block bin partials: FP64, or FP32 when VG_BIN32 is set (FP32 form only).
+#if VG_BIN32
+typedef float vg_bin_t;
+#else
+typedef double vg_bin_t;
 #endif
 
 extern "C" __global__ void __launch_bounds__(VG_BLOCK)
@@ -280,3 +284,116 @@ vg_dump(const vg_map_t* __restrict__ gmap, const long long* __restrict__ nh_sum,
     }
     jf[s] = o.jf;
 }
+
+#if VG_GRAD
+// ---------------------------------------------------------------------------
+// Backward replay: d I_k / d theta on a frozen iteration (map, allocation and
+// Philox counters of forward iteration it).  The integrand supplies
+//   vg_real integrand_grad(vg_real* x, const vg_real* theta, vg_real* grad)
+// returning f and writing df/dtheta[0..VG_NTHETA).  Samples are regenerated,
+// not stored.  One lane walks whole cubes (grid-stride), so per-cube sums of
+// J df/dtheta and their squares give the stratified variance of the gradient:
+//   G_p   = sum_h V/n_h sum_s J df_p
+//   Var_p = sum_h (V^2 sum_s (J df_p)^2 / n_h - (V sum_s J df_p / n_h)^2) / (n_h - 1)
+// (unbiased per-cube sample variance; n_h >= 2 always)
+// Also replays I_k and sigma_k^2 with the forward's expressions (results
+// kernel) so value(theta) of the frozen estimator is available for finite
+// differences.  part[block][2 + 2P]: fixed-order block partials.
+// ---------------------------------------------------------------------------
+constexpr int NT = VG_NTHETA;
+
+extern "C" __global__ void __launch_bounds__(256)
+vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
+        const long long* __restrict__ nh_sum, int C, unsigned it, double V,
+        const double* __restrict__ domain, double domain_vol, const vg_real* __restrict__ theta,
+        unsigned seed, unsigned stream, double* __restrict__ part) {
+    constexpr int NV = 2 + 2 * NT;
+    __shared__ double red[256];
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+    for (long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x; h < C;
+         h += (long long)gridDim.x * blockDim.x) {
+        const int n = nh[h];
+        const long long s0 = nh_sum[h];
+        double sf = 0.0, sf2 = 0.0, g1[NT], g2[NT];
+#pragma unroll
+        for (int p = 0; p < NT; ++p) { g1[p] = 0.0; g2[p] = 0.0; }
+        for (int e = 0; e < n; ++e) {
+            unsigned words[4 * NG];
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const vg_u4 v = vg_sample_words((unsigned long long)(s0 + e), it, (unsigned)g, seed, stream);
+                words[4 * g + 0] = v.x; words[4 * g + 1] = v.y;
+                words[4 * g + 2] = v.z; words[4 * g + 3] = v.w;
+            }
+            unsigned tmp = (unsigned)h;
+            vg_real x[D], dxv[D];
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                const unsigned q = tmp / NS;
+                const unsigned k = tmp - q * NS;
+                tmp = q;
+                const vg_binpos b = vg_bin_of(words[i], k * (unsigned)NB, (unsigned)NB, NS);
+#if VG_FP32
+                constexpr float inv_ns = 1.0f / (float)NS;
+                const float frac = ((float)b.rem + (float)(unsigned)(b.low >> 9) * 0x1p-24f) * inv_ns;
+                const float2 m = gmap[i * NB + b.id];
+                x[i] = fmaf(m.y, frac, m.x);
+#else
+                constexpr double inv_ns = 1.0 / (double)NS;
+                const double frac = ((double)b.rem + (double)b.low * 0x1p-33) * inv_ns;
+                const double2 m = gmap[i * NB + b.id];
+                x[i] = fma(m.y, frac, m.x);
+#endif
+                dxv[i] = m.y;
+            }
+            vg_real jac = (vg_real)1;
+#pragma unroll
+            for (int i = 0; i < D; ++i) jac = jac * ((vg_real)NB * dxv[i]);
+#if VG_HAS_DOMAIN
+#pragma unroll
+            for (int i = 0; i < D; ++i) x[i] = x[i] * (vg_real)domain[D + i] + (vg_real)domain[i];
+#endif
+            vg_real gr[NT];
+            vg_real f = integrand_grad(x, theta, gr);
+#if VG_HAS_DOMAIN
+            f = (vg_real)domain_vol * f;
+#pragma unroll
+            for (int p = 0; p < NT; ++p) gr[p] = (vg_real)domain_vol * gr[p];
+#endif
+            const double jf = (double)(f * jac);
+            sf = sf + jf;
+            sf2 = sf2 + jf * jf;
+#pragma unroll
+            for (int p = 0; p < NT; ++p) {
+                const double gj = (double)(gr[p] * jac);
+                g1[p] = g1[p] + gj;
+                g2[p] = g2[p] + gj * gj;
+            }
+        }
+        const double Ih = sf / n * V;
+        const double Sig2 = (sf2 / n * V * V) - (Ih * Ih);
+        acc[0] = acc[0] + Ih;
+        acc[1] = acc[1] + Sig2 / n;
+#pragma unroll
+        for (int p = 0; p < NT; ++p) {
+            const double Gh = g1[p] / n * V;
+            const double Gv = (g2[p] / n * V * V) - (Gh * Gh);
+            acc[2 + p] = acc[2 + p] + Gh;
+            acc[2 + NT + p] = acc[2 + NT + p] + Gv / (n - 1);
+        }
+    }
+    // fixed-order block reduction, one quantity at a time
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        __syncthreads();
+        red[threadIdx.x] = acc[q];
+        for (int stride = 128; stride >= 1; stride >>= 1) {
+            __syncthreads();
+            if ((int)threadIdx.x < stride) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + stride];
+        }
+        if (threadIdx.x == 0) part[(long long)blockIdx.x * NV + q] = red[0];
+    }
+}
+#endif

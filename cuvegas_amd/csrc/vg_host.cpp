@@ -72,7 +72,8 @@ int validate(const cvg_params* p) {
         return fail(CVG_ERR_INVALID, "n_strat^n_dim must be < 2^31");
     if (p->precision != CVG_FP64 && p->precision != CVG_FP32C)
         return fail(CVG_ERR_INVALID, "precision must be CVG_FP64 or CVG_FP32C");
-    if (p->run_len < 0 || p->run_len > 4096) return fail(CVG_ERR_INVALID, "run_len must be in 0..4096");
+    if (p->run_len < 0 || p->run_len > 4096 || (p->run_len & (p->run_len - 1)))
+        return fail(CVG_ERR_INVALID, "run_len must be 0 or a power of two up to 4096");
     if (p->has_domain && !p->domain) return fail(CVG_ERR_INVALID, "has_domain set but domain is NULL");
     if (p->n_theta < 0) return fail(CVG_ERR_INVALID, "n_theta must be >= 0");
     return CVG_OK;
@@ -86,13 +87,14 @@ int validate(const cvg_params* p) {
 struct cvg_program {
     int d = 0, nbins = 0, nstrat = 0, precision = 0, K = kDefaultRun, has_domain = 0;
     int block = kFillBlock, lds_bins = 0, lds_map = 0;
+    int has_grad = 0, n_theta = 0, bin32 = 0;
     std::string log;
     std::vector<char> code;
     double compile_ms = 0.0;
     // module loaded on first use, on the device current at that time
     int device = -1;
     hipModule_t mod = nullptr;
-    hipFunction_t fill = nullptr, dump = nullptr;
+    hipFunction_t fill = nullptr, dump = nullptr, grad = nullptr;
     int fill_blocks_per_cu = 0;
 
     int load() {
@@ -104,6 +106,7 @@ struct cvg_program {
         VG_HIP(hipModuleLoadData(&mod, code.data()));
         VG_HIP(hipModuleGetFunction(&fill, mod, "vg_fill"));
         VG_HIP(hipModuleGetFunction(&dump, mod, "vg_dump"));
+        if (has_grad) VG_HIP(hipModuleGetFunction(&grad, mod, "vg_grad"));
         VG_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&fill_blocks_per_cu, fill, block, 0));
         if (fill_blocks_per_cu < 1) fill_blocks_per_cu = 1;
         device = dev;
@@ -135,7 +138,7 @@ struct cvg_state {
     cvg_params p{};
     std::vector<double> domain_host;
     long long C = 0, nb = 0, s_cap = 0, run_cap = 0, win_cap = 0;
-    int K = kDefaultRun;
+    int K = kDefaultRun, k_shift = 4;
     double V = 0.0;
     int nh0 = 2;
     hipStream_t stream = nullptr;
@@ -168,6 +171,20 @@ struct cvg_state {
     void* theta = nullptr;
     int iterations_done = 0;
     double setup_ms = 0.0, loop_ms = 0.0;
+    // recorded forward iterations (for the backward replay): map edges and n_h
+    // of every iteration >= skip, written by a kernel that reads the counter
+    int record = 0;
+    double* snap_x = nullptr;   // [max_it - skip][d*(n+1)]
+    int* snap_nh = nullptr;     // [max_it - skip][C]
+    // backward scratch
+    long long* bw_nh_sum = nullptr;
+    long long* bw_s_total = nullptr;
+    double* bw_x = nullptr;
+    double2* bw_map64 = nullptr;
+    float2* bw_map32 = nullptr;
+    double* bw_part = nullptr;
+    int bw_grid = 0;
+    std::vector<char> recorded;  // per iteration: snapshot taken
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // optional per-iteration timing (eager launches): fill and whole iteration
     int timing = 0;
@@ -184,7 +201,8 @@ struct cvg_state {
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         void* bufs[] = {x_edges, map64, map32, weights, counts, refine_scratch, nh, nh_sum, dh,
                         JF, JF2, run_cube, win_first, win_last, part, bsum, boff, s_total,
-                        it_p, dsum, err, hist, domain, theta};
+                        it_p, dsum, err, hist, domain, theta, snap_x, snap_nh, bw_nh_sum,
+                        bw_s_total, bw_x, bw_map64, bw_map32, bw_part};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -220,14 +238,14 @@ struct cvg_state {
 
     int launch_results() {
         VG_HIP(vg_launch_results(stream, (int)C, V, p.beta, nh, nh_sum, JF, JF2, win_first, win_last,
-                                 64LL * K, dh, part));
+                                 6 + k_shift, dh, part));
         VG_HIP(vg_launch_finalize(stream, nb, part, s_total, it_p, p.max_it, hist, dsum, err));
         return CVG_OK;
     }
 
     int launch_realloc(int mode, int advance) {
         VG_HIP(vg_launch_alloc(stream, (int)C, mode, nh0, dh, dsum, p.n_eval_it, nh, bsum));
-        VG_HIP(vg_launch_scan(stream, (int)C, nh, bsum, boff, s_total, s_cap, it_p, advance, K,
+        VG_HIP(vg_launch_scan(stream, (int)C, nh, bsum, boff, s_total, s_cap, it_p, advance, k_shift,
                               run_cap, nh_sum, run_cube, err));
         return CVG_OK;
     }
@@ -238,6 +256,9 @@ struct cvg_state {
                           hipEvent_t e_end = nullptr) {
         int rc;
         if (e_start) VG_HIP(hipEventRecord(e_start, stream));
+        if (record)
+            VG_HIP(vg_launch_snapshot(stream, it_p, p.skip, p.max_it, (int)C, p.n_dim * (p.n_intervals + 1),
+                                      x_edges, nh, snap_x, snap_nh));
         if ((rc = launch_fill())) return rc;
         if (e_fill) VG_HIP(hipEventRecord(e_fill, stream));
         if ((rc = launch_refine())) return rc;
@@ -261,7 +282,7 @@ struct cvg_state {
 
 // Layout (nh_sum, run_cube) from the n_h already on the device.
 int cvg_state::launch_realloc_copy() {
-    VG_HIP(vg_launch_layout(stream, (int)C, nh, bsum, boff, s_total, s_cap, it_p, K, run_cap, nh_sum,
+    VG_HIP(vg_launch_layout(stream, (int)C, nh, bsum, boff, s_total, s_cap, it_p, k_shift, run_cap, nh_sum,
                             run_cube, err));
     return CVG_OK;
 }
@@ -303,12 +324,49 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
     prog->precision = shape->precision;
     prog->K = shape->run_len > 0 ? shape->run_len : kDefaultRun;
     prog->has_domain = shape->has_domain ? 1 : 0;
+    prog->n_theta = shape->n_theta;
     const long long dn = (long long)prog->d * prog->nbins;
-    const long long bin_bytes = dn * (prog->precision == CVG_FP32C ? 8 : 12);
+    long long bin_bytes = dn * 12;
     const long long map_bytes = prog->precision == CVG_FP32C ? dn * 8 : 0;
     prog->lds_bins = bin_bytes <= kLdsBytes;
     prog->lds_map = prog->precision == CVG_FP32C && prog->lds_bins && bin_bytes + map_bytes <= kLdsBytes;
 
+    // Launch-shape knobs in the options string (consumed here, not passed to
+    // hipRTC): lds_bins=0|1, lds_map=0|1, block=64..1024 (multiple of 64).
+    std::string rtc_extra;
+    if (options) {
+        std::istringstream ss(options);
+        std::string tok;
+        while (ss >> tok) {
+            const size_t eq = tok.find('=');
+            const std::string key = eq == std::string::npos ? tok : tok.substr(0, eq);
+            const int val = eq == std::string::npos ? 1 : std::atoi(tok.c_str() + eq + 1);
+            if (key == "lds_bins") {
+                if (val && bin_bytes > kLdsBytes) return fail(CVG_ERR_INVALID, "bins do not fit in LDS");
+                prog->lds_bins = val ? 1 : 0;
+                if (!prog->lds_bins) prog->lds_map = 0;
+            } else if (key == "lds_map") {
+                if (val && (prog->precision != CVG_FP32C || !prog->lds_bins || bin_bytes + map_bytes > kLdsBytes))
+                    return fail(CVG_ERR_INVALID, "map does not fit in LDS next to the bins");
+                prog->lds_map = val ? 1 : 0;
+            } else if (key == "bin32") {
+                if (val && prog->precision != CVG_FP32C) return fail(CVG_ERR_INVALID, "bin32 is for the FP32 form");
+                prog->bin32 = val ? 1 : 0;
+                bin_bytes = dn * (prog->bin32 ? 8 : 12);
+                prog->lds_bins = bin_bytes <= kLdsBytes;
+                prog->lds_map = prog->precision == CVG_FP32C && prog->lds_bins && bin_bytes + map_bytes <= kLdsBytes;
+            } else if (key == "grad") {
+                if (shape->n_theta < 1) return fail(CVG_ERR_INVALID, "grad needs n_theta >= 1");
+                prog->has_grad = val ? 1 : 0;
+            } else if (key == "block") {
+                if (val < 64 || val > 1024 || val % 64) return fail(CVG_ERR_INVALID, "block must be 64..1024, multiple of 64");
+                prog->block = val;
+            } else {
+                rtc_extra += tok + " ";
+            }
+        }
+    }
+    options = rtc_extra.c_str();
     const double t0 = now_ms();
     hiprtcProgram rtc;
     const char* hdr_src[2] = {kVgPhiloxSrc, integrand_src};
@@ -324,12 +382,16 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
         "-DVG_FP32=" + std::to_string(prog->precision == CVG_FP32C ? 1 : 0),
         "-DVG_LDS_BINS=" + std::to_string(prog->lds_bins),
         "-DVG_LDS_MAP=" + std::to_string(prog->lds_map),
-        "-DVG_HAS_DOMAIN=" + std::to_string(prog->has_domain)};
+        "-DVG_HAS_DOMAIN=" + std::to_string(prog->has_domain),
+        "-DVG_GRAD=" + std::to_string(prog->has_grad),
+        "-DVG_BIN32=" + std::to_string(prog->bin32),
+        "-DVG_NTHETA=" + std::to_string(prog->n_theta)};
     if (options) {
         std::istringstream ss(options);
         std::string tok;
         while (ss >> tok) opts.push_back(tok);
     }
+
     std::vector<const char*> copts;
     for (auto& o : opts) copts.push_back(o.c_str());
     hiprtcResult cr = hiprtcCompileProgram(rtc, (int)copts.size(), copts.data());
@@ -379,7 +441,7 @@ int cvg_state_create(cvg_program* prog, const cvg_params* params, cvg_state** ou
     if (rc) return rc;
     const int K = params->run_len > 0 ? params->run_len : kDefaultRun;
     if (params->n_dim != prog->d || params->n_intervals != prog->nbins || params->n_strat != prog->nstrat ||
-        params->precision != prog->precision || K != prog->K ||
+        params->precision != prog->precision || K != prog->K || params->n_theta != prog->n_theta ||
         (params->has_domain ? 1 : 0) != prog->has_domain)
         return fail(CVG_ERR_INVALID, "params do not match the program's compiled shape");
     if (params->max_it < 1) return fail(CVG_ERR_INVALID, "max_it must be >= 1");
@@ -394,6 +456,8 @@ int cvg_state_create(cvg_program* prog, const cvg_params* params, cvg_state** ou
     st->prog = prog;
     st->p = *params;
     st->K = K;
+    st->k_shift = 0;
+    while ((1 << st->k_shift) < K) ++st->k_shift;
     VG_HIP(hipGetDevice(&st->device));
     if (params->has_domain) {
         st->domain_host.assign(params->domain, params->domain + 2 * params->n_dim);
@@ -456,6 +520,7 @@ int cvg_state_create(cvg_program* prog, const cvg_params* params, cvg_state** ou
     VG_HIP(vg_launch_init_map(s, d, n, st->x_edges, st->map64, st->map32, st->weights, st->counts));
     if ((rc = st->launch_realloc(0, 0))) return rc;
     VG_HIP(hipStreamSynchronize(s));
+    st->recorded.assign((size_t)params->max_it, 0);
     st->setup_ms = now_ms() - t0;
     *out = st.release();
     return CVG_OK;
@@ -544,6 +609,10 @@ int cvg_state_run(cvg_state* st, int n_iter, int use_graph) {
     float ms = 0.0f;
     VG_HIP(hipEventElapsedTime(&ms, st->ev0, st->ev1));
     st->loop_ms += ms;
+    for (int i = 0; i < n_iter; ++i) {
+        const int k = st->iterations_done + i;
+        if (k < (int)st->recorded.size()) st->recorded[k] = (char)(st->record && k >= st->p.skip);
+    }
     st->iterations_done += n_iter;
     if (timed) {
         for (int i = 0; i < n_iter; ++i) {
@@ -768,6 +837,123 @@ int cvg_state_dump(cvg_state* st, int it, long long n, long long* cube, int* ids
     (void)hipFree(d_cube); (void)hipFree(d_ids); (void)hipFree(d_x); (void)hipFree(d_jf);
     if (e != hipSuccess) return fail(CVG_ERR_HIP, std::string("dump: ") + hipGetErrorString(e));
     return CVG_OK;
+}
+
+
+int cvg_state_set_record(cvg_state* st, int enable) {
+    if (!st) return fail(CVG_ERR_INVALID, "state is NULL");
+    if (enable && !st->snap_x) {
+        const long long slots = st->p.max_it - st->p.skip;
+        if (slots < 1) return fail(CVG_ERR_INVALID, "nothing to record: skip >= max_it");
+        const long long nx = (long long)st->p.n_dim * (st->p.n_intervals + 1);
+        int rc;
+        if ((rc = dalloc(&st->snap_x, slots * nx))) return rc;
+        if ((rc = dalloc(&st->snap_nh, slots * st->C))) return rc;
+    }
+    if ((enable ? 1 : 0) != st->record) {
+        if (st->graph_exec) { (void)hipGraphExecDestroy(st->graph_exec); st->graph_exec = nullptr; }
+        if (st->graph) { (void)hipGraphDestroy(st->graph); st->graph = nullptr; }
+    }
+    st->record = enable ? 1 : 0;
+    return CVG_OK;
+}
+
+int cvg_state_get_snapshot(cvg_state* st, int it, double* x_edges, int* nh) {
+    if (!st || !x_edges || !nh) return fail(CVG_ERR_INVALID, "NULL argument");
+    if (it < 0 || it >= st->iterations_done || !st->recorded[it])
+        return fail(CVG_ERR_INVALID, "iteration was not recorded");
+    const long long slot = it - st->p.skip;
+    const long long nx = (long long)st->p.n_dim * (st->p.n_intervals + 1);
+    int rc = copy_d2h(st, x_edges, st->snap_x + slot * nx, sizeof(double) * nx);
+    if (rc) return rc;
+    return copy_d2h(st, nh, st->snap_nh + slot * st->C, sizeof(int) * st->C);
+}
+
+int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double* grad_var,
+                       double* value, double* value_var) {
+    if (!st) return fail(CVG_ERR_INVALID, "state is NULL");
+    cvg_program* prog = st->prog;
+    if (!prog->has_grad || !prog->grad)
+        return fail(CVG_ERR_INVALID, "program was compiled without the gradient kernel (option grad)");
+    const int P = st->p.n_theta, d = st->p.n_dim, n = st->p.n_intervals;
+    const int NV = 2 + 2 * P;
+    int rc;
+    std::vector<int> its;
+    for (int k = st->p.skip; k < st->iterations_done; ++k)
+        if (st->recorded[k]) its.push_back(k);
+    if (its.empty()) return fail(CVG_ERR_INVALID, "no recorded iterations: enable recording before running");
+    if (!st->bw_part) {
+        int ncu = 0;
+        VG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, st->device));
+        st->bw_grid = ncu * 4;
+        if ((rc = dalloc(&st->bw_part, (long long)st->bw_grid * NV))) return rc;
+        if ((rc = dalloc(&st->bw_nh_sum, st->C + 1))) return rc;
+        if ((rc = dalloc(&st->bw_s_total, 1))) return rc;
+        if ((rc = dalloc(&st->bw_x, (long long)d * (n + 1)))) return rc;
+        if ((rc = dalloc(&st->bw_map64, (long long)d * n))) return rc;
+        if ((rc = dalloc(&st->bw_map32, (long long)d * n))) return rc;
+    }
+    // parameters for the replay: given theta, or the forward's
+    void* th = st->theta;
+    void* tmp_theta = nullptr;
+    if (theta) {
+        VG_HIP(hipMalloc(&tmp_theta, 8 * (size_t)P));
+        if (st->p.precision == CVG_FP32C) {
+            std::vector<float> f(theta, theta + P);
+            VG_HIP(hipMemcpyAsync(tmp_theta, f.data(), 4 * (size_t)P, hipMemcpyHostToDevice, st->stream));
+            VG_HIP(hipStreamSynchronize(st->stream));
+        } else {
+            VG_HIP(hipMemcpyAsync(tmp_theta, theta, 8 * (size_t)P, hipMemcpyHostToDevice, st->stream));
+        }
+        th = tmp_theta;
+    }
+    std::vector<double> hist(4 * (size_t)st->iterations_done);
+    VG_HIP(hipMemcpyAsync(hist.data(), st->hist, sizeof(double) * hist.size(), hipMemcpyDeviceToHost, st->stream));
+    const long long nx = (long long)d * (n + 1);
+    std::vector<double> part((size_t)st->bw_grid * NV);
+    double W = 0.0, val = 0.0, vvar = 0.0;
+    std::vector<double> g((size_t)P, 0.0), gv((size_t)P, 0.0);
+    int C = (int)st->C;
+    double vol = 1.0;
+    if (st->p.has_domain)
+        for (int i = 0; i < d; ++i) vol *= st->domain_host[d + i];
+    unsigned seed = st->p.seed, sid = st->p.stream_id;
+    for (int k : its) {
+        const long long slot = k - st->p.skip;
+        VG_HIP(hipMemcpyAsync(st->bw_x, st->snap_x + slot * nx, sizeof(double) * nx, hipMemcpyDeviceToDevice, st->stream));
+        VG_HIP(vg_launch_tables(st->stream, d, n, st->bw_x, st->bw_map64, st->bw_map32));
+        const int* snh = st->snap_nh + slot * st->C;
+        VG_HIP(vg_launch_layout(st->stream, C, snh, st->bsum, st->boff, st->bw_s_total, (1LL << 62), st->it_p,
+                                st->k_shift, 0, st->bw_nh_sum, nullptr, st->err));
+        const void* map = st->p.precision == CVG_FP32C ? (const void*)st->bw_map32 : (const void*)st->bw_map64;
+        unsigned itu = (unsigned)k;
+        double V = st->V;
+        void* args[] = {(void*)&map, (void*)&snh, (void*)&st->bw_nh_sum, (void*)&C, (void*)&itu, (void*)&V,
+                        (void*)&st->domain, (void*)&vol, (void*)&th, (void*)&seed, (void*)&sid,
+                        (void*)&st->bw_part};
+        VG_HIP(hipModuleLaunchKernel(prog->grad, (unsigned)st->bw_grid, 1, 1, 256, 1, 1, 0, st->stream, args, nullptr));
+        VG_HIP(hipMemcpyAsync(part.data(), st->bw_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st->stream));
+        VG_HIP(hipStreamSynchronize(st->stream));
+        std::vector<double> tot((size_t)NV, 0.0);
+        for (int b = 0; b < st->bw_grid; ++b)
+            for (int q = 0; q < NV; ++q) tot[q] += part[(size_t)b * NV + q];
+        const double w = 1.0 / hist[4 * (size_t)k + 1];  // forward weight, held fixed
+        W += w;
+        val += w * tot[0];
+        vvar += w * w * tot[1];
+        for (int q = 0; q < P; ++q) {
+            g[q] += w * tot[2 + q];
+            gv[q] += w * w * tot[2 + P + q];
+        }
+    }
+    if (tmp_theta) (void)hipFree(tmp_theta);
+    if (value) *value = val / W;
+    if (value_var) *value_var = vvar / (W * W);
+    for (int q = 0; q < P; ++q) {
+        if (grad) grad[q] = g[q] / W;
+        if (grad_var) grad_var[q] = gv[q] / (W * W);
+    }
+    return st->check_err();
 }
 
 int cvg_integrate_cuda(long long tot_eval, int max_it, int skip, long long max_batch_size,

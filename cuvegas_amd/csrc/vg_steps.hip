@@ -21,91 +21,161 @@ __device__ inline void vg_tree1024(double* t) {
     __syncthreads();
 }
 
+
+// Exclusive scan of one int64 per thread over the block (wave64 shuffles, then
+// the wave totals); *total receives the block sum.  wt: 16 int64 of LDS.
+__device__ inline long long vg_block_scan_excl(long long v, long long* wt, long long* total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    long long x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    __syncthreads();
+    if (lane == 63) wt[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        long long t = lane < nw ? wt[lane] : 0;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const long long y = __shfl_up(t, off, 64);
+            if (lane >= off) t += y;
+        }
+        if (lane < nw) wt[lane] = t;
+    }
+    __syncthreads();
+    const long long before = wid > 0 ? wt[wid - 1] : 0;
+    *total = wt[nw - 1];
+    return before + x - v;
+}
+
+
+// The 1024-wide strided pairwise tree (t[i] += t[i+512], then +256, ... +1)
+// evaluated by 256 threads that each hold the elements t, t+256, t+512, t+768:
+// the first two levels in registers, the next two through LDS, the last six
+// with wave shuffles.  Bit-identical to the element-wise definition.  red:
+// 256 doubles of LDS; the total is returned to thread 0.
+__device__ inline double vg_tree1024_256(double v0, double v1, double v2, double v3, double* red) {
+    const double a = v0 + v2;       // stride 512: i = t
+    const double b = v1 + v3;       // stride 512: i = t + 256
+    double x = a + b;               // stride 256: i = t
+    __syncthreads();
+    red[threadIdx.x] = x;
+    __syncthreads();
+    if (threadIdx.x < 128) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + 128];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        x = red[threadIdx.x] + red[threadIdx.x + 64];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double y = __shfl_down(x, off, 64);
+            x = x + y;  // lanes >= off compute values that are never read
+        }
+    }
+    return x;
+}
+
 // ---------------------------------------------------------------------------
 // Per-cube results (reference `results`, cuda_wrapper.cu:26-49).  Also folds
 // the window partials of cubes that crossed sampling windows and writes the
 // complete per-cube sums back to JF / JF2.  part[3b+q]: block partial of
 // q = 0: sum I_h, 1: sum sigma_h^2 / n_h, 2: sum d_h.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(VG_TREE)
+__device__ inline void vg_cube_result(long long h, int C, double V, double beta,
+                                      const int* __restrict__ nh, const long long* __restrict__ nh_sum,
+                                      double* __restrict__ JF, double* __restrict__ JF2,
+                                      const double2* __restrict__ win_first,
+                                      const double2* __restrict__ win_last, int ws_shift,
+                                      double* __restrict__ dh, double& sr, double& ss, double& dd) {
+    sr = 0.0; ss = 0.0; dd = 0.0;
+    if (h >= C) return;
+    const int neval = nh[h];
+    const long long st = nh_sum[h], en = nh_sum[h + 1];
+    const long long wa = st >> ws_shift, wb = (en - 1) >> ws_shift;
+    double j1, j2;
+    if (wa == wb) {
+        j1 = JF[h];
+        j2 = JF2[h];
+    } else {  // left fold of the window partials; loads batched ahead of the adds
+        double2 v = win_last[wa];
+        j1 = v.x;
+        j2 = v.y;
+        long long w = wa + 1;
+        for (; w + 8 <= wb + 1; w += 8) {
+            double2 t[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) t[j] = win_first[w + j];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) { j1 = j1 + t[j].x; j2 = j2 + t[j].y; }
+        }
+        for (; w <= wb; ++w) {
+            v = win_first[w];
+            j1 = j1 + v.x;
+            j2 = j2 + v.y;
+        }
+        JF[h] = j1;
+        JF2[h] = j2;
+    }
+    const double Ih = j1 / neval * V;
+    const double Sig2 = (j2 / neval * V * V) - (Ih * Ih);
+    sr = Ih;
+    ss = Sig2 / neval;
+    const double u = V * j1 / neval;
+    double t = V * V * j2 / neval - u * u;
+    if (t < 0) t = 0;
+    dd = vg_pow(t, beta);
+    dh[h] = dd;
+}
+
+__global__ void __launch_bounds__(256)
 vg_results_kernel(int C, double V, double beta, const int* __restrict__ nh,
                   const long long* __restrict__ nh_sum, double* __restrict__ JF,
                   double* __restrict__ JF2, const double2* __restrict__ win_first,
-                  const double2* __restrict__ win_last, long long WS, double* __restrict__ dh,
+                  const double2* __restrict__ win_last, int ws_shift, double* __restrict__ dh,
                   double* __restrict__ part) {
-    __shared__ double t0[VG_TREE], t1[VG_TREE], t2[VG_TREE];
-    const long long h = (long long)blockIdx.x * VG_TREE + threadIdx.x;
-    double sr = 0.0, ss = 0.0, dd = 0.0;
-    if (h < C) {
-        const int neval = nh[h];
-        const long long st = nh_sum[h], en = nh_sum[h + 1];
-        const long long wa = st / WS, wb = (en - 1) / WS;
-        double j1, j2;
-        if (wa == wb) {
-            j1 = JF[h];
-            j2 = JF2[h];
-        } else {
-            double2 v = win_last[wa];
-            j1 = v.x;
-            j2 = v.y;
-            for (long long w = wa + 1; w <= wb; ++w) {
-                v = win_first[w];
-                j1 = j1 + v.x;
-                j2 = j2 + v.y;
-            }
-            JF[h] = j1;
-            JF2[h] = j2;
-        }
-        const double Ih = j1 / neval * V;
-        const double Sig2 = (j2 / neval * V * V) - (Ih * Ih);
-        sr = Ih;
-        ss = Sig2 / neval;
-        const double u = V * j1 / neval;
-        double t = V * V * j2 / neval - u * u;
-        if (t < 0) t = 0;
-        dd = vg_pow(t, beta);
-        dh[h] = dd;
-    }
-    t0[threadIdx.x] = sr;
-    t1[threadIdx.x] = ss;
-    t2[threadIdx.x] = dd;
-    for (int stride = VG_TREE / 2; stride >= 1; stride >>= 1) {
-        __syncthreads();
-        if ((int)threadIdx.x < stride) {
-            t0[threadIdx.x] = t0[threadIdx.x] + t0[threadIdx.x + stride];
-            t1[threadIdx.x] = t1[threadIdx.x] + t1[threadIdx.x + stride];
-            t2[threadIdx.x] = t2[threadIdx.x] + t2[threadIdx.x + stride];
-        }
-    }
-    if (threadIdx.x == 0) {
-        part[3 * blockIdx.x + 0] = t0[0];
-        part[3 * blockIdx.x + 1] = t1[0];
-        part[3 * blockIdx.x + 2] = t2[0];
+    __shared__ double red[256];
+    const long long base = (long long)blockIdx.x * VG_TREE + threadIdx.x;
+    double r[4][3];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        vg_cube_result(base + 256 * j, C, V, beta, nh, nh_sum, JF, JF2, win_first, win_last, ws_shift,
+                       dh, r[j][0], r[j][1], r[j][2]);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double tot = vg_tree1024_256(r[0][q], r[1][q], r[2][q], r[3][q], red);
+        if (threadIdx.x == 0) part[3 * blockIdx.x + q] = tot;
     }
 }
 
 // Remaining tree levels over the block partials, then the iteration record:
 // hist[4*it + {0,1,2,3}] = {I_k, sigma_k^2, D, samples}; *dsum = D.
-__global__ void __launch_bounds__(VG_TREE)
+__global__ void __launch_bounds__(256)
 vg_finalize_kernel(long long nparts, double* __restrict__ part, const long long* __restrict__ s_total,
                    const int* __restrict__ it_p, int hist_cap, double* __restrict__ hist,
                    double* __restrict__ dsum, int* __restrict__ err) {
-    __shared__ double t[VG_TREE];
+    __shared__ double red[256];
     long long cur = nparts;
     while (cur > 1) {
         const long long nch = (cur + VG_TREE - 1) / VG_TREE;
         for (long long c = 0; c < nch; ++c) {
-            for (int q = 0; q < 3; ++q) {
-                const long long j = c * VG_TREE + threadIdx.x;
-                __syncthreads();
-                t[threadIdx.x] = j < cur ? part[3 * j + q] : 0.0;
-                vg_tree1024(t);
-                if (threadIdx.x == 0) part[3 * c + q] = t[0];
+            double v[4][3];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long i = c * VG_TREE + threadIdx.x + 256 * j;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) v[j][q] = i < cur ? part[3 * i + q] : 0.0;
             }
+            __syncthreads();  // every read of this chunk precedes the writes below
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const double tot = vg_tree1024_256(v[0][q], v[1][q], v[2][q], v[3][q], red);
+                if (threadIdx.x == 0) part[3 * c + q] = tot;
+            }
+            __syncthreads();
         }
         cur = nch;
     }
-    __syncthreads();
     if (threadIdx.x == 0) {
         const int it = *it_p;
         *dsum = part[2];
@@ -126,31 +196,35 @@ vg_finalize_kernel(long long nparts, double* __restrict__ part, const long long*
 //   n_h = nh_const                          (mode 0, initial allocation)
 // plus the per-block sums feeding the prefix sum.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(VG_TREE)
+__global__ void __launch_bounds__(256)
 vg_alloc_kernel(int C, int mode, int nh_const, const double* __restrict__ dh,
                 const double* __restrict__ dsum, long long N, int* __restrict__ nh,
                 long long* __restrict__ bsum) {
-    __shared__ long long t[VG_TREE];
-    const long long h = (long long)blockIdx.x * VG_TREE + threadIdx.x;
+    __shared__ long long red[4];
+    const long long base = (long long)blockIdx.x * VG_TREE + threadIdx.x;
+    const double D = mode ? *dsum : 1.0;
     long long v = 0;
-    if (h < C) {
-        int n;
-        if (mode == 0) {
-            n = nh_const;
-        } else {
-            const double f = dh[h] / *dsum;
-            n = vg_sat_int(f * (double)N);
-            n = n < 2 ? 2 : n;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const long long h = base + 256 * j;
+        if (h < C) {
+            int n;
+            if (mode == 0) {
+                n = nh_const;
+            } else {
+                const double f = dh[h] / D;
+                n = vg_sat_int(f * (double)N);
+                n = n < 2 ? 2 : n;
+            }
+            nh[h] = n;
+            v += n;
         }
-        nh[h] = n;
-        v = n;
     }
-    t[threadIdx.x] = v;
-    for (int stride = VG_TREE / 2; stride >= 1; stride >>= 1) {
-        __syncthreads();
-        if ((int)threadIdx.x < stride) t[threadIdx.x] += t[threadIdx.x + stride];
-    }
-    if (threadIdx.x == 0) bsum[blockIdx.x] = t[0];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 // Exclusive scan of the block sums (one block); total -> *s_total, capped at
@@ -160,57 +234,74 @@ __global__ void __launch_bounds__(VG_TREE)
 vg_scan_blocks_kernel(long long nb, const long long* __restrict__ bsum, long long* __restrict__ boff,
                       long long* __restrict__ s_total, long long s_cap, int* __restrict__ it_p,
                       int advance, int* __restrict__ err) {
-    __shared__ long long t[VG_TREE];
+    __shared__ long long wt[VG_TREE / 64];
     const long long per = (nb + VG_TREE - 1) / VG_TREE;
     const long long b0 = threadIdx.x * per;
     long long local = 0;
     for (long long b = b0; b < b0 + per && b < nb; ++b) local += bsum[b];
-    t[threadIdx.x] = local;
-    __syncthreads();
+    long long total;
+    long long acc = vg_block_scan_excl(local, wt, &total);
+    for (long long b = b0; b < b0 + per && b < nb; ++b) { boff[b] = acc; acc += bsum[b]; }
     if (threadIdx.x == 0) {
-        long long acc = 0;
-        for (int i = 0; i < VG_TREE; ++i) { const long long x = t[i]; t[i] = acc; acc += x; }
-        if (acc > s_cap) { atomicOr(err, VG_ERR_CAPACITY); acc = s_cap; }
-        *s_total = acc;
+        if (total > s_cap) { atomicOr(err, VG_ERR_CAPACITY); total = s_cap; }
+        *s_total = total;
         if (advance) *it_p = *it_p + 1;
     }
-    __syncthreads();
-    long long acc = t[threadIdx.x];
-    for (long long b = b0; b < b0 + per && b < nb; ++b) { boff[b] = acc; acc += bsum[b]; }
 }
 
 // Per-block local scan of n_h -> nh_sum (exclusive, nh_sum[C] = total), and
-// the cube holding the first sample of every run of K samples.
-__global__ void __launch_bounds__(VG_TREE)
+// the cube holding the first sample of every run of K = 2^k_shift samples.
+// 256 threads own 4 consecutive cubes each.  A cube writes its own runs; runs
+// of cubes with more than 16 of them are written by the whole block.
+__global__ void __launch_bounds__(256)
 vg_scan_cubes_kernel(int C, const int* __restrict__ nh, const long long* __restrict__ boff,
-                     const long long* __restrict__ s_total, int K, long long run_cap,
+                     const long long* __restrict__ s_total, int k_shift, long long run_cap,
                      long long* __restrict__ nh_sum, int* __restrict__ run_cube) {
-    __shared__ long long t[VG_TREE + 1];
+    __shared__ long long wt[4];
+    __shared__ int big[VG_TREE];
+    __shared__ long long big_r0[VG_TREE], big_r1[VG_TREE];
+    __shared__ int nbig;
     const long long base = (long long)blockIdx.x * VG_TREE;
-    const long long h = base + threadIdx.x;
-    const int cnt = (int)((C - base) < VG_TREE ? (C - base) : VG_TREE);
-    t[threadIdx.x] = h < C ? nh[h] : 0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        long long acc = boff[blockIdx.x];
-        for (int i = 0; i < VG_TREE; ++i) { const long long x = t[i]; t[i] = acc; acc += x; }
-        t[VG_TREE] = acc;
+    const long long h0 = base + 4 * threadIdx.x;
+    long long v[4];
+    long long own = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        v[j] = h0 + j < C ? nh[h0 + j] : 0;
+        own += v[j];
+    }
+    if (threadIdx.x == 0) nbig = 0;
+    long long total;
+    const long long S = *s_total;
+    long long ex = vg_block_scan_excl(own, wt, &total) + boff[blockIdx.x];
+    const long long kmask = (1LL << k_shift) - 1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const long long h = h0 + j;
+        if (h < C) {
+            const long long st = ex < S ? ex : S;
+            const long long en = ex + v[j] < S ? ex + v[j] : S;
+            nh_sum[h] = st;
+            if (h == C - 1) nh_sum[C] = S;
+            const long long r0 = (st + kmask) >> k_shift;
+            long long r1 = (en + kmask) >> k_shift;
+            if (r1 > run_cap) r1 = run_cap;
+            if (r1 - r0 > 16) {
+                const int slot = atomicAdd(&nbig, 1);
+                big[slot] = (int)(h - base);
+                big_r0[slot] = r0;
+                big_r1[slot] = r1;
+            } else {
+                for (long long r = r0; r < r1; ++r) run_cube[r] = (int)h;
+            }
+        }
+        ex += v[j];
     }
     __syncthreads();
-    const long long S = *s_total;
-    if (h < C) nh_sum[h] = t[threadIdx.x] < S ? t[threadIdx.x] : S;
-    if (h == C - 1) nh_sum[C] = S;
-    // runs whose first sample lies in this block's cubes
-    const long long lo = t[0], hi = t[cnt] < S ? t[cnt] : S;
-    const long long r0 = (lo + K - 1) / K, r1 = (hi + K - 1) / K;
-    for (long long r = r0 + threadIdx.x; r < r1 && r < run_cap; r += VG_TREE) {
-        const long long s = r * K;
-        int a = 0, b = cnt;  // largest a with t[a] <= s
-        while (b - a > 1) {
-            const int m = (a + b) >> 1;
-            if (t[m] <= s) a = m; else b = m;
-        }
-        run_cube[r] = (int)(base + a);
+    const int nb = nbig;
+    for (int i = 0; i < nb; ++i) {
+        const int h = (int)(base + big[i]);
+        for (long long r = big_r0[i] + threadIdx.x; r < big_r1[i]; r += 256) run_cube[r] = h;
     }
 }
 
@@ -274,23 +365,28 @@ __device__ double vg_block_tree_sum(const double* __restrict__ a, int m, double*
     return r;
 }
 
+// Refinement of one dimension per block.  Working arrays (normalised weights
+// w, damped weights d, cumulative sums, old edges) live in dynamic LDS when
+// they fit (lds != 0), else in the global scratch.
 __global__ void __launch_bounds__(VG_TREE)
-vg_refine_kernel(int n, double alpha, double* __restrict__ x_edges, double* __restrict__ weights,
-                 unsigned* __restrict__ counts, double* __restrict__ scratch,
-                 double2* __restrict__ map64, float2* __restrict__ map32) {
+vg_refine_kernel(int n, double alpha, int use_lds, double* __restrict__ x_edges,
+                 double* __restrict__ weights, unsigned* __restrict__ counts,
+                 double* __restrict__ scratch, double2* __restrict__ map64,
+                 float2* __restrict__ map32) {
     __shared__ double t[VG_TREE];
+    extern __shared__ double dyn[];
     const int dim = blockIdx.x;
     const int E = n + 1;
     double* x = x_edges + (long long)dim * E;
     double* wb = weights + (long long)dim * n;
     unsigned* cb = counts + (long long)dim * n;
-    // scratch per dim: w[n], d[n], cum[n], xo[n+1], part[n/1024 + 2]
-    double* base = scratch + (long long)dim * (4LL * n + 1 + n / VG_TREE + 2);
-    double* w = base;
-    double* d = w + n;
-    double* cum = d + n;
-    double* xo = cum + n;
-    double* part = xo + E;
+    // per dim: w[n], d[n], cum[n], xo[n+1], part[n/1024 + 2]
+    double* base = use_lds ? dyn : scratch + (long long)dim * (4LL * n + 1 + n / VG_TREE + 2);
+    double* __restrict__ w = base;
+    double* __restrict__ d = w + n;
+    double* __restrict__ cum = d + n;
+    double* __restrict__ xo = cum + n;
+    double* __restrict__ part = xo + E;
 
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         const unsigned c = cb[k];
@@ -312,9 +408,17 @@ vg_refine_kernel(int n, double alpha, double* __restrict__ x_edges, double* __re
         }
         __syncthreads();
         const double S = vg_block_tree_sum(d, n, part, t);
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0) {  // sequential inclusive sum, loads batched ahead of the adds
             double acc = 0.0;
-            for (int k = 0; k < n; ++k) { acc = acc + d[k]; cum[k] = acc; }
+            int k = 0;
+            for (; k + 8 <= n; k += 8) {
+                double v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = d[k + j];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) { acc = acc + v[j]; cum[k + j] = acc; }
+            }
+            for (; k < n; ++k) { acc = acc + d[k]; cum[k] = acc; }
         }
         __syncthreads();
         const double delta = S / n;
@@ -344,17 +448,17 @@ long long vg_refine_scratch_doubles(int d, int n) {
 
 hipError_t vg_launch_results(hipStream_t st, int C, double V, double beta, const int* nh,
                              const long long* nh_sum, double* JF, double* JF2,
-                             const double2* win_first, const double2* win_last, long long WS,
+                             const double2* win_first, const double2* win_last, int ws_shift,
                              double* dh, double* part) {
     const unsigned nb = (unsigned)((C + VG_TREE - 1) / VG_TREE);
-    hipLaunchKernelGGL(vg_results_kernel, dim3(nb), dim3(VG_TREE), 0, st, C, V, beta, nh, nh_sum,
-                       JF, JF2, win_first, win_last, WS, dh, part);
+    hipLaunchKernelGGL(vg_results_kernel, dim3(nb), dim3(256), 0, st, C, V, beta, nh, nh_sum,
+                       JF, JF2, win_first, win_last, ws_shift, dh, part);
     return hipGetLastError();
 }
 
 hipError_t vg_launch_finalize(hipStream_t st, long long nparts, double* part, const long long* s_total,
                               const int* it_p, int hist_cap, double* hist, double* dsum, int* err) {
-    hipLaunchKernelGGL(vg_finalize_kernel, dim3(1), dim3(VG_TREE), 0, st, nparts, part, s_total, it_p,
+    hipLaunchKernelGGL(vg_finalize_kernel, dim3(1), dim3(256), 0, st, nparts, part, s_total, it_p,
                        hist_cap, hist, dsum, err);
     return hipGetLastError();
 }
@@ -362,21 +466,21 @@ hipError_t vg_launch_finalize(hipStream_t st, long long nparts, double* part, co
 hipError_t vg_launch_alloc(hipStream_t st, int C, int mode, int nh_const, const double* dh,
                            const double* dsum, long long N, int* nh, long long* bsum) {
     const unsigned nb = (unsigned)((C + VG_TREE - 1) / VG_TREE);
-    hipLaunchKernelGGL(vg_alloc_kernel, dim3(nb), dim3(VG_TREE), 0, st, C, mode, nh_const, dh, dsum,
+    hipLaunchKernelGGL(vg_alloc_kernel, dim3(nb), dim3(256), 0, st, C, mode, nh_const, dh, dsum,
                        N, nh, bsum);
     return hipGetLastError();
 }
 
 hipError_t vg_launch_scan(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
-                          long long* s_total, long long s_cap, int* it_p, int advance, int K,
+                          long long* s_total, long long s_cap, int* it_p, int advance, int k_shift,
                           long long run_cap, long long* nh_sum, int* run_cube, int* err) {
     const long long nb = (C + VG_TREE - 1) / VG_TREE;
     hipLaunchKernelGGL(vg_scan_blocks_kernel, dim3(1), dim3(VG_TREE), 0, st, nb, bsum, boff, s_total,
                        s_cap, it_p, advance, err);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(vg_scan_cubes_kernel, dim3((unsigned)nb), dim3(VG_TREE), 0, st, C, nh, boff,
-                       s_total, K, run_cap, nh_sum, run_cube);
+    hipLaunchKernelGGL(vg_scan_cubes_kernel, dim3((unsigned)nb), dim3(256), 0, st, C, nh, boff,
+                       s_total, k_shift, run_cap, nh_sum, run_cube);
     return hipGetLastError();
 }
 
@@ -390,8 +494,19 @@ hipError_t vg_launch_init_map(hipStream_t st, int d, int n, double* x_edges, dou
 hipError_t vg_launch_refine(hipStream_t st, int d, int n, double alpha, double* x_edges,
                             double* weights, unsigned* counts, double* scratch, double2* map64,
                             float2* map32) {
-    hipLaunchKernelGGL(vg_refine_kernel, dim3(d), dim3(VG_TREE), 0, st, n, alpha, x_edges, weights,
-                       counts, scratch, map64, map32);
+    const size_t lds = sizeof(double) * (size_t)(4LL * n + 1 + n / VG_TREE + 2);
+    const int use_lds = lds <= 128 * 1024;
+    if (use_lds && lds > 48 * 1024) {
+        static bool raised = false;
+        if (!raised) {
+            hipError_t e = hipFuncSetAttribute((const void*)vg_refine_kernel,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
+            if (e != hipSuccess) return e;
+            raised = true;
+        }
+    }
+    hipLaunchKernelGGL(vg_refine_kernel, dim3(d), dim3(VG_TREE), use_lds ? lds : 0, st, n, alpha,
+                       use_lds, x_edges, weights, counts, scratch, map64, map32);
     return hipGetLastError();
 }
 
@@ -422,12 +537,38 @@ vg_bsum_kernel(int C, const int* __restrict__ nh, long long* __restrict__ bsum) 
 }
 
 hipError_t vg_launch_layout(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
-                            long long* s_total, long long s_cap, int* it_p, int K, long long run_cap,
+                            long long* s_total, long long s_cap, int* it_p, int k_shift, long long run_cap,
                             long long* nh_sum, int* run_cube, int* err) {
     const unsigned nb = (unsigned)((C + VG_TREE - 1) / VG_TREE);
     hipLaunchKernelGGL(vg_bsum_kernel, dim3(nb), dim3(VG_TREE), 0, st, C, nh, bsum);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return vg_launch_scan(st, C, nh, bsum, boff, s_total, s_cap, it_p, 0, K, run_cap, nh_sum,
+    return vg_launch_scan(st, C, nh, bsum, boff, s_total, s_cap, it_p, 0, k_shift, run_cap, nh_sum,
                           run_cube, err);
+}
+
+// Copy the map edges and allocation that the coming sampling pass will use
+// into slot (it - skip) of the record, for iterations skip <= it < cap.
+__global__ void vg_snapshot_kernel(const int* __restrict__ it_p, int skip, int cap, int C, int nx,
+                                   const double* __restrict__ x_edges, const int* __restrict__ nh,
+                                   double* __restrict__ snap_x, int* __restrict__ snap_nh) {
+    const int it = *it_p;
+    if (it < skip || it >= cap) return;
+    const long long slot = it - skip;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < C;
+         i += (long long)gridDim.x * blockDim.x)
+        snap_nh[slot * C + i] = nh[i];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nx;
+         i += (long long)gridDim.x * blockDim.x)
+        snap_x[slot * nx + i] = x_edges[i];
+}
+
+hipError_t vg_launch_snapshot(hipStream_t st, const int* it_p, int skip, int cap, int C, int nx,
+                              const double* x_edges, const int* nh, double* snap_x, int* snap_nh) {
+    const long long work = C > nx ? C : nx;
+    unsigned grid = (unsigned)((work + 255) / 256);
+    if (grid > 4096) grid = 4096;
+    hipLaunchKernelGGL(vg_snapshot_kernel, dim3(grid), dim3(256), 0, st, it_p, skip, cap, C, nx,
+                       x_edges, nh, snap_x, snap_nh);
+    return hipGetLastError();
 }

@@ -122,30 +122,97 @@ __device__ vg_real integrand(vg_real* x) {{
     return Integrand("corner_peak", src, n_dim, exact, tuple(a))
 
 
-def gaussian_mixture(n_dim: int, w: Sequence[float], s: Sequence[float],
-                     mu: Sequence[Sequence[float]]) -> Integrand:
-    """sum_k w_k prod_i N(x_i; mu_ki, s_k) with fixed parameters."""
-    K = len(w)
-    flat = [float(v) for v in w] + [float(v) for v in s] + [float(m) for row in mu for m in row]
-    src = _arr("vg_mix_p", flat) + f"""
-__device__ vg_real integrand(vg_real* x) {{
+_MIXTURE_SRC = """
+#define VG_MIX_K {K}
+template <bool G>
+__device__ inline vg_real vg_mixture(const vg_real* x, const vg_real* th, vg_real* gr) {{
+    const vg_real c = sqrt((vg_real)2 * (vg_real)3.14159265358979323846);
     vg_real f = (vg_real)0;
-    for (int k = 0; k < {K}; ++k) {{
-        const vg_real s = (vg_real)vg_mix_p[{K} + k];
+    for (int k = 0; k < VG_MIX_K; ++k) {{
+        const vg_real w = th[k], s = th[VG_MIX_K + k];
+        const vg_real* mu = th + 2 * VG_MIX_K + k * VG_NDIM;
         vg_real q = (vg_real)0;
-        for (int i = 0; i < VG_NDIM; ++i) {{
-            const vg_real a = x[i] - (vg_real)vg_mix_p[{2 * K} + k * VG_NDIM + i];
-            q = q + a * a;
-        }}
+        for (int i = 0; i < VG_NDIM; ++i) {{ const vg_real a = x[i] - mu[i]; q = q + a * a; }}
         vg_real norm = (vg_real)1;
-        for (int i = 0; i < VG_NDIM; ++i) norm = norm * (s * sqrt((vg_real)2 * (vg_real)3.14159265358979323846));
-        f = f + (vg_real)vg_mix_p[k] * exp(-q / ((vg_real)2 * s * s)) / norm;
+        for (int i = 0; i < VG_NDIM; ++i) norm = norm * (s * c);
+        const vg_real pk = exp(-q / ((vg_real)2 * s * s)) / norm;
+        f = f + w * pk;
+        if (G) {{
+            const vg_real wp = w * pk, is2 = (vg_real)1 / (s * s);
+            gr[k] = pk;
+            gr[VG_MIX_K + k] = wp * (q * is2 - (vg_real)VG_NDIM) / s;
+            for (int i = 0; i < VG_NDIM; ++i) gr[2 * VG_MIX_K + k * VG_NDIM + i] = wp * (x[i] - mu[i]) * is2;
+        }}
     }}
     return f;
 }}
 """
-    exact = sum(w[k] * math.prod(_erf_mass(mu[k][i], s[k]) for i in range(n_dim)) for k in range(K))
-    return Integrand("gaussian_mixture", src, n_dim, exact, tuple(flat))
+
+
+def mixture_exact(n_dim: int, theta: Sequence[float], K: int = 4):
+    """Closed-form integral over [0,1]^d of sum_k w_k prod_i N(x_i; mu_ki, s_k) and its
+    gradient with respect to theta = [w (K), s (K), mu (K x d)], from the normal CDF
+    (erf) and the normal density at the faces."""
+    th = [float(v) for v in theta]
+    w, s = th[:K], th[K:2 * K]
+    mu = [th[2 * K + k * n_dim: 2 * K + (k + 1) * n_dim] for k in range(K)]
+    pdf = lambda x, m, sd: math.exp(-0.5 * ((x - m) / sd) ** 2) / (sd * math.sqrt(2 * math.pi))
+    val = 0.0
+    grad = [0.0] * (2 * K + K * n_dim)
+    for k in range(K):
+        m = [_erf_mass(mu[k][i], s[k]) for i in range(n_dim)]
+        dm_dmu = [pdf(0.0, mu[k][i], s[k]) - pdf(1.0, mu[k][i], s[k]) for i in range(n_dim)]
+        dm_ds = [-((1.0 - mu[k][i]) * pdf(1.0, mu[k][i], s[k]) + mu[k][i] * pdf(0.0, mu[k][i], s[k])) / s[k]
+                 for i in range(n_dim)]
+        prod = math.prod(m)
+        val += w[k] * prod
+        grad[k] = prod
+        for i in range(n_dim):
+            others = math.prod(m[j] for j in range(n_dim) if j != i)
+            grad[2 * K + k * n_dim + i] = w[k] * others * dm_dmu[i]
+            grad[K + k] += w[k] * others * dm_ds[i]
+    return val, grad
+
+
+def mixture_theta(n_dim: int, seed: int = 2, K: int = 4):
+    """Parameters of the differentiable mixture workload: mu ~ U(0.2, 0.8),
+    s ~ U(0.03, 0.15), w ~ U(0.5, 1.5) (numpy default_rng(seed))."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    mu = rng.uniform(0.2, 0.8, (K, n_dim))
+    s = rng.uniform(0.03, 0.15, K)
+    w = rng.uniform(0.5, 1.5, K)
+    return [*w.tolist(), *s.tolist(), *mu.ravel().tolist()]
+
+
+def gaussian_mixture_param(n_dim: int, theta: Sequence[float], K: int = 4) -> Integrand:
+    """Parametrised mixture f(x; theta) = sum_k w_k prod_i N(x_i; mu_ki, s_k),
+    theta = [w (K), s (K), mu (K x d)], with integrand_grad for the backward."""
+    src = _MIXTURE_SRC.format(K=K) + """
+__device__ vg_real integrand(vg_real* x, const vg_real* th) { return vg_mixture<false>(x, th, nullptr); }
+__device__ vg_real integrand_grad(vg_real* x, const vg_real* th, vg_real* g) { return vg_mixture<true>(x, th, g); }
+"""
+    val, grad = mixture_exact(n_dim, theta, K)
+    itg = Integrand("gaussian_mixture", src, n_dim, val, tuple(float(t) for t in theta),
+                    n_theta=len(theta))
+    itg.exact_grad = lambda th, d=n_dim, K=K: mixture_exact(d, th, K)[1]
+    return itg
+
+
+def gaussian_mixture(n_dim: int, w: Sequence[float], s: Sequence[float],
+                     mu: Sequence[Sequence[float]]) -> Integrand:
+    """sum_k w_k prod_i N(x_i; mu_ki, s_k) with the parameters compiled in."""
+    K = len(w)
+    flat = [float(v) for v in w] + [float(v) for v in s] + [float(m) for row in mu for m in row]
+    src = _arr("vg_mix_p", flat) + _MIXTURE_SRC.format(K=K) + f"""
+__device__ vg_real integrand(vg_real* x) {{
+    vg_real th[{len(flat)}];
+    for (int i = 0; i < {len(flat)}; ++i) th[i] = (vg_real)vg_mix_p[i];
+    return vg_mixture<false>(x, th, nullptr);
+}}
+"""
+    val, _ = mixture_exact(n_dim, flat, K)
+    return Integrand("gaussian_mixture", src, n_dim, val, tuple(flat))
 
 
 # --- polynomial integrands (bit-exact parity: no transcendental functions) ----

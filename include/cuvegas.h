@@ -167,6 +167,21 @@ int cvg_state_reallocate(cvg_state* st, const double* dh, double D);
 int cvg_state_dump(cvg_state* st, int it, long long n, long long* cube, int* ids, double* x,
                    double* jf);
 
+/* Differentiation of parametrised integrands (no reference counterpart).
+ * The program must be compiled with the option "grad" and an integrand that
+ * defines  vg_real integrand_grad(vg_real* x, const vg_real* theta, vg_real* g)
+ * (returns f, writes df/dtheta).  set_record keeps the map and allocation of
+ * every iteration >= skip; backward replays those frozen iterations (same
+ * Philox counters), recomputing the samples, and returns with the forward's
+ * inverse-variance weights held fixed:
+ *   value = sum_k w_k I_k(theta) / sum_k w_k,  grad = d value / d theta,
+ * their variances, for theta (n_theta host doubles) or the state's own
+ * parameters if theta is NULL. */
+int cvg_state_set_record(cvg_state* st, int enable);
+int cvg_state_get_snapshot(cvg_state* st, int it, double* x_edges, int* nh);
+int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double* grad_var,
+                       double* value, double* value_var);
+
 /* One-shot call with the reference's argument list (cuda_wrapper.cu:290).
  * integrand_src replaces the numba PTX; options replaces the kernel PTX path
  * and may hold "fp32" (CVG_FP32C) and extra hipRTC options.  batch_size,

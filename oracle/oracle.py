@@ -68,6 +68,7 @@ _lib.orc_alloc.argtypes = [C.c_longlong, _PD, C.c_double, C.c_longlong, _PI]
 _lib.orc_prefix.argtypes = [C.c_longlong, _PI, _PLL]; _lib.orc_prefix.restype = C.c_longlong
 _lib.orc_refine_dim.argtypes = [C.c_int, C.c_double, _PD, _PU, _PD, _PD]
 _lib.orc_run.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcOut)]
+_lib.orc_grad_replay.argtypes = [C.POINTER(OrcParams), _PD, _PI, C.c_int, _PD, _PD]
 _lib.orc_sample_dump.argtypes = [C.POINTER(OrcParams), _PD, _PI, C.c_int, C.c_longlong, _PLL,
                                  _PI, _PD, _PD]
 
@@ -202,3 +203,15 @@ def sample_dump(p, keep, x_edges, nh, it, n):
                               cube.ctypes.data_as(_PLL), ids.ctypes.data_as(_PI), _d(x), _d(jf))
     assert rc == 0
     return cube, ids, x, jf
+
+
+def grad_replay(p, keep, x_edges, nh, it, theta):
+    """One frozen iteration of the mixture: (I_k, sigma_k^2, dI/dtheta, var)."""
+    x_edges = np.ascontiguousarray(x_edges, dtype=np.float64)
+    nh = np.ascontiguousarray(nh, dtype=np.int32)
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    P = th.size
+    out = np.zeros(2 + 2 * P)
+    rc = _lib.orc_grad_replay(C.byref(p), _d(x_edges), nh.ctypes.data_as(_PI), it, _d(th), _d(out))
+    assert rc == 0
+    return out[0], out[1], out[2:2 + P], out[2 + P:]

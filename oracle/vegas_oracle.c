@@ -455,6 +455,50 @@ enum {
 
 static const double ORC_PI = 3.14159265358979323846;
 
+/* 4-component Gaussian mixture, theta = {w[4], s[4], mu[4][d]}; writes
+ * df/dtheta to g when g != NULL. */
+static double mix64(const double* x, int d, const double* th, double* g) {
+    const double c = sqrt(2.0 * ORC_PI);
+    double f = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        double w = th[k], s = th[4 + k], q = 0.0;
+        const double* mu = th + 8 + k * d;
+        for (int i = 0; i < d; ++i) { double a = x[i] - mu[i]; q = q + a * a; }
+        double norm = 1.0;
+        for (int i = 0; i < d; ++i) norm = norm * (s * c);
+        double pk = exp(-q / (2.0 * s * s)) / norm;
+        f = f + w * pk;
+        if (g) {
+            double wp = w * pk, is2 = 1.0 / (s * s);
+            g[k] = pk;
+            g[4 + k] = wp * (q * is2 - (double)d) / s;
+            for (int i = 0; i < d; ++i) g[8 + k * d + i] = wp * (x[i] - mu[i]) * is2;
+        }
+    }
+    return f;
+}
+
+static float mix32(const float* x, int d, const float* th, float* g) {
+    const float c = sqrtf(2.0f * (float)ORC_PI);
+    float f = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        float w = th[k], s = th[4 + k], q = 0.0f;
+        const float* mu = th + 8 + k * d;
+        for (int i = 0; i < d; ++i) { float a = x[i] - mu[i]; q = q + a * a; }
+        float norm = 1.0f;
+        for (int i = 0; i < d; ++i) norm = norm * (s * c);
+        float pk = expf(-q / (2.0f * s * s)) / norm;
+        f = f + w * pk;
+        if (g) {
+            float wp = w * pk, is2 = 1.0f / (s * s);
+            g[k] = pk;
+            g[4 + k] = wp * (q * is2 - (float)d) / s;
+            for (int i = 0; i < d; ++i) g[8 + k * d + i] = wp * (x[i] - mu[i]) * is2;
+        }
+    }
+    return f;
+}
+
 static double f64_eval(int id, const double* x, int d, const double* p) {
     switch (id) {
     case ORC_F_ONE: return 1.0;
@@ -486,17 +530,7 @@ static double f64_eval(int id, const double* x, int d, const double* p) {
         for (int i = 0; i <= d; ++i) f = f * r;
         return f;
     }
-    case ORC_F_MIXTURE: { /* p = {w[4], s[4], mu[4][d]} */
-        double f = 0.0;
-        for (int k = 0; k < 4; ++k) {
-            double s = p[4 + k], q = 0.0;
-            for (int i = 0; i < d; ++i) { double a = x[i] - p[8 + k * d + i]; q = q + a * a; }
-            double norm = 1.0;
-            for (int i = 0; i < d; ++i) norm = norm * (s * sqrt(2.0 * ORC_PI));
-            f = f + p[k] * exp(-q / (2.0 * s * s)) / norm;
-        }
-        return f;
-    }
+    case ORC_F_MIXTURE: return mix64(x, d, p, NULL);
     }
     return NAN;
 }
@@ -532,17 +566,7 @@ static float f32_eval(int id, const float* x, int d, const float* p) {
         for (int i = 0; i <= d; ++i) f = f * r;
         return f;
     }
-    case ORC_F_MIXTURE: {
-        float f = 0.0f;
-        for (int k = 0; k < 4; ++k) {
-            float s = p[4 + k], q = 0.0f;
-            for (int i = 0; i < d; ++i) { float a = x[i] - p[8 + k * d + i]; q = q + a * a; }
-            float norm = 1.0f;
-            for (int i = 0; i < d; ++i) norm = norm * (s * sqrtf(2.0f * (float)ORC_PI));
-            f = f + p[k] * expf(-q / (2.0f * s * s)) / norm;
-        }
-        return f;
-    }
+    case ORC_F_MIXTURE: return mix32(x, d, p, NULL);
     }
     return NAN;
 }
@@ -816,3 +840,91 @@ ORC_EXPORT int orc_sample_dump(const orc_params* P, const double* x_edges, const
 /* Sizes of the structs, so the ctypes mirror can check its layout. */
 ORC_EXPORT int orc_sizeof_params(void) { return (int)sizeof(orc_params); }
 ORC_EXPORT int orc_sizeof_out(void) { return (int)sizeof(orc_out); }
+
+/* Backward replay of one frozen iteration for the Gaussian mixture: the
+ * samples of iteration `it` under map x_edges and allocation nh, with
+ * parameters theta.  out[0] = I_k, out[1] = sigma_k^2, out[2..2+P) = dI_k/dtheta,
+ * out[2+P..2+2P) = its stratified variance with the unbiased per-cube estimator
+ * (same per-cube expressions as the GPU replay; cubes summed in index order). */
+ORC_EXPORT int orc_grad_replay(const orc_params* P, const double* x_edges, const int* nh, int it,
+                               const double* theta, double* out) {
+    const int d = P->d, n = P->nbins, E = n + 1, NT = 8 + 4 * d;
+    long long C;
+    if (P->integrand != ORC_F_MIXTURE) return -1;
+    if (ipow_checked(P->nstrat, d, &C)) return -2;
+    const double V = pow(1.0 / P->nstrat, d);
+    const uint32_t ns = (uint32_t)P->nstrat;
+    double* dx = (double*)malloc(sizeof(double) * (size_t)(d * n));
+    float* ef = (float*)malloc(sizeof(float) * (size_t)(d * E));
+    float* dxf = (float*)malloc(sizeof(float) * (size_t)(d * n));
+    for (int i = 0; i < d; ++i)
+        for (int j = 0; j < n; ++j) dx[i * n + j] = x_edges[i * E + j + 1] - x_edges[i * E + j];
+    build_f32_map(d, n, x_edges, ef, dxf);
+    float thf[512];
+    for (int q = 0; q < NT; ++q) thf[q] = (float)theta[q];
+    for (int q = 0; q < 2 + 2 * NT; ++q) out[q] = 0.0;
+    long long s0 = 0;
+    double g1[512], g2[512];
+    for (long long h = 0; h < C; ++h) {
+        const int cnt = nh[h];
+        double sf = 0.0, sf2 = 0.0;
+        for (int q = 0; q < NT; ++q) { g1[q] = 0.0; g2[q] = 0.0; }
+        for (int e = 0; e < cnt; ++e) {
+            const long long s = s0 + e;
+            uint32_t words[68];
+            for (int g = 0; g < (d + 3) / 4; ++g) {
+                uint32_t ctr[4] = {(uint32_t)s, (uint32_t)((uint64_t)s >> 32), (uint32_t)it, (uint32_t)g};
+                orc_philox(ctr, P->seed, P->stream, &words[4 * g]);
+            }
+            long long tmp = h;
+            double jf, gr[512];
+            if (P->precision == 0) {
+                double x[64], jac = 1.0, dd[64];
+                const double inv_ns = 1.0 / (double)ns;
+                for (int i = 0; i < d; ++i) {
+                    uint32_t k = (uint32_t)(tmp % ns); tmp /= ns;
+                    uint32_t id, rem; uint64_t low;
+                    orc_bin_of(words[i], k * (uint32_t)n, (uint32_t)n, ns, &id, &rem, &low);
+                    double frac = ((double)rem + (double)low * 0x1p-33) * inv_ns;
+                    x[i] = fma(dx[i * n + id], frac, x_edges[i * E + id]);
+                    dd[i] = dx[i * n + id];
+                }
+                for (int i = 0; i < d; ++i) jac = jac * ((double)n * dd[i]);
+                double f = mix64(x, d, theta, gr);
+                jf = f * jac;
+                for (int q = 0; q < NT; ++q) gr[q] = gr[q] * jac;
+            } else {
+                float x[64], jac = 1.0f, dd[64], grf[512];
+                const float inv_ns = 1.0f / (float)ns;
+                for (int i = 0; i < d; ++i) {
+                    uint32_t k = (uint32_t)(tmp % ns); tmp /= ns;
+                    uint32_t id, rem; uint64_t low;
+                    orc_bin_of(words[i], k * (uint32_t)n, (uint32_t)n, ns, &id, &rem, &low);
+                    float frac = ((float)rem + (float)(uint32_t)(low >> 9) * 0x1p-24f) * inv_ns;
+                    x[i] = fmaf(dxf[i * n + id], frac, ef[i * E + id]);
+                    dd[i] = dxf[i * n + id];
+                }
+                for (int i = 0; i < d; ++i) jac = jac * ((float)n * dd[i]);
+                float f = mix32(x, d, thf, grf);
+                jf = (double)(f * jac);
+                for (int q = 0; q < NT; ++q) gr[q] = (double)(grf[q] * jac);
+            }
+            sf = sf + jf;
+            sf2 = sf2 + jf * jf;
+            for (int q = 0; q < NT; ++q) { g1[q] = g1[q] + gr[q]; g2[q] = g2[q] + gr[q] * gr[q]; }
+        }
+        const double Ih = sf / cnt * V;
+        const double Sig2 = (sf2 / cnt * V * V) - (Ih * Ih);
+        out[0] += Ih;
+        out[1] += Sig2 / cnt;
+        for (int q = 0; q < NT; ++q) {
+            const double Gh = g1[q] / cnt * V;
+            const double Gv = (g2[q] / cnt * V * V) - (Gh * Gh);
+            out[2 + q] += Gh;
+            out[2 + NT + q] += Gv / (cnt - 1);
+        }
+        s0 += cnt;
+    }
+    free(dx); free(ef); free(dxf);
+    return 0;
+}

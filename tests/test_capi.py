@@ -128,3 +128,12 @@ def test_closed_forms_of_library():
     q, _ = si.nquad(lambda x, y: (1 + 0.5 * x + 1.5 * y) ** -3, [[0, 1], [0, 1]])
     assert I.corner_peak(2, [0.5, 1.5]).exact == pytest.approx(q, rel=1e-10)
     assert I.oscillatory(1, 0.0, [1.0]).exact == pytest.approx(math.sin(1.0), rel=1e-14)
+
+
+def test_run_len_must_be_power_of_two():
+    from cuvegas_amd._lib import CVG_ERR_INVALID, lib
+    for bad in (3, 6, 5000):
+        h = C.c_void_p()
+        rc = lib.cvg_program_create(b"__device__ vg_real integrand(vg_real* x){return x[0];}",
+                                    C.byref(_params(run_len=bad)), None, C.byref(h))
+        assert rc == CVG_ERR_INVALID

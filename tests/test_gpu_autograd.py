@@ -1,0 +1,120 @@
+"""Backward replay of parametrised integrands: against the oracle's replay of
+the same frozen iteration, central finite differences of the frozen estimator,
+and the closed-form erf gradient of the Gaussian mixture."""
+import json
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+def _theta():
+    with open(os.path.join(GOLD, "workloads.json")) as f:
+        return np.array(json.load(f)["mixture_6d"]["theta"])
+
+
+def _state(prec, theta, N=400_000, iters=8, skip=4, ns=None, nbins=256, seed=2):
+    from cuvegas_amd import Integrator
+    from cuvegas_amd import integrands as I
+    itg = I.gaussian_mixture_param(6, theta)
+    st = Integrator(itg.source, 6, N, n_intervals=nbins, n_strat=ns, max_it=iters, skip=skip,
+                    seed=seed, precision=prec, theta=theta, options="grad")
+    st.set_record(True)
+    return st, itg
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_replay_matches_oracle(orc, prec):
+    """One recorded iteration: value, variance, gradient and gradient variance
+    against the oracle's replay of the same frozen map and allocation."""
+    th = _theta()
+    _say("grad-oracle", prec)
+    st, itg = _state(prec, th, N=60_000, iters=3, skip=2, ns=5, nbins=64)
+    st.run(3)
+    v, vv, g, gv = st.backward()
+    x, nh = st.snapshot(2)
+    p, keep = orc.make_params(6, 64, 5, 60_000, 3, orc.F_MIXTURE, list(th), skip=2, seed=2,
+                              precision=0 if prec == "fp64" else 1)
+    oI, oS2, og, ogv = orc.grad_replay(p, keep, x, nh, 2, th)
+    tol = 1e-11 if prec == "fp64" else 2e-5
+    assert abs(v - oI) <= tol * abs(oI)
+    assert abs(vv - oS2) <= 1e-6 * abs(oS2) + 1e-300
+    assert np.max(np.abs(g - og) / (np.abs(og) + 1e-12 * np.abs(og).max())) < tol * 10
+    assert np.max(np.abs(gv - ogv) / (np.abs(ogv) + 1e-30)) < 1e-4
+    # the replay at the forward's theta reproduces the forward estimate
+    I_k, S2, ne = st.history()
+    assert abs(v - I_k[2]) <= 1e-12 * abs(I_k[2]) + (1e-6 * abs(I_k[2]) if prec == "fp32" else 0)
+
+
+def test_gradient_vs_finite_differences():
+    """d value / d theta of the frozen estimator vs central differences (h = 1e-4)."""
+    th = _theta()
+    _say("grad-fd")
+    st, itg = _state("fp64", th, N=200_000, iters=6, skip=3)
+    st.run(6)
+    v0, _, g, _ = st.backward()
+    def fd(p, h):
+        tp, tm = th.copy(), th.copy()
+        tp[p] += h
+        tm[p] -= h
+        return (st.backward(tp)[0] - st.backward(tm)[0]) / (2 * h)
+
+    scale = np.abs(g).max()
+    for p in range(th.size):
+        f1 = fd(p, 1e-4)  # central difference, truncation error ~1e-6 relative at h = 1e-4
+        assert abs(f1 - g[p]) <= 3e-5 * max(abs(g[p]), 1e-3 * scale), (p, f1, g[p])
+        # Richardson extrapolation removes the h^2 term: agreement to ~1e-8
+        f2 = fd(p, 5e-5)
+        rich = (4 * f2 - f1) / 3
+        assert abs(rich - g[p]) <= 1e-7 * max(abs(g[p]), 1e-3 * scale), (p, rich, g[p])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_gradient_vs_closed_form(prec):
+    """Gradient within 4 sigma of the closed form per component, and the
+    squared pulls average below 2 (32 parameters)."""
+    th = _theta()
+    _say("grad-closed-form", prec)
+    st, itg = _state(prec, th, N=2_000_000, iters=10, skip=5)
+    st.run(10)
+    r = st.result()
+    v, vv, g, gv = st.backward()
+    exact_v = itg.exact
+    exact_g = np.array(itg.exact_grad(th))
+    assert abs(r["result"] - exact_v) < 4 * r["error"]
+    assert abs(v - exact_v) < 4 * math.sqrt(vv)
+    pulls = (g - exact_g) / np.sqrt(gv)
+    print("max |pull|", np.abs(pulls).max(), "within 3 sigma", np.mean(np.abs(pulls) < 3))
+    assert np.all(np.abs(pulls) < 4)
+    assert np.mean(pulls ** 2) < 2.0
+
+
+def test_torch_autograd_function():
+    import torch
+    from cuvegas_amd import integrands as I
+    from cuvegas_amd.autograd import DifferentiableIntegral
+    th = _theta()
+    _say("torch-autograd")
+    itg = I.gaussian_mixture_param(6, th)
+    mod = DifferentiableIntegral(itg.source, 6, th.size, 300_000, max_it=6, skip=3, n_intervals=256,
+                                 seed=4)
+    theta = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+    val = mod(theta)
+    val.backward()
+    v, _, g, _ = mod.state.backward()
+    assert val.item() == pytest.approx(mod.last["result"], rel=0, abs=0)
+    assert np.allclose(theta.grad.numpy(), g, rtol=0, atol=0)
+    assert abs(val.item() - itg.exact) < 4 * mod.last["error"]
+    # chain rule through a downstream op
+    theta.grad = None
+    (2.0 * mod(theta)).backward()
+    assert np.allclose(theta.grad.numpy(), 2.0 * mod.state.backward()[2])
