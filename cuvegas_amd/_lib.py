@@ -78,6 +78,15 @@ class cvg_ref_result(C.Structure):
     ]
 
 
+class cvg_batch_item(C.Structure):
+    _fields_ = [
+        ("n_dim", C.c_int),
+        ("n_strat", C.c_int),
+        ("n_eval_it", C.c_longlong),
+        ("theta_offset", C.c_int),
+    ]
+
+
 _P = C.c_void_p
 _PD = C.POINTER(C.c_double)
 _PI = C.POINTER(C.c_int)
@@ -119,6 +128,18 @@ SIGNATURES = {
     "cvg_state_set_record": (C.c_int, [_P, C.c_int]),
     "cvg_state_get_snapshot": (C.c_int, [_P, C.c_int, _PD, _PI]),
     "cvg_state_backward": (C.c_int, [_P, _PD, _PD, _PD, _PD, _PD]),
+    "cvg_batch_program_create": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(cvg_params), C.c_char_p,
+                                           C.POINTER(_P)]),
+    "cvg_batch_create": (C.c_int, [_P, C.POINTER(cvg_params), C.POINTER(cvg_batch_item), C.c_int, _PD,
+                                   C.c_int, C.POINTER(_P)]),
+    "cvg_batch_destroy": (None, [_P]),
+    "cvg_batch_run": (C.c_int, [_P, C.c_int, C.c_int]),
+    "cvg_batch_results": (C.c_int, [_P, _PD, _PD, _PD, _PLL, _PD]),
+    "cvg_batch_history": (C.c_int, [_P, C.c_int, _PD, _PD, _PLL, C.c_int]),
+    "cvg_batch_item_info": (C.c_int, [_P, C.c_int, _PI, _PI, _PLL]),
+    "cvg_batch_get_map": (C.c_int, [_P, C.c_int, _PD]),
+    "cvg_batch_get_nh": (C.c_int, [_P, C.c_int, _PI]),
+    "cvg_batch_get_cube_sums": (C.c_int, [_P, C.c_int, _PD, _PD]),
     "cvg_integrate_cuda": (C.c_int, [C.c_longlong, C.c_int, C.c_int, C.c_longlong, C.c_int, C.c_int,
                                      C.c_int, C.c_double, C.c_double, C.c_uint, C.c_int, C.c_int,
                                      C.c_int, C.c_int, C.c_double, C.c_char_p, C.c_char_p,

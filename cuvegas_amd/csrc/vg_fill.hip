@@ -29,6 +29,11 @@ typedef double vg_real;
 
 #include "vg_user.h"
 
+#ifndef VG_BATCH
+#define VG_BATCH 0
+#endif
+
+#if !VG_BATCH
 namespace vg_detail {
 // The integrand may be written in any of three forms:
 //   vg_real integrand(vg_real* x, const vg_real* theta)   parametrised
@@ -285,7 +290,9 @@ vg_dump(const vg_map_t* __restrict__ gmap, const long long* __restrict__ nh_sum,
     jf[s] = o.jf;
 }
 
-#if VG_GRAD
+#endif  // !VG_BATCH
+
+#if VG_GRAD && !VG_BATCH
 // ---------------------------------------------------------------------------
 // Backward replay: d I_k / d theta on a frozen iteration (map, allocation and
 // Philox counters of forward iteration it).  The integrand supplies
@@ -397,3 +404,207 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
     }
 }
 #endif
+
+#if VG_BATCH
+// ---------------------------------------------------------------------------
+// Batched sampling: many independent integrals (items), each with its own
+// dimension (<= VG_MAXD), strata, map, allocation and parameter block, in one
+// launch per iteration.  The integrand has the form
+//   vg_real integrand(vg_real* x, const vg_real* theta, int d)
+// where theta points at the item's parameters.  Item i draws from Philox
+// stream i, so it reproduces a single-integral run with stream_id = i.
+//
+// Work is a static list of tasks (item, chunk of VG_BLOCK/64 windows); a block
+// takes whole tasks, so its LDS bins belong to one item at a time and are
+// merged into the item's global bins at the end of the task.  Windows past the
+// item's sample count this iteration are skipped.
+// ---------------------------------------------------------------------------
+// Item descriptor; the host writes the same 80-byte layout (vg_host.cpp).
+struct vg_item {
+    long long cube_off;  // first cube of the item (nh, JF, JF2, dh)
+    long long nhs_off;   // first entry of the item's nh_sum (C+1 entries)
+    long long win_off;   // first window (win_first / win_last); runs start at 64 * win_off
+    double V;            // cube volume n_strat^-d
+    double inv_ns;       // 1.0 / n_strat
+    int d, C, map_off, theta_off;  // dims, cubes, first (dim, bin) entry, first parameter
+    unsigned ns, ns_m, ns_sh1, ns_sh2;  // n_strat and its division magic
+    float inv_ns_f, pad;
+};
+
+// floor(x / d) for the item's n_strat (Granlund & Montgomery, "Division by
+// invariant integers using multiplication", PLDI 1994, fig. 4.1).
+__device__ inline unsigned vg_udiv(unsigned x, unsigned m, unsigned sh1, unsigned sh2) {
+    const unsigned t1 = __umulhi(m, x);
+    return (t1 + ((x - t1) >> sh1)) >> sh2;
+}
+
+constexpr int MD = VG_MAXD;
+constexpr int MG = (MD + 3) / 4;
+constexpr int NB = VG_NBINS;
+constexpr int K = VG_K;
+constexpr int WPB = VG_BLOCK / 64;
+constexpr long long WS = 64LL * K;
+
+#if VG_FP32
+typedef float2 vg_map_t;
+#else
+typedef double2 vg_map_t;
+#endif
+
+__device__ inline void vg_beval(double& jf, unsigned* ids, long long s, int h, int d, unsigned it,
+                                unsigned seed, unsigned stream, const vg_item& I,
+                                const vg_map_t* __restrict__ map, const vg_real* __restrict__ theta) {
+    unsigned words[4 * MG];
+#pragma unroll
+    for (int g = 0; g < MG; ++g) {
+        if (4 * g < d) {
+            const vg_u4 v = vg_sample_words((unsigned long long)s, it, (unsigned)g, seed, stream);
+            words[4 * g + 0] = v.x; words[4 * g + 1] = v.y;
+            words[4 * g + 2] = v.z; words[4 * g + 3] = v.w;
+        }
+    }
+    unsigned tmp = (unsigned)h;
+    vg_real x[MD], jac = (vg_real)1;
+#pragma unroll
+    for (int i = 0; i < MD; ++i) {
+        if (i < d) {
+            const unsigned q = vg_udiv(tmp, I.ns_m, I.ns_sh1, I.ns_sh2);
+            const unsigned k = tmp - q * I.ns;
+            tmp = q;
+            const unsigned long long L = ((unsigned long long)words[i] * 2ull + 1ull) * (unsigned long long)NB;
+            const unsigned Q = k * (unsigned)NB + (unsigned)(L >> 33);
+            const unsigned id = vg_udiv(Q, I.ns_m, I.ns_sh1, I.ns_sh2);
+            const unsigned rem = Q - id * I.ns;
+            const unsigned long long low = L & ((1ull << 33) - 1ull);
+#if VG_FP32
+            const float frac = ((float)rem + (float)(unsigned)(low >> 9) * 0x1p-24f) * I.inv_ns_f;
+            const float2 m = map[i * NB + id];
+            x[i] = fmaf(m.y, frac, m.x);
+#else
+            const double frac = ((double)rem + (double)low * 0x1p-33) * I.inv_ns;
+            const double2 m = map[i * NB + id];
+            x[i] = fma(m.y, frac, m.x);
+#endif
+            jac = jac * ((vg_real)NB * m.y);
+            ids[i] = id;
+        }
+    }
+    const vg_real f = integrand(x, theta, d);
+    jf = (double)(f * jac);
+}
+
+extern "C" __global__ void __launch_bounds__(VG_BLOCK)
+vg_bfill(const vg_item* __restrict__ items, const int* __restrict__ task_item,
+         const int* __restrict__ task_w0, int ntasks, const vg_map_t* __restrict__ gmap,
+         const long long* __restrict__ nh_sum, const int* __restrict__ run_cube,
+         const long long* __restrict__ s_item, const int* __restrict__ it_p,
+         double* __restrict__ JF, double* __restrict__ JF2,
+         double2* __restrict__ win_first, double2* __restrict__ win_last,
+         double* __restrict__ weights, unsigned* __restrict__ counts,
+         const vg_real* __restrict__ theta_all, unsigned seed) {
+    __shared__ double lw[MD * NB];
+    __shared__ unsigned lc[MD * NB];
+    const unsigned it = (unsigned)*it_p;
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    for (int task = blockIdx.x; task < ntasks; task += gridDim.x) {
+        const int item = task_item[task];
+        const long long w0 = task_w0[task];
+        const long long S = s_item[item];
+        const long long nwin = (S + WS - 1) / WS;
+        if (w0 >= nwin) continue;  // uniform over the block
+        const vg_item I = items[item];
+        const int d = I.d;
+        const vg_map_t* map = gmap + I.map_off;
+        const long long* nhs = nh_sum + I.nhs_off;
+        const vg_real* theta = theta_all + I.theta_off;
+        double* jf1 = JF + I.cube_off;
+        double* jf2 = JF2 + I.cube_off;
+        for (int i = threadIdx.x; i < d * NB; i += VG_BLOCK) { lw[i] = 0.0; lc[i] = 0u; }
+        __syncthreads();
+        const long long w = w0 + wid;
+        if (w < nwin) {
+            const long long r = w * 64 + lane;
+            const long long rs = r * K;
+            const bool active = rs < S;
+            const long long re = active ? (rs + K < S ? rs + K : S) : rs;
+            int h = 0;
+            long long cstart = 0, cend = 0;
+            if (active) {
+                h = run_cube[I.win_off * 64 + r];
+                cstart = nhs[h];
+                cend = nhs[h + 1];
+            }
+            const long long startF = cstart;
+            const bool sb = active && cstart < rs;
+            bool hasF = false;
+            int hF = 0, npieces = 0;
+            double f1 = 0.0, f2 = 0.0, a1 = 0.0, a2 = 0.0;
+            for (int j = 0; j < K; ++j) {
+                const long long s = rs + j;
+                if (s < re) {
+                    if (s == cend) {
+                        if (npieces == 0 && sb) { hasF = true; hF = h; f1 = a1; f2 = a2; }
+                        else { jf1[h] = a1; jf2[h] = a2; }
+                        ++npieces;
+                        ++h;
+                        cstart = cend;
+                        cend = nhs[h + 1];
+                        a1 = 0.0; a2 = 0.0;
+                    }
+                    double jf;
+                    unsigned ids[MD];
+                    vg_beval(jf, ids, s, h, d, it, seed, (unsigned)item, I, map, theta);
+                    const double jq = jf * jf;
+#pragma unroll
+                    for (int i = 0; i < MD; ++i) {
+                        if (i < d) {
+                            atomicAdd(&lw[i * NB + ids[i]], jq);
+                            atomicAdd(&lc[i * NB + ids[i]], 1u);
+                        }
+                    }
+                    a1 = a1 + jf;
+                    a2 = a2 + jq;
+                }
+            }
+            const bool cont = cend > re;
+            int key = active ? h : -1;
+            double s1 = active ? a1 : 0.0, s2 = active ? a2 : 0.0;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int kp = __shfl_up(key, off, 64);
+                const double p1 = __shfl_up(s1, off, 64);
+                const double p2 = __shfl_up(s2, off, 64);
+                if (lane >= off && kp == key) { s1 = p1 + s1; s2 = p2 + s2; }
+            }
+            const double q1 = __shfl_up(s1, 1, 64), q2 = __shfl_up(s2, 1, 64);
+            const long long wstart = w * WS;
+            const long long gw = I.win_off + w;
+            if (active) {
+                if (hasF) {
+                    double t1 = f1, t2 = f2;
+                    if (lane > 0) { t1 = q1 + t1; t2 = q2 + t2; }
+                    if (startF < wstart) win_first[gw] = make_double2(t1, t2);
+                    else { jf1[hF] = t1; jf2[hF] = t2; }
+                }
+                if (!cont) {
+                    if (cstart < wstart) win_first[gw] = make_double2(s1, s2);
+                    else { jf1[h] = s1; jf2[h] = s2; }
+                } else if (lane == 63) {
+                    win_last[gw] = make_double2(s1, s2);
+                    if (cstart < wstart) win_first[gw] = make_double2(s1, s2);
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < d * NB; i += VG_BLOCK) {
+            const unsigned c = lc[i];
+            if (c) {
+                unsafeAtomicAdd(&weights[I.map_off + i], lw[i]);
+                atomicAdd(&counts[I.map_off + i], c);
+            }
+        }
+        __syncthreads();
+    }
+}
+#endif  // VG_BATCH

@@ -88,13 +88,14 @@ struct cvg_program {
     int d = 0, nbins = 0, nstrat = 0, precision = 0, K = kDefaultRun, has_domain = 0;
     int block = kFillBlock, lds_bins = 0, lds_map = 0;
     int has_grad = 0, n_theta = 0, bin32 = 0;
+    int batch = 0, max_dim = 0;
     std::string log;
     std::vector<char> code;
     double compile_ms = 0.0;
     // module loaded on first use, on the device current at that time
     int device = -1;
     hipModule_t mod = nullptr;
-    hipFunction_t fill = nullptr, dump = nullptr, grad = nullptr;
+    hipFunction_t fill = nullptr, dump = nullptr, grad = nullptr, bfill = nullptr;
     int fill_blocks_per_cu = 0;
 
     int load() {
@@ -104,10 +105,15 @@ struct cvg_program {
         if (mod) return fail(CVG_ERR_INVALID, "program was loaded on another device");
         const double t0 = now_ms();
         VG_HIP(hipModuleLoadData(&mod, code.data()));
-        VG_HIP(hipModuleGetFunction(&fill, mod, "vg_fill"));
-        VG_HIP(hipModuleGetFunction(&dump, mod, "vg_dump"));
-        if (has_grad) VG_HIP(hipModuleGetFunction(&grad, mod, "vg_grad"));
-        VG_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&fill_blocks_per_cu, fill, block, 0));
+        if (batch) {
+            VG_HIP(hipModuleGetFunction(&bfill, mod, "vg_bfill"));
+            VG_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&fill_blocks_per_cu, bfill, block, 0));
+        } else {
+            VG_HIP(hipModuleGetFunction(&fill, mod, "vg_fill"));
+            VG_HIP(hipModuleGetFunction(&dump, mod, "vg_dump"));
+            if (has_grad) VG_HIP(hipModuleGetFunction(&grad, mod, "vg_grad"));
+            VG_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&fill_blocks_per_cu, fill, block, 0));
+        }
         if (fill_blocks_per_cu < 1) fill_blocks_per_cu = 1;
         device = dev;
         compile_ms += now_ms() - t0;
@@ -169,6 +175,11 @@ struct cvg_state {
     double* hist = nullptr;
     double* domain = nullptr;
     void* theta = nullptr;
+    int* fold_list = nullptr;   // cubes whose samples span more than 64 windows
+    int* fold_count = nullptr;
+    int fold_cap = 0;
+    int* bw_fold_list = nullptr;
+    int* bw_fold_count = nullptr;
     int iterations_done = 0;
     double setup_ms = 0.0, loop_ms = 0.0;
     // recorded forward iterations (for the backward replay): map edges and n_h
@@ -201,7 +212,8 @@ struct cvg_state {
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         void* bufs[] = {x_edges, map64, map32, weights, counts, refine_scratch, nh, nh_sum, dh,
                         JF, JF2, run_cube, win_first, win_last, part, bsum, boff, s_total,
-                        it_p, dsum, err, hist, domain, theta, snap_x, snap_nh, bw_nh_sum,
+                        it_p, dsum, err, hist, domain, theta, fold_list, fold_count, bw_fold_list,
+                        bw_fold_count, snap_x, snap_nh, bw_nh_sum,
                         bw_s_total, bw_x, bw_map64, bw_map32, bw_part};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
@@ -237,6 +249,8 @@ struct cvg_state {
     }
 
     int launch_results() {
+        VG_HIP(vg_launch_bigfold(stream, fold_list, fold_count, fold_cap, nh_sum, win_first, win_last,
+                                 6 + k_shift, JF, JF2));
         VG_HIP(vg_launch_results(stream, (int)C, V, p.beta, nh, nh_sum, JF, JF2, win_first, win_last,
                                  6 + k_shift, dh, part));
         VG_HIP(vg_launch_finalize(stream, nb, part, s_total, it_p, p.max_it, hist, dsum, err));
@@ -246,7 +260,7 @@ struct cvg_state {
     int launch_realloc(int mode, int advance) {
         VG_HIP(vg_launch_alloc(stream, (int)C, mode, nh0, dh, dsum, p.n_eval_it, nh, bsum));
         VG_HIP(vg_launch_scan(stream, (int)C, nh, bsum, boff, s_total, s_cap, it_p, advance, k_shift,
-                              run_cap, nh_sum, run_cube, err));
+                              run_cap, nh_sum, run_cube, err, fold_list, fold_count, fold_cap));
         return CVG_OK;
     }
 
@@ -283,7 +297,7 @@ struct cvg_state {
 // Layout (nh_sum, run_cube) from the n_h already on the device.
 int cvg_state::launch_realloc_copy() {
     VG_HIP(vg_launch_layout(stream, (int)C, nh, bsum, boff, s_total, s_cap, it_p, k_shift, run_cap, nh_sum,
-                            run_cube, err));
+                            run_cube, err, fold_list, fold_count, fold_cap));
     return CVG_OK;
 }
 
@@ -504,6 +518,9 @@ int cvg_state_create(cvg_program* prog, const cvg_params* params, cvg_state** ou
     if ((rc = dalloc(&st->err, 1))) return rc;
     if ((rc = dalloc(&st->hist, 4LL * params->max_it))) return rc;
     if ((rc = dalloc(&st->domain, 2LL * d))) return rc;
+    st->fold_cap = (int)(st->s_cap / (64LL * 64 * K) + 2);
+    if ((rc = dalloc(&st->fold_list, st->fold_cap))) return rc;
+    if ((rc = dalloc(&st->fold_count, 1))) return rc;
     if ((rc = dalloc((char**)&st->theta, 8LL * (params->n_theta > 0 ? params->n_theta : 1)))) return rc;
 
     VG_HIP(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
@@ -892,6 +909,8 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         if ((rc = dalloc(&st->bw_x, (long long)d * (n + 1)))) return rc;
         if ((rc = dalloc(&st->bw_map64, (long long)d * n))) return rc;
         if ((rc = dalloc(&st->bw_map32, (long long)d * n))) return rc;
+        if ((rc = dalloc(&st->bw_fold_list, 1))) return rc;
+        if ((rc = dalloc(&st->bw_fold_count, 1))) return rc;
     }
     // parameters for the replay: given theta, or the forward's
     void* th = st->theta;
@@ -924,7 +943,8 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         VG_HIP(vg_launch_tables(st->stream, d, n, st->bw_x, st->bw_map64, st->bw_map32));
         const int* snh = st->snap_nh + slot * st->C;
         VG_HIP(vg_launch_layout(st->stream, C, snh, st->bsum, st->boff, st->bw_s_total, (1LL << 62), st->it_p,
-                                st->k_shift, 0, st->bw_nh_sum, nullptr, st->err));
+                                st->k_shift, 0, st->bw_nh_sum, nullptr, st->err, st->bw_fold_list,
+                                st->bw_fold_count, 1));
         const void* map = st->p.precision == CVG_FP32C ? (const void*)st->bw_map32 : (const void*)st->bw_map64;
         unsigned itu = (unsigned)k;
         double V = st->V;
@@ -1016,6 +1036,495 @@ int cvg_integrate_cuda(long long tot_eval, int max_it, int skip, long long max_b
     out->gpu_num = 1;
     out->omp_threads_num = 0;
     out->gpu_thr = gpu_thr;
+    return CVG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: many independent integrals per launch (no reference counterpart).
+// ---------------------------------------------------------------------------
+
+// Item descriptor of the batched sampling kernel (layout of vg_item in vg_fill.hip).
+struct vg_fitem {
+    long long cube_off, nhs_off, win_off;
+    double V, inv_ns;
+    int d, C, map_off, theta_off;
+    unsigned ns, ns_m, ns_sh1, ns_sh2;
+    float inv_ns_f, pad;
+};
+static_assert(sizeof(vg_fitem) == 80, "vg_fitem layout");
+
+// Magic numbers for floor(x / d), 32-bit x (Granlund & Montgomery 1994, fig. 4.1).
+static void udiv_magic(unsigned d, unsigned* m, unsigned* sh1, unsigned* sh2) {
+    int l = 0;
+    while ((1ull << l) < d) ++l;
+    *m = (unsigned)((((1ull << l) - d) << 32) / d + 1);
+    *sh1 = l < 1 ? (unsigned)l : 1u;
+    *sh2 = l > 1 ? (unsigned)(l - 1) : 0u;
+}
+
+template <typename T>
+static int upload(hipStream_t st, T** dptr, const std::vector<T>& v) {
+    int rc = dalloc(dptr, (long long)v.size());
+    if (rc) return rc;
+    if (!v.empty()) {
+        VG_HIP(hipMemcpyAsync(*dptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+    }
+    return CVG_OK;
+}
+
+struct cvg_batch {
+    cvg_program* prog = nullptr;
+    cvg_params p{};
+    int n_items = 0, K = kDefaultRun, k_shift = 4, device = -1;
+    long long tot_cubes = 0, tot_nhs = 0, tot_win = 0, tot_blk = 0, tot_xe = 0, tot_map = 0;
+    int n_refine = 0, n_tasks = 0;
+    std::vector<vg_bitem> hitems;
+    std::vector<int> dims;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    vg_fitem* fitems = nullptr;
+    vg_bitem* bitems = nullptr;
+    int *task_item = nullptr, *task_w0 = nullptr, *blk_item = nullptr, *rd_item = nullptr, *rd_dim = nullptr;
+    int* nh = nullptr;
+    long long* nh_sum = nullptr;
+    double *JF = nullptr, *JF2 = nullptr, *dh = nullptr;
+    int* run_cube = nullptr;
+    double2 *win_first = nullptr, *win_last = nullptr;
+    double *part = nullptr, *dsum = nullptr;
+    long long *bsum = nullptr, *boff = nullptr, *s_item = nullptr;
+    int *it_p = nullptr, *err = nullptr;
+    double* hist = nullptr;
+    double* x_all = nullptr;
+    double2* map64 = nullptr;
+    float2* map32 = nullptr;
+    double* weights = nullptr;
+    unsigned* counts = nullptr;
+    void* theta = nullptr;
+    long long* fold_list = nullptr;
+    int* fold_count = nullptr;
+    int fold_cap = 0;
+    int iterations_done = 0;
+    double loop_ms = 0.0, setup_ms = 0.0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
+    ~cvg_batch() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        void* bufs[] = {fitems, bitems, task_item, task_w0, blk_item, rd_item, rd_dim, nh, nh_sum, JF, JF2,
+                        dh, run_cube, win_first, win_last, part, dsum, bsum, boff, s_item, it_p, err,
+                        hist, x_all, map64, map32, weights, counts, theta, fold_list, fold_count};
+        for (void* b : bufs)
+            if (b) (void)hipFree(b);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+
+    int steps(int phase) {
+        VG_HIP(vg_launch_batch_iteration_steps(
+            stream, bitems, n_items, tot_blk, blk_item, n_refine, rd_item, rd_dim, p.n_intervals, p.alpha,
+            p.beta, k_shift, p.max_it, nh, nh_sum, JF, JF2, win_first, win_last, dh, part, dsum, bsum, boff,
+            s_item, run_cube, it_p, hist, x_all, weights, counts, map64, map32, err, phase, fold_list,
+            fold_count, fold_cap));
+        return CVG_OK;
+    }
+
+    int fill() {
+        int ncu = 0;
+        VG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+        unsigned grid = (unsigned)(ncu * prog->fill_blocks_per_cu);
+        if ((long long)grid > n_tasks) grid = (unsigned)n_tasks;
+        const void* map = p.precision == CVG_FP32C ? (const void*)map32 : (const void*)map64;
+        unsigned seed = p.seed;
+        int nt = n_tasks;
+        void* args[] = {(void*)&fitems,  (void*)&task_item, (void*)&task_w0,  (void*)&nt,
+                        (void*)&map,     (void*)&nh_sum,    (void*)&run_cube, (void*)&s_item,
+                        (void*)&it_p,    (void*)&JF,        (void*)&JF2,      (void*)&win_first,
+                        (void*)&win_last, (void*)&weights,  (void*)&counts,   (void*)&theta,
+                        (void*)&seed};
+        VG_HIP(hipModuleLaunchKernel(prog->bfill, grid, 1, 1, (unsigned)prog->block, 1, 1, 0, stream, args,
+                                     nullptr));
+        return CVG_OK;
+    }
+
+    int iteration() {
+        int rc = fill();
+        if (rc) return rc;
+        return steps(1);
+    }
+};
+
+extern "C" {
+
+int cvg_batch_program_create(const char* integrand_src, int max_dim, const cvg_params* shape,
+                             const char* options, cvg_program** out) {
+    if (!out) return fail(CVG_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!integrand_src || !shape) return fail(CVG_ERR_INVALID, "NULL argument");
+    if (max_dim < 1 || max_dim > 16) return fail(CVG_ERR_INVALID, "max_dim must be in 1..16");
+    if (shape->n_intervals < 2 || shape->n_intervals > 1024)
+        return fail(CVG_ERR_INVALID, "batched n_intervals must be in 2..1024");
+    if (shape->precision != CVG_FP64 && shape->precision != CVG_FP32C)
+        return fail(CVG_ERR_INVALID, "precision must be CVG_FP64 or CVG_FP32C");
+    const int K = shape->run_len > 0 ? shape->run_len : kDefaultRun;
+    if (K > 4096 || (K & (K - 1))) return fail(CVG_ERR_INVALID, "run_len must be a power of two up to 4096");
+    if ((long long)max_dim * shape->n_intervals * 12 > kLdsBytes)
+        return fail(CVG_ERR_INVALID, "max_dim * n_intervals bins do not fit in LDS");
+    auto prog = std::make_unique<cvg_program>();
+    prog->batch = 1;
+    prog->max_dim = max_dim;
+    prog->d = max_dim;
+    prog->nbins = shape->n_intervals;
+    prog->precision = shape->precision;
+    prog->K = K;
+    int block = kFillBlock;
+    std::string rtc_extra;
+    if (options) {
+        std::istringstream ss(options);
+        std::string tok;
+        while (ss >> tok) {
+            const size_t eq = tok.find('=');
+            const std::string key = eq == std::string::npos ? tok : tok.substr(0, eq);
+            const int val = eq == std::string::npos ? 1 : std::atoi(tok.c_str() + eq + 1);
+            if (key == "block") {
+                if (val < 64 || val > 1024 || val % 64) return fail(CVG_ERR_INVALID, "block must be 64..1024, multiple of 64");
+                block = val;
+            } else {
+                rtc_extra += tok + " ";
+            }
+        }
+    }
+    prog->block = block;
+    const double t0 = now_ms();
+    hiprtcProgram rtc;
+    const char* hdr_src[2] = {kVgPhiloxSrc, integrand_src};
+    const char* hdr_name[2] = {"vg_philox.h", "vg_user.h"};
+    if (hiprtcCreateProgram(&rtc, kVgFillSrc, "vg_fill.hip", 2, hdr_src, hdr_name) != HIPRTC_SUCCESS)
+        return fail(CVG_ERR_COMPILE, "hiprtcCreateProgram failed");
+    std::vector<std::string> opts = {
+        "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-DVG_BATCH=1", "-DVG_MAXD=" + std::to_string(max_dim),
+        "-DVG_NDIM=" + std::to_string(max_dim), "-DVG_NBINS=" + std::to_string(prog->nbins),
+        "-DVG_NSTRAT=1u", "-DVG_K=" + std::to_string(K), "-DVG_BLOCK=" + std::to_string(block),
+        "-DVG_FP32=" + std::to_string(prog->precision == CVG_FP32C ? 1 : 0)};
+    {
+        std::istringstream ss(rtc_extra);
+        std::string tok;
+        while (ss >> tok) opts.push_back(tok);
+    }
+    std::vector<const char*> copts;
+    for (auto& o : opts) copts.push_back(o.c_str());
+    hiprtcResult cr = hiprtcCompileProgram(rtc, (int)copts.size(), copts.data());
+    size_t log_size = 0;
+    hiprtcGetProgramLogSize(rtc, &log_size);
+    if (log_size > 1) {
+        prog->log.resize(log_size);
+        hiprtcGetProgramLog(rtc, &prog->log[0]);
+        while (!prog->log.empty() && prog->log.back() == '\0') prog->log.pop_back();
+    }
+    if (cr != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&rtc);
+        *out = prog.release();
+        return fail(CVG_ERR_COMPILE, std::string("hipRTC compile failed: ") + hiprtcGetErrorString(cr) + "\n" +
+                                         (*out)->log);
+    }
+    size_t code_size = 0;
+    hiprtcGetCodeSize(rtc, &code_size);
+    prog->code.resize(code_size);
+    hiprtcGetCode(rtc, prog->code.data());
+    hiprtcDestroyProgram(&rtc);
+    prog->compile_ms = now_ms() - t0;
+    *out = prog.release();
+    return CVG_OK;
+}
+
+int cvg_batch_create(cvg_program* prog, const cvg_params* common, const cvg_batch_item* items, int n_items,
+                     const double* theta, int n_theta, cvg_batch** out) {
+    if (!out) return fail(CVG_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!prog || !prog->batch || prog->code.empty()) return fail(CVG_ERR_INVALID, "not a batch program");
+    if (!common || !items || n_items < 1) return fail(CVG_ERR_INVALID, "no items");
+    if (common->n_intervals != prog->nbins || common->precision != prog->precision ||
+        (common->run_len > 0 ? common->run_len : kDefaultRun) != prog->K)
+        return fail(CVG_ERR_INVALID, "common params do not match the batch program");
+    if (common->max_it < 1) return fail(CVG_ERR_INVALID, "max_it must be >= 1");
+    if (n_theta < 0 || (n_theta > 0 && !theta)) return fail(CVG_ERR_INVALID, "bad theta");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CVG_ERR_NODEVICE, "no HIP device visible");
+    int rc;
+    if ((rc = prog->load())) return rc;
+    const double t0 = now_ms();
+    auto b = std::make_unique<cvg_batch>();
+    b->prog = prog;
+    b->p = *common;
+    b->p.domain = nullptr;
+    b->p.has_domain = 0;
+    b->n_items = n_items;
+    b->K = prog->K;
+    b->k_shift = 0;
+    while ((1 << b->k_shift) < b->K) ++b->k_shift;
+    VG_HIP(hipGetDevice(&b->device));
+    const int n = common->n_intervals;
+    const long long WS = 64LL * b->K;
+    const int WPB = prog->block / 64;
+    std::vector<vg_fitem> fi((size_t)n_items);
+    std::vector<vg_bitem>& bi = b->hitems;
+    bi.resize((size_t)n_items);
+    std::vector<int> task_item, task_w0, blk_item, rd_item, rd_dim;
+    for (int i = 0; i < n_items; ++i) {
+        const cvg_batch_item& it = items[i];
+        if (it.n_dim < 1 || it.n_dim > prog->max_dim) return fail(CVG_ERR_INVALID, "item dimension out of range");
+        if (it.n_eval_it < 1) return fail(CVG_ERR_INVALID, "item budget must be >= 1");
+        if (it.theta_offset < 0 || (n_theta > 0 && it.theta_offset >= n_theta))
+            return fail(CVG_ERR_INVALID, "item theta_offset out of range");
+        int ns = it.n_strat;
+        if (ns <= 0) ns = (int)std::pow(it.n_eval_it / 4.0, 1.0 / it.n_dim);  // python_wrapper.py:15-18
+        if (ns < 1) ns = 1;
+        if (ns >= (1 << 16)) return fail(CVG_ERR_INVALID, "item n_strat must be < 65536");
+        long long C;
+        if (!checked_cubes(ns, it.n_dim, &C)) return fail(CVG_ERR_INVALID, "item n_strat^n_dim must be < 2^31");
+        const double V = std::pow(1.0 / ns, it.n_dim);
+        const double v0 = 1.0 / (double)C * (double)it.n_eval_it;
+        int nh0 = v0 != v0 ? 0 : (v0 >= 2147483647.0 ? 2147483647 : (int)v0);
+        if (nh0 < 2) nh0 = 2;
+        long long s_cap = it.n_eval_it + 2 * C + WS + 1024;
+        if ((long long)nh0 * C > s_cap) s_cap = (long long)nh0 * C;
+        const long long wcap = (s_cap + WS - 1) / WS + 1;
+        const long long nblk = (C + 1023) / 1024;
+        vg_fitem& f = fi[(size_t)i];
+        vg_bitem& g = bi[(size_t)i];
+        f.cube_off = g.cube_off = b->tot_cubes;
+        f.nhs_off = g.nhs_off = b->tot_nhs;
+        f.win_off = g.win_off = b->tot_win;
+        g.wcap = wcap;
+        g.s_cap = s_cap;
+        g.blk_off = b->tot_blk;
+        g.nblk = nblk;
+        g.xe_off = b->tot_xe;
+        f.map_off = (int)b->tot_map;
+        g.map_off = b->tot_map;
+        g.N = it.n_eval_it;
+        f.V = g.V = V;
+        f.inv_ns = 1.0 / (double)ns;
+        f.inv_ns_f = 1.0f / (float)ns;
+        f.pad = 0.0f;
+        f.d = g.d = it.n_dim;
+        f.C = g.C = (int)C;
+        g.nh0 = nh0;
+        g.pad = 0;
+        f.theta_off = it.theta_offset;
+        f.ns = (unsigned)ns;
+        udiv_magic((unsigned)ns, &f.ns_m, &f.ns_sh1, &f.ns_sh2);
+        for (long long t = 0; t * WPB < wcap; ++t) {
+            task_item.push_back(i);
+            task_w0.push_back((int)(t * WPB));
+        }
+        for (long long k = 0; k < nblk; ++k) blk_item.push_back(i);
+        for (int dd = 0; dd < it.n_dim; ++dd) { rd_item.push_back(i); rd_dim.push_back(dd); }
+        b->dims.push_back(it.n_dim);
+        b->tot_cubes += C;
+        b->tot_nhs += C + 1;
+        b->tot_win += wcap;
+        b->tot_blk += nblk;
+        b->tot_xe += (long long)it.n_dim * (n + 1);
+        b->tot_map += (long long)it.n_dim * n;
+        if (b->tot_win * 64 > 2147483647LL * 16) return fail(CVG_ERR_CAPACITY, "batch too large");
+    }
+    b->n_tasks = (int)task_item.size();
+    b->n_refine = (int)rd_item.size();
+    VG_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    b->own_stream = true;
+    VG_HIP(hipEventCreate(&b->ev0));
+    VG_HIP(hipEventCreate(&b->ev1));
+    hipStream_t s = b->stream;
+    if ((rc = upload(s, &b->fitems, fi))) return rc;
+    if ((rc = upload(s, &b->bitems, bi))) return rc;
+    if ((rc = upload(s, &b->task_item, task_item))) return rc;
+    if ((rc = upload(s, &b->task_w0, task_w0))) return rc;
+    if ((rc = upload(s, &b->blk_item, blk_item))) return rc;
+    if ((rc = upload(s, &b->rd_item, rd_item))) return rc;
+    if ((rc = upload(s, &b->rd_dim, rd_dim))) return rc;
+    if ((rc = dalloc(&b->nh, b->tot_cubes))) return rc;
+    if ((rc = dalloc(&b->nh_sum, b->tot_nhs))) return rc;
+    if ((rc = dalloc(&b->JF, b->tot_cubes))) return rc;
+    if ((rc = dalloc(&b->JF2, b->tot_cubes))) return rc;
+    if ((rc = dalloc(&b->dh, b->tot_cubes))) return rc;
+    if ((rc = dalloc(&b->run_cube, b->tot_win * 64))) return rc;
+    if ((rc = dalloc(&b->win_first, b->tot_win))) return rc;
+    if ((rc = dalloc(&b->win_last, b->tot_win))) return rc;
+    if ((rc = dalloc(&b->part, 3 * b->tot_blk))) return rc;
+    if ((rc = dalloc(&b->bsum, b->tot_blk))) return rc;
+    if ((rc = dalloc(&b->boff, b->tot_blk))) return rc;
+    if ((rc = dalloc(&b->dsum, n_items))) return rc;
+    if ((rc = dalloc(&b->s_item, n_items))) return rc;
+    if ((rc = dalloc(&b->it_p, 1))) return rc;
+    if ((rc = dalloc(&b->err, 1))) return rc;
+    if ((rc = dalloc(&b->hist, 4LL * n_items * common->max_it))) return rc;
+    if ((rc = dalloc(&b->x_all, b->tot_xe))) return rc;
+    if ((rc = dalloc(&b->map64, b->tot_map))) return rc;
+    if ((rc = dalloc(&b->map32, b->tot_map))) return rc;
+    if ((rc = dalloc(&b->weights, b->tot_map))) return rc;
+    if ((rc = dalloc(&b->counts, b->tot_map))) return rc;
+    b->fold_cap = (int)(b->tot_win / 64 + 2 * n_items + 2);
+    if ((rc = dalloc(&b->fold_list, b->fold_cap))) return rc;
+    if ((rc = dalloc(&b->fold_count, 1))) return rc;
+    const int nth = n_theta > 0 ? n_theta : 1;
+    if ((rc = dalloc((char**)&b->theta, 8LL * nth))) return rc;
+    if (n_theta > 0) {
+        if (prog->precision == CVG_FP32C) {
+            std::vector<float> tf(theta, theta + n_theta);
+            VG_HIP(hipMemcpyAsync(b->theta, tf.data(), 4 * (size_t)n_theta, hipMemcpyHostToDevice, s));
+            VG_HIP(hipStreamSynchronize(s));
+        } else {
+            VG_HIP(hipMemcpyAsync(b->theta, theta, 8 * (size_t)n_theta, hipMemcpyHostToDevice, s));
+        }
+    }
+    VG_HIP(hipMemsetAsync(b->it_p, 0, sizeof(int), s));
+    VG_HIP(hipMemsetAsync(b->err, 0, sizeof(int), s));
+    if ((rc = b->steps(0))) return rc;
+    VG_HIP(hipStreamSynchronize(s));
+    b->setup_ms = now_ms() - t0;
+    *out = b.release();
+    return CVG_OK;
+}
+
+void cvg_batch_destroy(cvg_batch* b) {
+    if (b && b->stream) (void)hipStreamSynchronize(b->stream);
+    delete b;
+}
+
+int cvg_batch_run(cvg_batch* b, int n_iter, int use_graph) {
+    if (!b) return fail(CVG_ERR_INVALID, "batch is NULL");
+    if (n_iter < 0) return fail(CVG_ERR_INVALID, "n_iter must be >= 0");
+    if (b->iterations_done + n_iter > b->p.max_it) return fail(CVG_ERR_CAPACITY, "more iterations than max_it");
+    if (n_iter == 0) return CVG_OK;
+    int rc;
+    if (use_graph && !b->graph_exec) {
+        VG_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        rc = b->iteration();
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(b->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ce != hipSuccess) return fail(CVG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        b->graph = g;
+        VG_HIP(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+    }
+    VG_HIP(hipEventRecord(b->ev0, b->stream));
+    for (int i = 0; i < n_iter; ++i) {
+        if (use_graph) {
+            VG_HIP(hipGraphLaunch(b->graph_exec, b->stream));
+        } else if ((rc = b->iteration())) {
+            return rc;
+        }
+    }
+    VG_HIP(hipEventRecord(b->ev1, b->stream));
+    VG_HIP(hipEventSynchronize(b->ev1));
+    float ms = 0.0f;
+    VG_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->loop_ms += ms;
+    b->iterations_done += n_iter;
+    int h_err = 0;
+    VG_HIP(hipMemcpy(&h_err, b->err, sizeof(int), hipMemcpyDeviceToHost));
+    if (h_err & VG_ERR_CAPACITY) return fail(CVG_ERR_CAPACITY, "an item's allocation exceeded its buffers");
+    if (h_err & VG_ERR_HIST) return fail(CVG_ERR_CAPACITY, "more iterations than max_it");
+    return CVG_OK;
+}
+
+int cvg_batch_history(cvg_batch* b, int item, double* I, double* sigma2, long long* nevals, int cap) {
+    if (!b || item < 0 || item >= b->n_items) return fail(CVG_ERR_INVALID, "bad item");
+    const int n = b->iterations_done;
+    if (n == 0) return 0;
+    std::vector<double> h(4 * (size_t)n);
+    VG_HIP(hipMemcpy(h.data(), b->hist + (size_t)item * b->p.max_it * 4, sizeof(double) * h.size(),
+                     hipMemcpyDeviceToHost));
+    for (int i = 0; i < n && i < cap; ++i) {
+        if (I) I[i] = h[4 * i];
+        if (sigma2) sigma2[i] = h[4 * i + 1];
+        if (nevals) nevals[i] = (long long)h[4 * i + 3];
+    }
+    return n;
+}
+
+int cvg_batch_results(cvg_batch* b, double* result, double* error, double* chi2_dof, long long* nevals,
+                      double* loop_ms) {
+    if (!b) return fail(CVG_ERR_INVALID, "batch is NULL");
+    VG_HIP(hipStreamSynchronize(b->stream));
+    const int n = b->iterations_done, cap = b->p.max_it;
+    std::vector<double> h(4 * (size_t)b->n_items * cap);
+    VG_HIP(hipMemcpy(h.data(), b->hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->n_items; ++i) {
+        const double* row = h.data() + (size_t)i * cap * 4;
+        double res = 0.0, sig = 0.0, chi2 = 0.0;
+        int Kc = 0;
+        long long tot = 0;
+        for (int k = 0; k < n; ++k) tot += (long long)row[4 * k + 3];
+        for (int k = b->p.skip; k < n; ++k) {
+            res += row[4 * k] / row[4 * k + 1];
+            sig += 1.0 / row[4 * k + 1];
+            ++Kc;
+        }
+        if (Kc > 0) res /= sig;
+        for (int k = b->p.skip; k < n; ++k) {
+            const double dd = row[4 * k] - res;
+            chi2 += dd * dd / row[4 * k + 1];
+        }
+        if (result) result[i] = res;
+        if (error) error[i] = 1.0 / std::sqrt(sig);
+        if (chi2_dof) chi2_dof[i] = Kc > 1 ? chi2 / (Kc - 1) : 0.0;
+        if (nevals) nevals[i] = tot;
+    }
+    if (loop_ms) *loop_ms = b->loop_ms;
+    return CVG_OK;
+}
+
+int cvg_batch_item_info(cvg_batch* b, int item, int* n_dim, int* n_strat, long long* n_cubes) {
+    if (!b || item < 0 || item >= b->n_items) return fail(CVG_ERR_INVALID, "bad item");
+    const vg_bitem& g = b->hitems[(size_t)item];
+    if (n_dim) *n_dim = g.d;
+    if (n_cubes) *n_cubes = g.C;
+    if (n_strat) {
+        int ns = 1;
+        while (true) {
+            long long c;
+            if (!checked_cubes(ns + 1, g.d, &c) || c > g.C) break;
+            ++ns;
+        }
+        *n_strat = ns;
+    }
+    return CVG_OK;
+}
+
+int cvg_batch_get_map(cvg_batch* b, int item, double* x_edges) {
+    if (!b || !x_edges || item < 0 || item >= b->n_items) return fail(CVG_ERR_INVALID, "bad argument");
+    const vg_bitem& g = b->hitems[(size_t)item];
+    VG_HIP(hipStreamSynchronize(b->stream));
+    VG_HIP(hipMemcpy(x_edges, b->x_all + g.xe_off, sizeof(double) * g.d * (b->p.n_intervals + 1),
+                     hipMemcpyDeviceToHost));
+    return CVG_OK;
+}
+
+int cvg_batch_get_nh(cvg_batch* b, int item, int* nh) {
+    if (!b || !nh || item < 0 || item >= b->n_items) return fail(CVG_ERR_INVALID, "bad argument");
+    const vg_bitem& g = b->hitems[(size_t)item];
+    VG_HIP(hipStreamSynchronize(b->stream));
+    VG_HIP(hipMemcpy(nh, b->nh + g.cube_off, sizeof(int) * g.C, hipMemcpyDeviceToHost));
+    return CVG_OK;
+}
+
+int cvg_batch_get_cube_sums(cvg_batch* b, int item, double* jf, double* jf2) {
+    if (!b || !jf || !jf2 || item < 0 || item >= b->n_items) return fail(CVG_ERR_INVALID, "bad argument");
+    const vg_bitem& g = b->hitems[(size_t)item];
+    VG_HIP(hipStreamSynchronize(b->stream));
+    VG_HIP(hipMemcpy(jf, b->JF + g.cube_off, sizeof(double) * g.C, hipMemcpyDeviceToHost));
+    VG_HIP(hipMemcpy(jf2, b->JF2 + g.cube_off, sizeof(double) * g.C, hipMemcpyDeviceToHost));
     return CVG_OK;
 }
 

@@ -15,7 +15,11 @@ hipError_t vg_launch_alloc(hipStream_t st, int C, int mode, int nh_const, const 
                            const double* dsum, long long N, int* nh, long long* bsum);
 hipError_t vg_launch_scan(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
                           long long* s_total, long long s_cap, int* it_p, int advance, int k_shift,
-                          long long run_cap, long long* nh_sum, int* run_cube, int* err);
+                          long long run_cap, long long* nh_sum, int* run_cube, int* err,
+                          int* fold_list, int* fold_count, int fold_cap);
+hipError_t vg_launch_bigfold(hipStream_t st, const int* list, const int* count, int cap,
+                             const long long* nh_sum, const double2* win_first,
+                             const double2* win_last, int ws_shift, double* JF, double* JF2);
 hipError_t vg_launch_init_map(hipStream_t st, int d, int n, double* x_edges, double2* map64,
                               float2* map32, double* weights, unsigned* counts);
 hipError_t vg_launch_refine(hipStream_t st, int d, int n, double alpha, double* x_edges,
@@ -26,6 +30,36 @@ hipError_t vg_launch_tables(hipStream_t st, int d, int n, const double* x_edges,
 long long vg_refine_scratch_doubles(int d, int n);
 hipError_t vg_launch_layout(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
                             long long* s_total, long long s_cap, int* it_p, int k_shift, long long run_cap,
-                            long long* nh_sum, int* run_cube, int* err);
+                            long long* nh_sum, int* run_cube, int* err, int* fold_list,
+                            int* fold_count, int fold_cap);
 hipError_t vg_launch_snapshot(hipStream_t st, const int* it_p, int skip, int cap, int C, int nx,
                               const double* x_edges, const int* nh, double* snap_x, int* snap_nh);
+
+// Host-side descriptor of one item of a batch (the step kernels' view).
+struct vg_bitem {
+    long long cube_off;  // first cube of the item
+    long long nhs_off;   // first nh_sum entry (C + 1 per item)
+    long long win_off;   // first window
+    long long wcap;      // window capacity (runs = 64 * wcap)
+    long long s_cap;     // sample capacity
+    long long blk_off;   // first 1024-cube block
+    long long nblk;      // number of 1024-cube blocks
+    long long xe_off;    // first map edge (d * (n + 1) per item)
+    long long map_off;   // first (dim, bin) entry of maps and bins
+    long long N;         // samples per iteration
+    double V;            // cube volume
+    int C, d, nh0, pad;
+};
+
+hipError_t vg_launch_batch_iteration_steps(hipStream_t st, const vg_bitem* items, int n_items,
+                                           long long nblk, const int* blk_item, int n_refine,
+                                           const int* rd_item, const int* rd_dim, int n,
+                                           double alpha, double beta, int k_shift, int hist_cap,
+                                           int* nh, long long* nh_sum, double* JF, double* JF2,
+                                           const double2* win_first, const double2* win_last,
+                                           double* dh, double* part, double* dsum, long long* bsum,
+                                           long long* boff, long long* s_item, int* run_cube,
+                                           int* it_p, double* hist, double* x_all,
+                                           double* weights, unsigned* counts, double2* map64,
+                                           float2* map32, int* err, int phase,
+                                           long long* fold_list, int* fold_count, int fold_cap);

@@ -11,6 +11,7 @@
 #include "vg_steps.h"
 
 #define VG_TREE 1024
+#define VG_FOLD_MAX 64  // cubes spanning more windows are folded by a tree (vg_bigfold)
 
 // In-place strided pairwise tree over t[0..1023] (block of VG_TREE threads).
 __device__ inline void vg_tree1024(double* t) {
@@ -87,14 +88,15 @@ __device__ inline void vg_cube_result(long long h, int C, double V, double beta,
                                       double* __restrict__ JF, double* __restrict__ JF2,
                                       const double2* __restrict__ win_first,
                                       const double2* __restrict__ win_last, int ws_shift,
-                                      double* __restrict__ dh, double& sr, double& ss, double& dd) {
+                                      double* __restrict__ dh, double& sr, double& ss, double& dd,
+                                      long long win_off = 0) {
     sr = 0.0; ss = 0.0; dd = 0.0;
     if (h >= C) return;
     const int neval = nh[h];
     const long long st = nh_sum[h], en = nh_sum[h + 1];
-    const long long wa = st >> ws_shift, wb = (en - 1) >> ws_shift;
+    const long long wa = (st >> ws_shift) + win_off, wb = ((en - 1) >> ws_shift) + win_off;
     double j1, j2;
-    if (wa == wb) {
+    if (wa == wb || wb - wa + 1 > VG_FOLD_MAX) {  // inside one window, or folded by vg_bigfold
         j1 = JF[h];
         j2 = JF2[h];
     } else {  // left fold of the window partials; loads batched ahead of the adds
@@ -146,6 +148,70 @@ vg_results_kernel(int C, double V, double beta, const int* __restrict__ nh,
         const double tot = vg_tree1024_256(r[0][q], r[1][q], r[2][q], r[3][q], red);
         if (threadIdx.x == 0) part[3 * blockIdx.x + q] = tot;
     }
+}
+
+// Cubes whose samples span more than VG_FOLD_MAX windows: the window partials
+// win_last[wa], win_first[wa+1..wb] are summed with the 1024-wide tree, level
+// after level, one block per cube (grid-stride over the list).
+__device__ void vg_fold_tree(long long wa, long long m, const double2* __restrict__ win_first,
+                             const double2* __restrict__ win_last, double* part, double* red,
+                             double& o1, double& o2) {
+    // level 1 from the window partials, later levels from part[] (2 x 2048 doubles:
+    // m <= 2^21 windows, which holds for any allocation below 2^31 samples)
+    long long cur = m;
+    bool first = true;
+    while (true) {
+        const long long nch = (cur + VG_TREE - 1) / VG_TREE;
+        for (long long c = 0; c < nch; ++c) {
+            double v1[4], v2[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long k = c * VG_TREE + threadIdx.x + 256 * j;
+                if (k >= cur) { v1[j] = 0.0; v2[j] = 0.0; }
+                else if (first) {
+                    const double2 v = k == 0 ? win_last[wa] : win_first[wa + k];
+                    v1[j] = v.x; v2[j] = v.y;
+                } else { v1[j] = part[2 * k]; v2[j] = part[2 * k + 1]; }
+            }
+            __syncthreads();
+            const double t1 = vg_tree1024_256(v1[0], v1[1], v1[2], v1[3], red);
+            const double t2 = vg_tree1024_256(v2[0], v2[1], v2[2], v2[3], red);
+            if (threadIdx.x == 0) { part[2 * c] = t1; part[2 * c + 1] = t2; }
+            __syncthreads();
+        }
+        if (nch == 1) break;
+        cur = nch;
+        first = false;
+    }
+    o1 = part[0];
+    o2 = part[1];
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256)
+vg_bigfold_kernel(const int* __restrict__ list, const int* __restrict__ count, int cap,
+                  const long long* __restrict__ nh_sum, const double2* __restrict__ win_first,
+                  const double2* __restrict__ win_last, int ws_shift, double* __restrict__ JF,
+                  double* __restrict__ JF2) {
+    __shared__ double red[256];
+    __shared__ double part[2 * 2048];
+    const int n = *count < cap ? *count : cap;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int h = list[i];
+        const long long st = nh_sum[h], en = nh_sum[h + 1];
+        const long long wa = st >> ws_shift, wb = (en - 1) >> ws_shift;
+        double o1, o2;
+        vg_fold_tree(wa, wb - wa + 1, win_first, win_last, part, red, o1, o2);
+        if (threadIdx.x == 0) { JF[h] = o1; JF2[h] = o2; }
+    }
+}
+
+hipError_t vg_launch_bigfold(hipStream_t st, const int* list, const int* count, int cap,
+                             const long long* nh_sum, const double2* win_first,
+                             const double2* win_last, int ws_shift, double* JF, double* JF2) {
+    hipLaunchKernelGGL(vg_bigfold_kernel, dim3(128), dim3(256), 0, st, list, count, cap, nh_sum,
+                       win_first, win_last, ws_shift, JF, JF2);
+    return hipGetLastError();
 }
 
 // Remaining tree levels over the block partials, then the iteration record:
@@ -233,7 +299,7 @@ vg_alloc_kernel(int C, int mode, int nh_const, const double* __restrict__ dh,
 __global__ void __launch_bounds__(VG_TREE)
 vg_scan_blocks_kernel(long long nb, const long long* __restrict__ bsum, long long* __restrict__ boff,
                       long long* __restrict__ s_total, long long s_cap, int* __restrict__ it_p,
-                      int advance, int* __restrict__ err) {
+                      int advance, int* __restrict__ err, int* __restrict__ fold_count) {
     __shared__ long long wt[VG_TREE / 64];
     const long long per = (nb + VG_TREE - 1) / VG_TREE;
     const long long b0 = threadIdx.x * per;
@@ -246,6 +312,7 @@ vg_scan_blocks_kernel(long long nb, const long long* __restrict__ bsum, long lon
         if (total > s_cap) { atomicOr(err, VG_ERR_CAPACITY); total = s_cap; }
         *s_total = total;
         if (advance) *it_p = *it_p + 1;
+        *fold_count = 0;
     }
 }
 
@@ -256,7 +323,8 @@ vg_scan_blocks_kernel(long long nb, const long long* __restrict__ bsum, long lon
 __global__ void __launch_bounds__(256)
 vg_scan_cubes_kernel(int C, const int* __restrict__ nh, const long long* __restrict__ boff,
                      const long long* __restrict__ s_total, int k_shift, long long run_cap,
-                     long long* __restrict__ nh_sum, int* __restrict__ run_cube) {
+                     long long* __restrict__ nh_sum, int* __restrict__ run_cube,
+                     int* __restrict__ fold_list, int* __restrict__ fold_count, int fold_cap) {
     __shared__ long long wt[4];
     __shared__ int big[VG_TREE];
     __shared__ long long big_r0[VG_TREE], big_r1[VG_TREE];
@@ -293,6 +361,10 @@ vg_scan_cubes_kernel(int C, const int* __restrict__ nh, const long long* __restr
                 big_r1[slot] = r1;
             } else {
                 for (long long r = r0; r < r1; ++r) run_cube[r] = (int)h;
+            }
+            if (en > st && ((en - 1) >> (k_shift + 6)) - (st >> (k_shift + 6)) + 1 > VG_FOLD_MAX) {
+                const int slot = atomicAdd(fold_count, 1);
+                if (slot < fold_cap) fold_list[slot] = (int)h;
             }
         }
         ex += v[j];
@@ -365,23 +437,14 @@ __device__ double vg_block_tree_sum(const double* __restrict__ a, int m, double*
     return r;
 }
 
-// Refinement of one dimension per block.  Working arrays (normalised weights
-// w, damped weights d, cumulative sums, old edges) live in dynamic LDS when
-// they fit (lds != 0), else in the global scratch.
-__global__ void __launch_bounds__(VG_TREE)
-vg_refine_kernel(int n, double alpha, int use_lds, double* __restrict__ x_edges,
-                 double* __restrict__ weights, unsigned* __restrict__ counts,
-                 double* __restrict__ scratch, double2* __restrict__ map64,
-                 float2* __restrict__ map32) {
-    __shared__ double t[VG_TREE];
-    extern __shared__ double dyn[];
-    const int dim = blockIdx.x;
+// Refinement of one dimension: normalise the bin sums, smooth, damp, cumulate,
+// invert (reference normalizeWeights / smoothWeights / setMap / updateXEdges /
+// updateDxEdges, cuda_wrapper.cu:51-90,134-191).  base holds the working
+// arrays w[n], d[n], cum[n], xo[n+1], part[n/1024 + 2] (LDS or global).
+__device__ void vg_refine_dim(int n, double alpha, double* __restrict__ x, double* __restrict__ wb,
+                              unsigned* __restrict__ cb, double* base, double2* __restrict__ map64,
+                              float2* __restrict__ map32, double* t) {
     const int E = n + 1;
-    double* x = x_edges + (long long)dim * E;
-    double* wb = weights + (long long)dim * n;
-    unsigned* cb = counts + (long long)dim * n;
-    // per dim: w[n], d[n], cum[n], xo[n+1], part[n/1024 + 2]
-    double* base = use_lds ? dyn : scratch + (long long)dim * (4LL * n + 1 + n / VG_TREE + 2);
     double* __restrict__ w = base;
     double* __restrict__ d = w + n;
     double* __restrict__ cum = d + n;
@@ -435,7 +498,23 @@ vg_refine_kernel(int n, double alpha, int use_lds, double* __restrict__ x_edges,
         }
     }
     __syncthreads();
-    vg_write_tables(n, x, map64 + (long long)dim * n, map32 + (long long)dim * n);
+    vg_write_tables(n, x, map64, map32);
+}
+
+// One dimension per block; working arrays in dynamic LDS when they fit
+// (use_lds != 0), else in the global scratch.
+__global__ void __launch_bounds__(VG_TREE)
+vg_refine_kernel(int n, double alpha, int use_lds, double* __restrict__ x_edges,
+                 double* __restrict__ weights, unsigned* __restrict__ counts,
+                 double* __restrict__ scratch, double2* __restrict__ map64,
+                 float2* __restrict__ map32) {
+    __shared__ double t[VG_TREE];
+    extern __shared__ double dyn[];
+    const int dim = blockIdx.x;
+    double* base = use_lds ? dyn : scratch + (long long)dim * (4LL * n + 1 + n / VG_TREE + 2);
+    vg_refine_dim(n, alpha, x_edges + (long long)dim * (n + 1), weights + (long long)dim * n,
+                  counts + (long long)dim * n, base, map64 + (long long)dim * n,
+                  map32 + (long long)dim * n, t);
 }
 
 long long vg_refine_scratch_doubles(int d, int n) {
@@ -473,14 +552,15 @@ hipError_t vg_launch_alloc(hipStream_t st, int C, int mode, int nh_const, const 
 
 hipError_t vg_launch_scan(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
                           long long* s_total, long long s_cap, int* it_p, int advance, int k_shift,
-                          long long run_cap, long long* nh_sum, int* run_cube, int* err) {
+                          long long run_cap, long long* nh_sum, int* run_cube, int* err,
+                          int* fold_list, int* fold_count, int fold_cap) {
     const long long nb = (C + VG_TREE - 1) / VG_TREE;
     hipLaunchKernelGGL(vg_scan_blocks_kernel, dim3(1), dim3(VG_TREE), 0, st, nb, bsum, boff, s_total,
-                       s_cap, it_p, advance, err);
+                       s_cap, it_p, advance, err, fold_count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(vg_scan_cubes_kernel, dim3((unsigned)nb), dim3(256), 0, st, C, nh, boff,
-                       s_total, k_shift, run_cap, nh_sum, run_cube);
+                       s_total, k_shift, run_cap, nh_sum, run_cube, fold_list, fold_count, fold_cap);
     return hipGetLastError();
 }
 
@@ -538,13 +618,14 @@ vg_bsum_kernel(int C, const int* __restrict__ nh, long long* __restrict__ bsum) 
 
 hipError_t vg_launch_layout(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
                             long long* s_total, long long s_cap, int* it_p, int k_shift, long long run_cap,
-                            long long* nh_sum, int* run_cube, int* err) {
+                            long long* nh_sum, int* run_cube, int* err, int* fold_list,
+                            int* fold_count, int fold_cap) {
     const unsigned nb = (unsigned)((C + VG_TREE - 1) / VG_TREE);
     hipLaunchKernelGGL(vg_bsum_kernel, dim3(nb), dim3(VG_TREE), 0, st, C, nh, bsum);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return vg_launch_scan(st, C, nh, bsum, boff, s_total, s_cap, it_p, 0, k_shift, run_cap, nh_sum,
-                          run_cube, err);
+                          run_cube, err, fold_list, fold_count, fold_cap);
 }
 
 // Copy the map edges and allocation that the coming sampling pass will use
@@ -570,5 +651,293 @@ hipError_t vg_launch_snapshot(hipStream_t st, const int* it_p, int skip, int cap
     if (grid > 4096) grid = 4096;
     hipLaunchKernelGGL(vg_snapshot_kernel, dim3(grid), dim3(256), 0, st, it_p, skip, cap, C, nx,
                        x_edges, nh, snap_x, snap_nh);
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// Batched steps: many independent integrals (items) per launch.  Each item
+// keeps its own cube range, block range, map and bins; every per-item
+// reduction uses the same 1024-wide trees as the single-integral kernels, so
+// an item reproduces a single-integral run bit for bit.
+// ===========================================================================
+
+__global__ void __launch_bounds__(256)
+vg_bresults_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ blk_item,
+                   const int* __restrict__ nh, const long long* __restrict__ nh_sum,
+                   double* __restrict__ JF, double* __restrict__ JF2,
+                   const double2* __restrict__ win_first, const double2* __restrict__ win_last,
+                   int ws_shift, double beta, double* __restrict__ dh, double* __restrict__ part) {
+    __shared__ double red[256];
+    const vg_bitem I = items[blk_item[blockIdx.x]];
+    const long long lb = blockIdx.x - I.blk_off;
+    const long long base = lb * VG_TREE + threadIdx.x;
+    double r[4][3];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        vg_cube_result(base + 256 * j, I.C, I.V, beta, nh + I.cube_off, nh_sum + I.nhs_off,
+                       JF + I.cube_off, JF2 + I.cube_off, win_first, win_last, ws_shift,
+                       dh + I.cube_off, r[j][0], r[j][1], r[j][2], I.win_off);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double tot = vg_tree1024_256(r[0][q], r[1][q], r[2][q], r[3][q], red);
+        if (threadIdx.x == 0) part[3 * blockIdx.x + q] = tot;
+    }
+}
+
+// One block per item: remaining tree levels over the item's block partials
+// (in place), then hist[(item * cap + it) * 4 + q] and dsum[item].
+__global__ void __launch_bounds__(256)
+vg_bfinalize_kernel(const vg_bitem* __restrict__ items, double* __restrict__ part_all,
+                    const long long* __restrict__ s_item, const int* __restrict__ it_p, int hist_cap,
+                    double* __restrict__ hist, double* __restrict__ dsum, int* __restrict__ err) {
+    __shared__ double red[256];
+    const int item = blockIdx.x;
+    const vg_bitem I = items[item];
+    double* part = part_all + 3 * I.blk_off;
+    long long cur = I.nblk;
+    while (cur > 1) {
+        const long long nch = (cur + VG_TREE - 1) / VG_TREE;
+        for (long long c = 0; c < nch; ++c) {
+            double v[4][3];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long i = c * VG_TREE + threadIdx.x + 256 * j;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) v[j][q] = i < cur ? part[3 * i + q] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const double tot = vg_tree1024_256(v[0][q], v[1][q], v[2][q], v[3][q], red);
+                if (threadIdx.x == 0) part[3 * c + q] = tot;
+            }
+            __syncthreads();
+        }
+        cur = nch;
+    }
+    if (threadIdx.x == 0) {
+        const int it = *it_p;
+        dsum[item] = part[2];
+        if (it < hist_cap) {
+            double* hrow = hist + ((long long)item * hist_cap + it) * 4;
+            hrow[0] = part[0];
+            hrow[1] = part[1];
+            hrow[2] = part[2];
+            hrow[3] = (double)s_item[item];
+        } else {
+            atomicOr(err, VG_ERR_HIST);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+vg_balloc_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ blk_item, int mode,
+                 const double* __restrict__ dh, const double* __restrict__ dsum,
+                 int* __restrict__ nh, long long* __restrict__ bsum) {
+    __shared__ long long red[4];
+    const int item = blk_item[blockIdx.x];
+    const vg_bitem I = items[item];
+    const long long lb = blockIdx.x - I.blk_off;
+    const long long base = lb * VG_TREE + threadIdx.x;
+    const double D = mode ? dsum[item] : 1.0;
+    long long v = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const long long h = base + 256 * j;
+        if (h < I.C) {
+            int n;
+            if (mode == 0) {
+                n = I.nh0;
+            } else {
+                const double f = dh[I.cube_off + h] / D;
+                n = vg_sat_int(f * (double)I.N);
+                n = n < 2 ? 2 : n;
+            }
+            nh[I.cube_off + h] = n;
+            v += n;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// One block per item: exclusive scan of the item's block sums; total ->
+// s_item[item] (capped); item 0 advances the iteration counter.
+__global__ void __launch_bounds__(VG_TREE)
+vg_bscan_blocks_kernel(const vg_bitem* __restrict__ items, const long long* __restrict__ bsum,
+                       long long* __restrict__ boff, long long* __restrict__ s_item,
+                       int* __restrict__ it_p, int advance, int* __restrict__ err,
+                       int* __restrict__ fold_count) {
+    __shared__ long long wt[VG_TREE / 64];
+    const int item = blockIdx.x;
+    const vg_bitem I = items[item];
+    const long long nb = I.nblk, off = I.blk_off;
+    const long long per = (nb + VG_TREE - 1) / VG_TREE;
+    const long long b0 = threadIdx.x * per;
+    long long local = 0;
+    for (long long b = b0; b < b0 + per && b < nb; ++b) local += bsum[off + b];
+    long long total;
+    long long acc = vg_block_scan_excl(local, wt, &total);
+    for (long long b = b0; b < b0 + per && b < nb; ++b) { boff[off + b] = acc; acc += bsum[off + b]; }
+    if (threadIdx.x == 0) {
+        if (total > I.s_cap) { atomicOr(err, VG_ERR_CAPACITY); total = I.s_cap; }
+        s_item[item] = total;
+    }
+    __syncthreads();
+    if (item == 0 && threadIdx.x == 0) {
+        if (advance) *it_p = *it_p + 1;
+        *fold_count = 0;
+    }
+}
+
+// Batched counterpart of vg_bigfold_kernel: list entries are (item << 32) | cube.
+__global__ void __launch_bounds__(256)
+vg_bbigfold_kernel(const vg_bitem* __restrict__ items, const long long* __restrict__ list,
+                   const int* __restrict__ count, int cap, const long long* __restrict__ nh_sum,
+                   const double2* __restrict__ win_first, const double2* __restrict__ win_last,
+                   int ws_shift, double* __restrict__ JF, double* __restrict__ JF2) {
+    __shared__ double red[256];
+    __shared__ double part[2 * 2048];
+    const int n = *count < cap ? *count : cap;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int item = (int)(list[i] >> 32);
+        const int h = (int)(list[i] & 0xffffffffLL);
+        const vg_bitem I = items[item];
+        const long long st = nh_sum[I.nhs_off + h], en = nh_sum[I.nhs_off + h + 1];
+        const long long wa = (st >> ws_shift) + I.win_off, wb = ((en - 1) >> ws_shift) + I.win_off;
+        double o1, o2;
+        vg_fold_tree(wa, wb - wa + 1, win_first, win_last, part, red, o1, o2);
+        if (threadIdx.x == 0) { JF[I.cube_off + h] = o1; JF2[I.cube_off + h] = o2; }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ blk_item,
+                      const int* __restrict__ nh_all, const long long* __restrict__ boff,
+                      const long long* __restrict__ s_item, int k_shift,
+                      long long* __restrict__ nh_sum_all, int* __restrict__ run_cube_all,
+                      long long* __restrict__ fold_list, int* __restrict__ fold_count, int fold_cap) {
+    __shared__ long long wt[4];
+    __shared__ int big[VG_TREE];
+    __shared__ long long big_r0[VG_TREE], big_r1[VG_TREE];
+    __shared__ int nbig;
+    const int item = blk_item[blockIdx.x];
+    const vg_bitem I = items[item];
+    const int* nh = nh_all + I.cube_off;
+    long long* nh_sum = nh_sum_all + I.nhs_off;
+    int* run_cube = run_cube_all + I.win_off * 64;
+    const long long run_cap = I.wcap * 64;
+    const int C = I.C;
+    const long long base = (blockIdx.x - I.blk_off) * VG_TREE;
+    const long long h0 = base + 4 * threadIdx.x;
+    long long v[4];
+    long long own = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        v[j] = h0 + j < C ? nh[h0 + j] : 0;
+        own += v[j];
+    }
+    if (threadIdx.x == 0) nbig = 0;
+    long long total;
+    const long long S = s_item[item];
+    long long ex = vg_block_scan_excl(own, wt, &total) + boff[blockIdx.x];
+    const long long kmask = (1LL << k_shift) - 1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const long long h = h0 + j;
+        if (h < C) {
+            const long long st = ex < S ? ex : S;
+            const long long en = ex + v[j] < S ? ex + v[j] : S;
+            nh_sum[h] = st;
+            if (h == C - 1) nh_sum[C] = S;
+            const long long r0 = (st + kmask) >> k_shift;
+            long long r1 = (en + kmask) >> k_shift;
+            if (r1 > run_cap) r1 = run_cap;
+            if (r1 - r0 > 16) {
+                const int slot = atomicAdd(&nbig, 1);
+                big[slot] = (int)(h - base);
+                big_r0[slot] = r0;
+                big_r1[slot] = r1;
+            } else {
+                for (long long r = r0; r < r1; ++r) run_cube[r] = (int)h;
+            }
+            if (en > st && ((en - 1) >> (k_shift + 6)) - (st >> (k_shift + 6)) + 1 > VG_FOLD_MAX) {
+                const int slot = atomicAdd(fold_count, 1);
+                if (slot < fold_cap) fold_list[slot] = ((long long)item << 32) | (long long)h;
+            }
+        }
+        ex += v[j];
+    }
+    __syncthreads();
+    const int nbg = nbig;
+    for (int i = 0; i < nbg; ++i) {
+        const int h = (int)(base + big[i]);
+        for (long long r = big_r0[i] + threadIdx.x; r < big_r1[i]; r += 256) run_cube[r] = h;
+    }
+}
+
+// One block per (item, dimension).
+__global__ void __launch_bounds__(VG_TREE)
+vg_brefine_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ rd_item,
+                  const int* __restrict__ rd_dim, int n, double alpha, int init,
+                  double* __restrict__ x_all, double* __restrict__ weights,
+                  unsigned* __restrict__ counts, double2* __restrict__ map64,
+                  float2* __restrict__ map32) {
+    __shared__ double t[VG_TREE];
+    extern __shared__ double dyn[];
+    const vg_bitem I = items[rd_item[blockIdx.x]];
+    const int dim = rd_dim[blockIdx.x];
+    double* x = x_all + I.xe_off + (long long)dim * (n + 1);
+    const long long mo = I.map_off + (long long)dim * n;
+    if (init) {
+        const double step = 1.0 / n;
+        for (int j = threadIdx.x; j <= n; j += blockDim.x) x[j] = j * step;
+        for (int k = threadIdx.x; k < n; k += blockDim.x) { weights[mo + k] = 0.0; counts[mo + k] = 0u; }
+        __syncthreads();
+        vg_write_tables(n, x, map64 + mo, map32 + mo);
+        return;
+    }
+    vg_refine_dim(n, alpha, x, weights + mo, counts + mo, dyn, map64 + mo, map32 + mo, t);
+}
+
+hipError_t vg_launch_batch_iteration_steps(hipStream_t st, const vg_bitem* items, int n_items,
+                                           long long nblk, const int* blk_item, int n_refine,
+                                           const int* rd_item, const int* rd_dim, int n,
+                                           double alpha, double beta, int k_shift, int hist_cap,
+                                           int* nh, long long* nh_sum, double* JF, double* JF2,
+                                           const double2* win_first, const double2* win_last,
+                                           double* dh, double* part, double* dsum, long long* bsum,
+                                           long long* boff, long long* s_item, int* run_cube,
+                                           int* it_p, double* hist, double* x_all,
+                                           double* weights, unsigned* counts, double2* map64,
+                                           float2* map32, int* err, int phase,
+                                           long long* fold_list, int* fold_count, int fold_cap) {
+    const size_t lds = sizeof(double) * (size_t)(4LL * n + 1 + n / VG_TREE + 2);
+    if (phase == 0) {  // initial map and allocation
+        hipLaunchKernelGGL(vg_brefine_kernel, dim3(n_refine), dim3(256), lds, st, items, rd_item, rd_dim,
+                           n, alpha, 1, x_all, weights, counts, map64, map32);
+        hipLaunchKernelGGL(vg_balloc_kernel, dim3((unsigned)nblk), dim3(256), 0, st, items, blk_item, 0,
+                           dh, dsum, nh, bsum);
+    } else {
+        hipLaunchKernelGGL(vg_brefine_kernel, dim3(n_refine), dim3(VG_TREE), lds, st, items, rd_item,
+                           rd_dim, n, alpha, 0, x_all, weights, counts, map64, map32);
+        hipLaunchKernelGGL(vg_bbigfold_kernel, dim3(128), dim3(256), 0, st, items, fold_list, fold_count,
+                           fold_cap, nh_sum, win_first, win_last, 6 + k_shift, JF, JF2);
+        hipLaunchKernelGGL(vg_bresults_kernel, dim3((unsigned)nblk), dim3(256), 0, st, items, blk_item,
+                           nh, nh_sum, JF, JF2, win_first, win_last, 6 + k_shift, beta, dh, part);
+        hipLaunchKernelGGL(vg_bfinalize_kernel, dim3(n_items), dim3(256), 0, st, items, part, s_item,
+                           it_p, hist_cap, hist, dsum, err);
+        hipLaunchKernelGGL(vg_balloc_kernel, dim3((unsigned)nblk), dim3(256), 0, st, items, blk_item, 1,
+                           dh, dsum, nh, bsum);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(vg_bscan_blocks_kernel, dim3(n_items), dim3(VG_TREE), 0, st, items, bsum, boff,
+                       s_item, it_p, phase == 0 ? 0 : 1, err, fold_count);
+    hipLaunchKernelGGL(vg_bscan_cubes_kernel, dim3((unsigned)nblk), dim3(256), 0, st, items, blk_item, nh,
+                       boff, s_item, k_shift, nh_sum, run_cube, fold_list, fold_count, fold_cap);
     return hipGetLastError();
 }

@@ -182,6 +182,37 @@ int cvg_state_get_snapshot(cvg_state* st, int it, double* x_edges, int* nh);
 int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double* grad_var,
                        double* value, double* value_var);
 
+/* Batches (no reference counterpart): many independent integrals, each with
+ * its own dimension, budget, strata, map and parameter block, advanced in
+ * lockstep with one sampling launch per iteration.  The integrand has the form
+ *   vg_real integrand(vg_real* x, const vg_real* theta, int d)
+ * with theta pointing at the item's parameters.  Item i draws from Philox
+ * stream i: it reproduces a single-integral run with stream_id = i. */
+typedef struct cvg_batch cvg_batch;
+typedef struct cvg_batch_item {
+    int n_dim;            /* 1..max_dim */
+    int n_strat;          /* 0: int((n_eval_it / 4)^(1/n_dim)) */
+    long long n_eval_it;  /* samples per iteration */
+    int theta_offset;     /* first parameter of this item */
+} cvg_batch_item;
+/* shape supplies n_intervals (<= 1024), precision and run_len; options may
+ * hold block=N and hipRTC options. */
+int cvg_batch_program_create(const char* integrand_src, int max_dim, const cvg_params* shape,
+                             const char* options, cvg_program** out);
+/* common supplies n_intervals, precision, run_len, max_it, skip, alpha, beta, seed. */
+int cvg_batch_create(cvg_program* prog, const cvg_params* common, const cvg_batch_item* items,
+                     int n_items, const double* theta, int n_theta, cvg_batch** out);
+void cvg_batch_destroy(cvg_batch* b);
+int cvg_batch_run(cvg_batch* b, int n_iter, int use_graph);
+/* Per item: weighted mean, error, chi2/dof, evaluations; loop_ms: device time. */
+int cvg_batch_results(cvg_batch* b, double* result, double* error, double* chi2_dof,
+                      long long* nevals, double* loop_ms);
+int cvg_batch_history(cvg_batch* b, int item, double* I, double* sigma2, long long* nevals, int cap);
+int cvg_batch_item_info(cvg_batch* b, int item, int* n_dim, int* n_strat, long long* n_cubes);
+int cvg_batch_get_map(cvg_batch* b, int item, double* x_edges);
+int cvg_batch_get_nh(cvg_batch* b, int item, int* nh);
+int cvg_batch_get_cube_sums(cvg_batch* b, int item, double* jf, double* jf2);
+
 /* One-shot call with the reference's argument list (cuda_wrapper.cu:290).
  * integrand_src replaces the numba PTX; options replaces the kernel PTX path
  * and may hold "fp32" (CVG_FP32C) and extra hipRTC options.  batch_size,

@@ -267,7 +267,10 @@ static void window_finish(cube_acc* A, long long w) {
     }
 }
 
-/* Level 2 for every cube (done by the results kernel on the GPU). */
+/* Level 2 for every cube spanning windows wa..wb (m = wb - wa + 1 > 1 window
+ * partials: win_last[wa], win_first[wa+1..wb]): a left fold when m <= 64
+ * (done inside the GPU results kernel), else the 1024-wide tree of
+ * orc_tree_sum (a separate GPU kernel, one block per such cube). */
 static void cube_fold(cube_acc* A) {
     const long long WS = ORC_WAVE * A->K;
     for (long long h = 0; h < A->C; ++h) {
@@ -275,9 +278,20 @@ static void cube_fold(cube_acc* A) {
         if (en <= st) continue;
         long long wa = st / WS, wb = (en - 1) / WS;
         if (wa == wb) continue;
-        double t1 = A->wl1[wa], t2 = A->wl2[wa];
-        for (long long w = wa + 1; w <= wb; ++w) { t1 = t1 + A->wf1[w]; t2 = t2 + A->wf2[w]; }
-        A->JF[h] = t1; A->JF2[h] = t2;
+        const long long m = wb - wa + 1;
+        if (m <= 64) {
+            double t1 = A->wl1[wa], t2 = A->wl2[wa];
+            for (long long w = wa + 1; w <= wb; ++w) { t1 = t1 + A->wf1[w]; t2 = t2 + A->wf2[w]; }
+            A->JF[h] = t1; A->JF2[h] = t2;
+        } else {
+            double* v1 = (double*)malloc(sizeof(double) * (size_t)m);
+            double* v2 = (double*)malloc(sizeof(double) * (size_t)m);
+            v1[0] = A->wl1[wa]; v2[0] = A->wl2[wa];
+            for (long long k = 1; k < m; ++k) { v1[k] = A->wf1[wa + k]; v2[k] = A->wf2[wa + k]; }
+            A->JF[h] = orc_tree_sum(v1, m);
+            A->JF2[h] = orc_tree_sum(v2, m);
+            free(v1); free(v2);
+        }
     }
 }
 
