@@ -80,18 +80,25 @@ def run_form(precision, itg, n_eval_it, warmup, steps, n_strat, nbins, seed, use
 
 
 def cpu_baseline(itg, nbins, n_strat, n_eval_it, seed):
-    """The CPU oracle (FP64, 1 thread) on a bounded sample: the first iteration
-    of the same workload."""
+    """The CPU oracle (FP64 restatement of the reference algorithm) on a bounded
+    sample: the first iteration of the same workload, on every host core this
+    process may use (OpenMP over sample windows), and on one core."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    p, keep = orc.make_params(itg.n_dim, nbins, n_strat, n_eval_it, 1, orc.F_DGAUSS,
-                              list(itg.params), skip=0, seed=seed)
-    t0 = time.perf_counter()
-    r = orc.run(p, keep, n_strat ** itg.n_dim, snapshots=False)
-    dt = time.perf_counter() - t0
-    return {"value": r["tot_nevals"] / dt, "unit": "evals/s", "cores": 1, "kind": "port",
-            "sample": f"first iteration of the same workload ({r['tot_nevals']} evals, FP64, "
-                      f"{dt:.1f} s, 1 thread of {os.cpu_count()} host CPUs)"}
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    out = {}
+    for threads in (cores, 1):
+        p, keep = orc.make_params(itg.n_dim, nbins, n_strat, n_eval_it, 1, orc.F_DGAUSS,
+                                  list(itg.params), skip=0, seed=seed, threads=threads)
+        t0 = time.perf_counter()
+        r = orc.run(p, keep, n_strat ** itg.n_dim, snapshots=False)
+        dt = time.perf_counter() - t0
+        out[threads] = (r["tot_nevals"] / dt, dt, r["tot_nevals"])
+    v, dt, ne = out[cores]
+    return {"value": v, "unit": "evals/s", "cores": cores, "kind": "port",
+            "sample": f"first iteration of the same workload ({ne} evals, FP64), {dt:.1f} s on "
+                      f"{cores} threads; 1 thread: {out[1][0]:.3e} evals/s ({out[1][1]:.1f} s)",
+            "value_1core": out[1][0]}
 
 
 def batch_bench(args):
@@ -153,6 +160,40 @@ def batch_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def copy_probe():
+    """Three device-to-device copies of 1 GiB: the calibration kernel for the
+    FETCH_SIZE / WRITE_SIZE counters (bytes known exactly)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = 1 << 30
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(n)) != 0:
+            sys.exit("hipMalloc failed")
+    if hip.hipMemset(a, 1, C.c_size_t(n)) != 0:
+        sys.exit("hipMemset failed")
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, C.c_size_t(n), 3) != 0:  # device to device
+            sys.exit("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+
+
+def _traffic_from_profile():
+    """HBM bytes per vg_fill launch (FP32 form) from the committed PMC summary."""
+    path = os.path.join(ROOT, "profiles", "vg_fill_dg8_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,8 +202,9 @@ def main():
     ap.add_argument("--neval", type=float, default=1e8, help="evaluations per iteration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
-    ap.add_argument("--workload", choices=["dg8", "batch512"], default="dg8",
-                    help="dg8: the 8-D double Gaussian headline; batch512: 512-integral batch")
+    ap.add_argument("--workload", choices=["dg8", "batch512", "copy"], default="dg8",
+                    help="dg8: the 8-D double Gaussian headline; batch512: 512-integral batch; "
+                         "copy: 1 GiB device copies for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -172,6 +214,9 @@ def main():
         sys.exit("steps must be >= 1 and warmup >= 0")
     if args.workload == "batch512":
         batch_bench(args)
+        return
+    if args.workload == "copy":
+        copy_probe()
         return
 
     from cuvegas_amd import integrands as I
@@ -203,7 +248,7 @@ def main():
     achieved = S * head["evals_per_fill_avg"] / (head["fill_ms_avg"] * 1e-3)
     roof = {"bound": "valu", "kernel": "vg_fill",
             "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tslot/s",
-            "frac": achieved / peak, "slots_per_eval": S, "traffic": None,
+            "frac": achieved / peak, "slots_per_eval": S, "traffic": _traffic_from_profile(),
             "fill_ms_avg": head["fill_ms_avg"], "evals_per_fill": head["evals_per_fill_avg"]}
 
     cpu = None

@@ -38,7 +38,7 @@ class OrcParams(C.Structure):
         ("skip", C.c_int), ("K", C.c_int), ("n_eval_it", C.c_longlong), ("alpha", C.c_double),
         ("beta", C.c_double), ("seed", C.c_uint32), ("stream", C.c_uint32),
         ("precision", C.c_int), ("integrand", C.c_int), ("iparams", _PD), ("n_iparams", C.c_int),
-        ("domain", _PD), ("domain_vol", C.c_double), ("iters", C.c_int),
+        ("domain", _PD), ("domain_vol", C.c_double), ("iters", C.c_int), ("threads", C.c_int),
     ]
 
 
@@ -147,7 +147,7 @@ def refine(alpha, weights, counts, x_edges):
 
 
 def make_params(d, nbins, nstrat, n_eval_it, iters, integrand, iparams=(), *, skip=5, K=16,
-                alpha=0.5, beta=0.75, seed=0, stream=0, precision=0, domain=None):
+                alpha=0.5, beta=0.75, seed=0, stream=0, precision=0, domain=None, threads=1):
     p = OrcParams()
     keep = {}
     p.d, p.nbins, p.nstrat, p.max_it, p.skip, p.K = d, nbins, nstrat, iters, skip, K
@@ -162,6 +162,7 @@ def make_params(d, nbins, nstrat, n_eval_it, iters, integrand, iparams=(), *, sk
         p.domain = _d(dom)
         p.domain_vol = float(np.prod(dom[d:]))
     p.iters = iters
+    p.threads = threads
     return p, keep
 
 

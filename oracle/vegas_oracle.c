@@ -39,6 +39,7 @@
  *   orc_run           closed-form integrals (pull < 4 sigma, chi2/dof < 2)
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -299,11 +300,16 @@ static void cube_fold(cube_acc* A) {
  * and accumulates the bin histograms. */
 typedef void (*sample_fn)(void* ctx, long long s, long long h, double* jf, double* jf2);
 
-static void cube_sums_driver(cube_acc* A, sample_fn fn, void* ctx) {
-    const long long S = A->S, K = A->K, WS = ORC_WAVE * K;
-    const long long nwin = (S + WS - 1) / WS;
+/* Windows [w0, w1) of the sample stream (levels 0 and 1). */
+static void cube_sums_range(cube_acc* A, long long w0, long long w1, sample_fn fn, void* ctx) {
+    const long long S = A->S, K = A->K;
     long long h = 0;   /* cube of the current sample, advances monotonically */
-    for (long long w = 0; w < nwin; ++w) {
+    if (w0 > 0) {      /* largest h with nh_sum[h] <= first sample of the range */
+        long long s0 = w0 * ORC_WAVE * K, lo = 0, hi = A->C;
+        while (hi - lo > 1) { long long m = (lo + hi) / 2; if (A->nh_sum[m] <= s0) lo = m; else hi = m; }
+        h = lo;
+    }
+    for (long long w = w0; w < w1; ++w) {
         for (int l = 0; l < ORC_WAVE; ++l) {
             long long r = w * ORC_WAVE + l, rs = r * K;
             long long re = rs + K < S ? rs + K : S;
@@ -338,6 +344,11 @@ static void cube_sums_driver(cube_acc* A, sample_fn fn, void* ctx) {
         }
         window_finish(A, w);
     }
+}
+
+static void cube_sums_driver(cube_acc* A, sample_fn fn, void* ctx) {
+    const long long WS = ORC_WAVE * A->K;
+    cube_sums_range(A, 0, (A->S + WS - 1) / WS, fn, ctx);
     cube_fold(A);
 }
 
@@ -601,6 +612,7 @@ typedef struct {
     const double* domain;     /* NULL or 2*d: start[d], range[d] */
     double domain_vol;
     int iters;                /* iterations to run (<= max_it) */
+    int threads;              /* OpenMP threads for the sampling pass (<= 1: serial) */
 } orc_params;
 
 typedef struct {
@@ -762,7 +774,33 @@ ORC_EXPORT int orc_run(const orc_params* P, orc_out* O) {
         F.it = it;
         cube_acc A;
         if (cube_acc_init(&A, C, nh_sum, P->K, JF, JF2)) { rc = -3; break; }
-        cube_sums_driver(&A, sample_fn_impl, &F);
+        if (P->threads > 1) {
+            /* contiguous window ranges per thread; private bins merged in thread
+             * order; per-cube sums are identical to the serial pass */
+            const int T = P->threads;
+            const long long WS = ORC_WAVE * A.K, nwin = (A.S + WS - 1) / WS;
+            double** pw = (double**)calloc((size_t)T, sizeof(double*));
+            uint32_t** pc = (uint32_t**)calloc((size_t)T, sizeof(uint32_t*));
+            #pragma omp parallel num_threads(T)
+            {
+                int t = omp_get_thread_num();
+                cube_acc Al = A;
+                fill_ctx Fl = F;
+                pw[t] = (double*)calloc((size_t)(d * n), sizeof(double));
+                pc[t] = (uint32_t*)calloc((size_t)(d * n), sizeof(uint32_t));
+                Fl.weights = pw[t];
+                Fl.counts = pc[t];
+                cube_sums_range(&Al, nwin * t / T, nwin * (t + 1) / T, sample_fn_impl, &Fl);
+            }
+            for (int t = 0; t < T; ++t) {
+                for (int k = 0; k < d * n; ++k) { weights[k] += pw[t][k]; counts[k] += pc[t][k]; }
+                free(pw[t]); free(pc[t]);
+            }
+            free(pw); free(pc);
+            cube_fold(&A);
+        } else {
+            cube_sums_driver(&A, sample_fn_impl, &F);
+        }
         cube_acc_free(&A);
 
         double sums[3];
