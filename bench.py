@@ -114,8 +114,8 @@ def batch_bench(args):
     _log("autotune")
     tune = autotune_batch(FAMILY_SOURCE, items, theta, max_dim=12, n_intervals=256, precision="fp32",
                           use_cache=False, verbose=True)
-    prog = program(FAMILY_SOURCE, 12, 256, "fp32", tune["block"], tune["run_len"])
-    _log("batched run", tune["block"], tune["run_len"])
+    prog = program(FAMILY_SOURCE, 12, 256, "fp32", tune["block"], tune["run_len"], tune["task_windows"])
+    _log("batched run", tune["block"], tune["run_len"], tune["task_windows"])
     b = Batch(prog, items, theta, max_it=iters, skip=skip, seed=0)
     t0 = time.perf_counter()
     b.run(iters, use_graph=True)
@@ -143,7 +143,8 @@ def batch_bench(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seed 3 parameter draws)",
         "config": {"workload": "512 integrals, d~U{2..12}, Gaussian/oscillatory/corner, neval "
                                "log-U[1e4,1e7]/iteration, nbins=256, 10 iterations, FP32-compensated",
-                   "block": tune["block"], "run_len": tune["run_len"]},
+                   "block": tune["block"], "run_len": tune["run_len"],
+                   "task_windows": tune["task_windows"]},
         "batched": {"wall_ms": batch_wall * 1e3, "device_loop_ms": r["loop_ms"], "evals": evals,
                     "completion_p50_ms": batch_wall * 1e3, "completion_p99_ms": batch_wall * 1e3},
         "sequential": {"wall_ms": seq_wall * 1e3, "evals": seq_evals,

@@ -17,8 +17,8 @@ from typing import Dict, Sequence, Tuple
 
 from .batch import Batch, BatchProgram
 
-DEFAULT_CANDIDATES: Tuple[Tuple[int, int], ...] = (
-    (256, 16), (512, 16), (1024, 16), (512, 8), (512, 32), (1024, 32))
+DEFAULT_CANDIDATES: Tuple[Tuple[int, int, int], ...] = (
+    (256, 16, 8), (512, 16, 8), (1024, 16, 4), (512, 16, 2), (512, 32, 4), (512, 8, 8))
 
 _PROGRAMS: Dict[tuple, BatchProgram] = {}
 
@@ -37,11 +37,11 @@ def _signature(source, items, precision, n_intervals, max_dim, device) -> str:
 
 
 def program(source: str, max_dim: int, n_intervals: int, precision: str, block: int,
-            run_len: int) -> BatchProgram:
-    key = (source, max_dim, n_intervals, precision, block, run_len)
+            run_len: int, task_windows: int = 8) -> BatchProgram:
+    key = (source, max_dim, n_intervals, precision, block, run_len, task_windows)
     if key not in _PROGRAMS:
         _PROGRAMS[key] = BatchProgram(source, max_dim, n_intervals, precision, run_len=run_len,
-                                      options=f"block={block}")
+                                      options=f"block={block} task_windows={task_windows}")
     return _PROGRAMS[key]
 
 
@@ -49,7 +49,8 @@ def autotune_batch(source: str, items: Sequence, theta: Sequence[float], *, max_
                    n_intervals: int = 256, precision: str = "fp32", iters: int = 2,
                    candidates=DEFAULT_CANDIDATES, use_cache: bool = True, device: str = "",
                    verbose: bool = False) -> dict:
-    """Returns {"block", "run_len", "ms_per_iter", "table"} for the fastest candidate."""
+    """Returns {"block", "run_len", "task_windows", "ms_per_iter", "table"} for the
+    fastest candidate (block size, samples per lane run, windows per wave per task)."""
     sig = _signature(source, items, precision, n_intervals, max_dim, device)
     path = _cache_path()
     cache = {}
@@ -62,22 +63,23 @@ def autotune_batch(source: str, items: Sequence, theta: Sequence[float], *, max_
         if sig in cache:
             return dict(cache[sig], cached=True)
     table = []
-    for block, run_len in candidates:
-        prog = program(source, max_dim, n_intervals, precision, block, run_len)
+    for block, run_len, task_windows in candidates:
+        prog = program(source, max_dim, n_intervals, precision, block, run_len, task_windows)
         b = Batch(prog, items, theta, max_it=iters, skip=0, seed=0)
         t0 = time.perf_counter()
         b.run(iters, use_graph=True)
         r = b.results()
         wall = time.perf_counter() - t0
         b.close()
-        row = {"block": block, "run_len": run_len, "ms_per_iter": r["loop_ms"] / iters,
+        row = {"block": block, "run_len": run_len, "task_windows": task_windows,
+               "ms_per_iter": r["loop_ms"] / iters,
                "evals_per_s": float(r["nevals"].sum()) / (r["loop_ms"] * 1e-3), "wall_s": wall}
         table.append(row)
         if verbose:
             print("[autotune]", row, flush=True)
     best = min(table, key=lambda r: r["ms_per_iter"])
-    out = {"block": best["block"], "run_len": best["run_len"], "ms_per_iter": best["ms_per_iter"],
-           "table": table}
+    out = {"block": best["block"], "run_len": best["run_len"], "task_windows": best["task_windows"],
+           "ms_per_iter": best["ms_per_iter"], "table": table}
     if use_cache:
         try:
             os.makedirs(os.path.dirname(path), exist_ok=True)

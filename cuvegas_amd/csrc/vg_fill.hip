@@ -493,6 +493,11 @@ __device__ inline void vg_beval(double& jf, unsigned* ids, long long s, int h, i
     jf = (double)(f * jac);
 }
 
+// Windows per wave per task: a task covers VG_TASKW * (VG_BLOCK / 64) windows.
+#ifndef VG_TASKW
+#define VG_TASKW 8
+#endif
+
 extern "C" __global__ void __launch_bounds__(VG_BLOCK)
 vg_bfill(const vg_item* __restrict__ items, const int* __restrict__ task_item,
          const int* __restrict__ task_w0, int ntasks, const vg_map_t* __restrict__ gmap,
@@ -522,7 +527,8 @@ vg_bfill(const vg_item* __restrict__ items, const int* __restrict__ task_item,
         double* jf2 = JF2 + I.cube_off;
         for (int i = threadIdx.x; i < d * NB; i += VG_BLOCK) { lw[i] = 0.0; lc[i] = 0u; }
         __syncthreads();
-        const long long w = w0 + wid;
+        for (int tw = 0; tw < VG_TASKW; ++tw) {
+        const long long w = w0 + wid + (long long)tw * WPB;
         if (w < nwin) {
             const long long r = w * 64 + lane;
             const long long rs = r * K;
@@ -596,6 +602,7 @@ vg_bfill(const vg_item* __restrict__ items, const int* __restrict__ task_item,
                 }
             }
         }
+        }  // windows of this task
         __syncthreads();
         for (int i = threadIdx.x; i < d * NB; i += VG_BLOCK) {
             const unsigned c = lc[i];

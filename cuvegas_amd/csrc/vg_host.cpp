@@ -88,7 +88,7 @@ struct cvg_program {
     int d = 0, nbins = 0, nstrat = 0, precision = 0, K = kDefaultRun, has_domain = 0;
     int block = kFillBlock, lds_bins = 0, lds_map = 0;
     int has_grad = 0, n_theta = 0, bin32 = 0;
-    int batch = 0, max_dim = 0;
+    int batch = 0, max_dim = 0, task_w = 8;
     std::string log;
     std::vector<char> code;
     double compile_ms = 0.0;
@@ -149,6 +149,8 @@ struct cvg_state {
     int nh0 = 2;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    hipStream_t side = nullptr;            // map refinement runs here, beside the results chain
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int device = -1;
     // device buffers
     double* x_edges = nullptr;
@@ -210,6 +212,9 @@ struct cvg_state {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
         void* bufs[] = {x_edges, map64, map32, weights, counts, refine_scratch, nh, nh_sum, dh,
                         JF, JF2, run_cube, win_first, win_last, part, bsum, boff, s_total,
                         it_p, dsum, err, hist, domain, theta, fold_list, fold_count, bw_fold_list,
@@ -275,9 +280,15 @@ struct cvg_state {
                                       x_edges, nh, snap_x, snap_nh));
         if ((rc = launch_fill())) return rc;
         if (e_fill) VG_HIP(hipEventRecord(e_fill, stream));
-        if ((rc = launch_refine())) return rc;
+        // refinement (reads the bins) beside results -> reallocation (reads the cube sums)
+        VG_HIP(hipEventRecord(ev_fork, stream));
+        VG_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+        VG_HIP(vg_launch_refine(side, p.n_dim, p.n_intervals, p.alpha, x_edges, weights, counts,
+                                refine_scratch, map64, map32));
+        VG_HIP(hipEventRecord(ev_join, side));
         if ((rc = launch_results())) return rc;
         if ((rc = launch_realloc(1, 1))) return rc;
+        VG_HIP(hipStreamWaitEvent(stream, ev_join, 0));
         if (e_end) VG_HIP(hipEventRecord(e_end, stream));
         return CVG_OK;
     }
@@ -525,6 +536,9 @@ int cvg_state_create(cvg_program* prog, const cvg_params* params, cvg_state** ou
 
     VG_HIP(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
     st->own_stream = true;
+    VG_HIP(hipStreamCreateWithFlags(&st->side, hipStreamNonBlocking));
+    VG_HIP(hipEventCreateWithFlags(&st->ev_fork, hipEventDisableTiming));
+    VG_HIP(hipEventCreateWithFlags(&st->ev_join, hipEventDisableTiming));
     VG_HIP(hipEventCreate(&st->ev0));
     VG_HIP(hipEventCreate(&st->ev1));
     hipStream_t s = st->stream;
@@ -1084,6 +1098,8 @@ struct cvg_batch {
     std::vector<int> dims;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     vg_fitem* fitems = nullptr;
     vg_bitem* bitems = nullptr;
     int *task_item = nullptr, *task_w0 = nullptr, *blk_item = nullptr, *rd_item = nullptr, *rd_dim = nullptr;
@@ -1116,6 +1132,9 @@ struct cvg_batch {
         if (graph) (void)hipGraphDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
         void* bufs[] = {fitems, bitems, task_item, task_w0, blk_item, rd_item, rd_dim, nh, nh_sum, JF, JF2,
                         dh, run_cube, win_first, win_last, part, dsum, bsum, boff, s_item, it_p, err,
                         hist, x_all, map64, map32, weights, counts, theta, fold_list, fold_count};
@@ -1124,12 +1143,25 @@ struct cvg_batch {
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 
+    // phase 0: initial map and allocation; 1: refinement on the side stream
+    // beside results -> reallocation on the main stream.
     int steps(int phase) {
+        if (phase == 0) {
+            VG_HIP(vg_launch_batch_refine(stream, bitems, n_refine, rd_item, rd_dim, p.n_intervals, p.alpha, 1,
+                                          x_all, weights, counts, map64, map32));
+        } else {
+            VG_HIP(hipEventRecord(ev_fork, stream));
+            VG_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+            VG_HIP(vg_launch_batch_refine(side, bitems, n_refine, rd_item, rd_dim, p.n_intervals, p.alpha, 0,
+                                          x_all, weights, counts, map64, map32));
+            VG_HIP(hipEventRecord(ev_join, side));
+        }
         VG_HIP(vg_launch_batch_iteration_steps(
             stream, bitems, n_items, tot_blk, blk_item, n_refine, rd_item, rd_dim, p.n_intervals, p.alpha,
             p.beta, k_shift, p.max_it, nh, nh_sum, JF, JF2, win_first, win_last, dh, part, dsum, bsum, boff,
             s_item, run_cube, it_p, hist, x_all, weights, counts, map64, map32, err, phase, fold_list,
             fold_count, fold_cap));
+        if (phase != 0) VG_HIP(hipStreamWaitEvent(stream, ev_join, 0));
         return CVG_OK;
     }
 
@@ -1193,6 +1225,9 @@ int cvg_batch_program_create(const char* integrand_src, int max_dim, const cvg_p
             if (key == "block") {
                 if (val < 64 || val > 1024 || val % 64) return fail(CVG_ERR_INVALID, "block must be 64..1024, multiple of 64");
                 block = val;
+            } else if (key == "task_windows") {
+                if (val < 1 || val > 64) return fail(CVG_ERR_INVALID, "task_windows must be in 1..64");
+                prog->task_w = val;
             } else {
                 rtc_extra += tok + " ";
             }
@@ -1211,7 +1246,8 @@ int cvg_batch_program_create(const char* integrand_src, int max_dim, const cvg_p
         "-DVG_BATCH=1", "-DVG_MAXD=" + std::to_string(max_dim),
         "-DVG_NDIM=" + std::to_string(max_dim), "-DVG_NBINS=" + std::to_string(prog->nbins),
         "-DVG_NSTRAT=1u", "-DVG_K=" + std::to_string(K), "-DVG_BLOCK=" + std::to_string(block),
-        "-DVG_FP32=" + std::to_string(prog->precision == CVG_FP32C ? 1 : 0)};
+        "-DVG_FP32=" + std::to_string(prog->precision == CVG_FP32C ? 1 : 0),
+        "-DVG_TASKW=" + std::to_string(prog->task_w)};
     {
         std::istringstream ss(rtc_extra);
         std::string tok;
@@ -1320,9 +1356,10 @@ int cvg_batch_create(cvg_program* prog, const cvg_params* common, const cvg_batc
         f.theta_off = it.theta_offset;
         f.ns = (unsigned)ns;
         udiv_magic((unsigned)ns, &f.ns_m, &f.ns_sh1, &f.ns_sh2);
-        for (long long t = 0; t * WPB < wcap; ++t) {
+        const long long task_windows = (long long)WPB * prog->task_w;
+        for (long long t = 0; t * task_windows < wcap; ++t) {
             task_item.push_back(i);
-            task_w0.push_back((int)(t * WPB));
+            task_w0.push_back((int)(t * task_windows));
         }
         for (long long k = 0; k < nblk; ++k) blk_item.push_back(i);
         for (int dd = 0; dd < it.n_dim; ++dd) { rd_item.push_back(i); rd_dim.push_back(dd); }
@@ -1339,6 +1376,9 @@ int cvg_batch_create(cvg_program* prog, const cvg_params* common, const cvg_batc
     b->n_refine = (int)rd_item.size();
     VG_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     b->own_stream = true;
+    VG_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+    VG_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    VG_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
     VG_HIP(hipEventCreate(&b->ev0));
     VG_HIP(hipEventCreate(&b->ev1));
     hipStream_t s = b->stream;

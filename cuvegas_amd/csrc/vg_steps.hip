@@ -915,15 +915,10 @@ hipError_t vg_launch_batch_iteration_steps(hipStream_t st, const vg_bitem* items
                                            double* weights, unsigned* counts, double2* map64,
                                            float2* map32, int* err, int phase,
                                            long long* fold_list, int* fold_count, int fold_cap) {
-    const size_t lds = sizeof(double) * (size_t)(4LL * n + 1 + n / VG_TREE + 2);
-    if (phase == 0) {  // initial map and allocation
-        hipLaunchKernelGGL(vg_brefine_kernel, dim3(n_refine), dim3(256), lds, st, items, rd_item, rd_dim,
-                           n, alpha, 1, x_all, weights, counts, map64, map32);
+    if (phase == 0) {  // initial allocation (the map is initialised by vg_launch_batch_refine)
         hipLaunchKernelGGL(vg_balloc_kernel, dim3((unsigned)nblk), dim3(256), 0, st, items, blk_item, 0,
                            dh, dsum, nh, bsum);
     } else {
-        hipLaunchKernelGGL(vg_brefine_kernel, dim3(n_refine), dim3(VG_TREE), lds, st, items, rd_item,
-                           rd_dim, n, alpha, 0, x_all, weights, counts, map64, map32);
         hipLaunchKernelGGL(vg_bbigfold_kernel, dim3(128), dim3(256), 0, st, items, fold_list, fold_count,
                            fold_cap, nh_sum, win_first, win_last, 6 + k_shift, JF, JF2);
         hipLaunchKernelGGL(vg_bresults_kernel, dim3((unsigned)nblk), dim3(256), 0, st, items, blk_item,
@@ -939,5 +934,14 @@ hipError_t vg_launch_batch_iteration_steps(hipStream_t st, const vg_bitem* items
                        s_item, it_p, phase == 0 ? 0 : 1, err, fold_count);
     hipLaunchKernelGGL(vg_bscan_cubes_kernel, dim3((unsigned)nblk), dim3(256), 0, st, items, blk_item, nh,
                        boff, s_item, k_shift, nh_sum, run_cube, fold_list, fold_count, fold_cap);
+    return hipGetLastError();
+}
+
+hipError_t vg_launch_batch_refine(hipStream_t st, const vg_bitem* items, int n_refine, const int* rd_item,
+                                  const int* rd_dim, int n, double alpha, int init, double* x_all,
+                                  double* weights, unsigned* counts, double2* map64, float2* map32) {
+    const size_t lds = init ? 0 : sizeof(double) * (size_t)(4LL * n + 1 + n / VG_TREE + 2);
+    hipLaunchKernelGGL(vg_brefine_kernel, dim3(n_refine), dim3(init ? 256 : VG_TREE), lds, st, items,
+                       rd_item, rd_dim, n, alpha, init, x_all, weights, counts, map64, map32);
     return hipGetLastError();
 }
