@@ -122,10 +122,10 @@ class Batch:
 FAMILY_SOURCE = """
 __device__ vg_real integrand(vg_real* x, const vg_real* th, int d) {
     const int fam = (int)th[0];
-    if (fam == 0) {          // normalised Gaussian peak: sigma, norm, centre[d]
+    if (fam == 0) {          // normalised Gaussian peak: sigma, norm, m = -1/(2 sigma^2), centre[d]
         vg_real q = (vg_real)0;
-        for (int i = 0; i < d; ++i) { const vg_real a = x[i] - th[3 + i]; q = q + a * a; }
-        return th[2] * exp(-q / ((vg_real)2 * th[1] * th[1]));
+        for (int i = 0; i < d; ++i) { const vg_real a = x[i] - th[4 + i]; q = fma(a, a, q); }
+        return th[2] * vg_expr(q * th[3]);
     } else if (fam == 1) {   // oscillatory: u, a[d]
         vg_real t = (vg_real)2 * (vg_real)3.14159265358979323846 * th[1];
         for (int i = 0; i < d; ++i) t = t + th[2 + i] * x[i];
@@ -159,7 +159,7 @@ def serving_workload(n: int = 512, seed: int = 3):
             sigma = float(rng.uniform(0.02, 0.2))
             centre = rng.uniform(0.3, 0.7, d).tolist()
             norm = (1.0 / (sigma * math.sqrt(2.0 * math.pi))) ** d
-            theta += [0.0, sigma, norm, *centre]
+            theta += [0.0, sigma, norm, -1.0 / (2.0 * sigma * sigma), *centre]
             exact.append(math.prod(I._erf_mass(c, sigma) for c in centre))
         elif fam == "oscillatory":
             u = float(rng.uniform(0, 1))

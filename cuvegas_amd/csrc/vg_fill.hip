@@ -27,6 +27,22 @@ typedef float vg_real;
 typedef double vg_real;
 #endif
 
+// exp in the working precision, for integrands.  FP64: the library exp.
+// FP32: e^x = 2^(x log2 e) with the product carried as hi + lo (the fma gives
+// the rounding error of hi exactly, plus x times the tail of log2 e), then
+// 2^(hi + lo) = 2^hi (1 + lo ln 2) to second order in lo; v_exp_f32 gives 2^hi
+// to 1 ulp, so the result is within about 2 ulp.  The argument is clamped to
+// [-150, 88.7]: below, e^x underflows to 0; 88.72 is the overflow threshold.
+__device__ inline double vg_expr(double x) { return exp(x); }
+__device__ inline float vg_expr(float x) {
+    x = __builtin_amdgcn_fmed3f(x, -150.0f, 88.7f);
+    const float l2e_hi = 0x1.715476p+0f, l2e_lo = 0x1.4ae0c0p-26f;
+    const float hi = x * l2e_hi;
+    const float lo = __builtin_fmaf(x, l2e_lo, __builtin_fmaf(x, l2e_hi, -hi));
+    const float e = __builtin_amdgcn_exp2f(hi);
+    return __builtin_fmaf(e, lo * 0x1.62e430p-1f, e);
+}
+
 #include "vg_user.h"
 
 #ifndef VG_BATCH
@@ -95,12 +111,25 @@ __device__ inline void vg_eval(vg_sample& o, long long s, int h, unsigned it, un
         const vg_binpos b = vg_bin_of(words[i], k * (unsigned)NB, (unsigned)NB, NS);
 #if VG_FP32
         constexpr float inv_ns = 1.0f / (float)NS;
+#if VG_NSTRAT <= 256
+        // rem * 2^24 + (low >> 9) < 2^32: one conversion rounds the same real
+        // number as (float)rem + (float)(low >> 9) * 2^-24 does, times 2^24.
+        const float frac = (float)((b.rem << 24) | (unsigned)(b.low >> 9)) * (0x1p-24f * inv_ns);
+#else
         const float frac = ((float)b.rem + (float)(unsigned)(b.low >> 9) * 0x1p-24f) * inv_ns;
+#endif
         const float2 m = map[i * NB + b.id];
         o.x[i] = fmaf(m.y, frac, m.x);
 #else
         constexpr double inv_ns = 1.0 / (double)NS;
+#if VG_NSTRAT <= 256
+        // rem * 2^33 + low < 2^41 is exact in FP64 (as is rem + low * 2^-33),
+        // so scaling it by 2^-33 / n_strat rounds exactly as the sum does.
+        const unsigned hi = (b.rem << 1) | (unsigned)(b.low >> 32);
+        const double frac = fma((double)hi, 0x1p32, (double)(unsigned)b.low) * (0x1p-33 * inv_ns);
+#else
         const double frac = ((double)b.rem + (double)b.low * 0x1p-33) * inv_ns;
+#endif
         const double2 m = map[i * NB + b.id];
         o.x[i] = fma(m.y, frac, m.x);
 #endif

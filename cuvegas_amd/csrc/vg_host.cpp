@@ -149,8 +149,6 @@ struct cvg_state {
     int nh0 = 2;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    hipStream_t side = nullptr;            // map refinement runs here, beside the results chain
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int device = -1;
     // device buffers
     double* x_edges = nullptr;
@@ -212,9 +210,6 @@ struct cvg_state {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
-        if (ev_fork) (void)hipEventDestroy(ev_fork);
-        if (ev_join) (void)hipEventDestroy(ev_join);
-        if (side) (void)hipStreamDestroy(side);
         void* bufs[] = {x_edges, map64, map32, weights, counts, refine_scratch, nh, nh_sum, dh,
                         JF, JF2, run_cube, win_first, win_last, part, bsum, boff, s_total,
                         it_p, dsum, err, hist, domain, theta, fold_list, fold_count, bw_fold_list,
@@ -280,15 +275,9 @@ struct cvg_state {
                                       x_edges, nh, snap_x, snap_nh));
         if ((rc = launch_fill())) return rc;
         if (e_fill) VG_HIP(hipEventRecord(e_fill, stream));
-        // refinement (reads the bins) beside results -> reallocation (reads the cube sums)
-        VG_HIP(hipEventRecord(ev_fork, stream));
-        VG_HIP(hipStreamWaitEvent(side, ev_fork, 0));
-        VG_HIP(vg_launch_refine(side, p.n_dim, p.n_intervals, p.alpha, x_edges, weights, counts,
-                                refine_scratch, map64, map32));
-        VG_HIP(hipEventRecord(ev_join, side));
+        if ((rc = launch_refine())) return rc;
         if ((rc = launch_results())) return rc;
         if ((rc = launch_realloc(1, 1))) return rc;
-        VG_HIP(hipStreamWaitEvent(stream, ev_join, 0));
         if (e_end) VG_HIP(hipEventRecord(e_end, stream));
         return CVG_OK;
     }
@@ -536,9 +525,6 @@ int cvg_state_create(cvg_program* prog, const cvg_params* params, cvg_state** ou
 
     VG_HIP(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
     st->own_stream = true;
-    VG_HIP(hipStreamCreateWithFlags(&st->side, hipStreamNonBlocking));
-    VG_HIP(hipEventCreateWithFlags(&st->ev_fork, hipEventDisableTiming));
-    VG_HIP(hipEventCreateWithFlags(&st->ev_join, hipEventDisableTiming));
     VG_HIP(hipEventCreate(&st->ev0));
     VG_HIP(hipEventCreate(&st->ev1));
     hipStream_t s = st->stream;
@@ -1098,8 +1084,6 @@ struct cvg_batch {
     std::vector<int> dims;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     vg_fitem* fitems = nullptr;
     vg_bitem* bitems = nullptr;
     int *task_item = nullptr, *task_w0 = nullptr, *blk_item = nullptr, *rd_item = nullptr, *rd_dim = nullptr;
@@ -1132,9 +1116,6 @@ struct cvg_batch {
         if (graph) (void)hipGraphDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
-        if (ev_fork) (void)hipEventDestroy(ev_fork);
-        if (ev_join) (void)hipEventDestroy(ev_join);
-        if (side) (void)hipStreamDestroy(side);
         void* bufs[] = {fitems, bitems, task_item, task_w0, blk_item, rd_item, rd_dim, nh, nh_sum, JF, JF2,
                         dh, run_cube, win_first, win_last, part, dsum, bsum, boff, s_item, it_p, err,
                         hist, x_all, map64, map32, weights, counts, theta, fold_list, fold_count};
@@ -1143,25 +1124,15 @@ struct cvg_batch {
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 
-    // phase 0: initial map and allocation; 1: refinement on the side stream
-    // beside results -> reallocation on the main stream.
+    // phase 0: initial map and allocation; 1: refinement, results, reallocation.
     int steps(int phase) {
-        if (phase == 0) {
-            VG_HIP(vg_launch_batch_refine(stream, bitems, n_refine, rd_item, rd_dim, p.n_intervals, p.alpha, 1,
-                                          x_all, weights, counts, map64, map32));
-        } else {
-            VG_HIP(hipEventRecord(ev_fork, stream));
-            VG_HIP(hipStreamWaitEvent(side, ev_fork, 0));
-            VG_HIP(vg_launch_batch_refine(side, bitems, n_refine, rd_item, rd_dim, p.n_intervals, p.alpha, 0,
-                                          x_all, weights, counts, map64, map32));
-            VG_HIP(hipEventRecord(ev_join, side));
-        }
+        VG_HIP(vg_launch_batch_refine(stream, bitems, n_refine, rd_item, rd_dim, p.n_intervals, p.alpha,
+                                      phase == 0 ? 1 : 0, x_all, weights, counts, map64, map32));
         VG_HIP(vg_launch_batch_iteration_steps(
             stream, bitems, n_items, tot_blk, blk_item, n_refine, rd_item, rd_dim, p.n_intervals, p.alpha,
             p.beta, k_shift, p.max_it, nh, nh_sum, JF, JF2, win_first, win_last, dh, part, dsum, bsum, boff,
             s_item, run_cube, it_p, hist, x_all, weights, counts, map64, map32, err, phase, fold_list,
             fold_count, fold_cap));
-        if (phase != 0) VG_HIP(hipStreamWaitEvent(stream, ev_join, 0));
         return CVG_OK;
     }
 
@@ -1376,9 +1347,6 @@ int cvg_batch_create(cvg_program* prog, const cvg_params* common, const cvg_batc
     b->n_refine = (int)rd_item.size();
     VG_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     b->own_stream = true;
-    VG_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
-    VG_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
-    VG_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
     VG_HIP(hipEventCreate(&b->ev0));
     VG_HIP(hipEventCreate(&b->ev1));
     hipStream_t s = b->stream;

@@ -50,35 +50,37 @@ def _erf_mass(mu: float, s: float) -> float:
 # --- benchmark families --------------------------------------------------------
 
 def gaussian_peak(n_dim: int, mu: float = 0.5, sigma: float = 0.1) -> Integrand:
-    """prod_i N(x_i; mu, sigma): norm * exp(-sum (x_i - mu)^2 / (2 sigma^2))."""
+    """prod_i N(x_i; mu, sigma): norm * exp(m * sum (x_i - mu)^2), m = -1 / (2 sigma^2)."""
     norm = (1.0 / (sigma * math.sqrt(2.0 * math.pi))) ** n_dim
+    m = -1.0 / (2.0 * sigma * sigma)
     src = f"""
 __device__ vg_real integrand(vg_real* x) {{
     vg_real q = (vg_real)0;
-    for (int i = 0; i < VG_NDIM; ++i) {{ const vg_real a = x[i] - {_lit(mu)}; q = q + a * a; }}
-    return {_lit(norm)} * exp(-q / ((vg_real)2 * {_lit(sigma)} * {_lit(sigma)}));
+    for (int i = 0; i < VG_NDIM; ++i) {{ const vg_real a = x[i] - {_lit(mu)}; q = fma(a, a, q); }}
+    return {_lit(norm)} * vg_expr(q * {_lit(m)});
 }}
 """
-    return Integrand("gaussian_peak", src, n_dim, _erf_mass(mu, sigma) ** n_dim, (mu, sigma, norm))
+    return Integrand("gaussian_peak", src, n_dim, _erf_mass(mu, sigma) ** n_dim, (mu, sigma, norm, m))
 
 
 def double_gaussian(n_dim: int, c1: float = 0.33, c2: float = 0.67, sigma: float = 0.05) -> Integrand:
-    """1/2 [g(x; c1) + g(x; c2)], g a normalised isotropic Gaussian of width sigma."""
+    """1/2 [g(x; c1) + g(x; c2)], g a normalised isotropic Gaussian of width sigma:
+    g = norm * exp(m * |x - c|^2), m = -1 / (2 sigma^2)."""
     norm = (1.0 / (sigma * math.sqrt(2.0 * math.pi))) ** n_dim
+    m = -1.0 / (2.0 * sigma * sigma)
     src = f"""
 __device__ vg_real integrand(vg_real* x) {{
     vg_real q1 = (vg_real)0, q2 = (vg_real)0;
     for (int i = 0; i < VG_NDIM; ++i) {{
         const vg_real a = x[i] - {_lit(c1)}, b = x[i] - {_lit(c2)};
-        q1 = q1 + a * a;
-        q2 = q2 + b * b;
+        q1 = fma(a, a, q1);
+        q2 = fma(b, b, q2);
     }}
-    const vg_real s2 = (vg_real)2 * {_lit(sigma)} * {_lit(sigma)};
-    return (vg_real)0.5 * {_lit(norm)} * (exp(-q1 / s2) + exp(-q2 / s2));
+    return (vg_real)0.5 * {_lit(norm)} * (vg_expr(q1 * {_lit(m)}) + vg_expr(q2 * {_lit(m)}));
 }}
 """
     exact = 0.5 * (_erf_mass(c1, sigma) ** n_dim + _erf_mass(c2, sigma) ** n_dim)
-    return Integrand("double_gaussian", src, n_dim, exact, (c1, c2, sigma, norm))
+    return Integrand("double_gaussian", src, n_dim, exact, (c1, c2, sigma, norm, m))
 
 
 def oscillatory(n_dim: int, u: float, a: Sequence[float]) -> Integrand:

@@ -529,19 +529,18 @@ static double f64_eval(int id, const double* x, int d, const double* p) {
     case ORC_F_ONE: return 1.0;
     case ORC_F_LINEAR: { double f = 0.0; for (int i = 0; i < d; ++i) f = f + x[i]; return f; }
     case ORC_F_POLY: { double f = 1.0; for (int i = 0; i < d; ++i) f = f * (6.0 * x[i] * (1.0 - x[i])); return f; }
-    case ORC_F_GAUSS: { /* p = {mu, sigma, norm} : norm * exp(-sum (x-mu)^2 / (2 sigma^2)) */
+    case ORC_F_GAUSS: { /* p = {mu, sigma, norm, m}: norm * exp(m * sum (x-mu)^2), m = -1/(2 sigma^2) */
         double q = 0.0;
-        for (int i = 0; i < d; ++i) { double a = x[i] - p[0]; q = q + a * a; }
-        return p[2] * exp(-q / (2.0 * p[1] * p[1]));
+        for (int i = 0; i < d; ++i) { double a = x[i] - p[0]; q = fma(a, a, q); }
+        return p[2] * exp(q * p[3]);
     }
-    case ORC_F_DGAUSS: { /* p = {c1, c2, sigma, norm} */
+    case ORC_F_DGAUSS: { /* p = {c1, c2, sigma, norm, m} */
         double q1 = 0.0, q2 = 0.0;
         for (int i = 0; i < d; ++i) {
             double a = x[i] - p[0], b = x[i] - p[1];
-            q1 = q1 + a * a; q2 = q2 + b * b;
+            q1 = fma(a, a, q1); q2 = fma(b, b, q2);
         }
-        double s2 = 2.0 * p[2] * p[2];
-        return 0.5 * p[3] * (exp(-q1 / s2) + exp(-q2 / s2));
+        return 0.5 * p[3] * (exp(q1 * p[4]) + exp(q2 * p[4]));
     }
     case ORC_F_OSC: { /* p = {u, a_0..a_{d-1}} */
         double t = 2.0 * ORC_PI * p[0];
@@ -567,17 +566,16 @@ static float f32_eval(int id, const float* x, int d, const float* p) {
     case ORC_F_POLY: { float f = 1.0f; for (int i = 0; i < d; ++i) f = f * (6.0f * x[i] * (1.0f - x[i])); return f; }
     case ORC_F_GAUSS: {
         float q = 0.0f;
-        for (int i = 0; i < d; ++i) { float a = x[i] - p[0]; q = q + a * a; }
-        return p[2] * expf(-q / (2.0f * p[1] * p[1]));
+        for (int i = 0; i < d; ++i) { float a = x[i] - p[0]; q = fmaf(a, a, q); }
+        return p[2] * expf(q * p[3]);
     }
     case ORC_F_DGAUSS: {
         float q1 = 0.0f, q2 = 0.0f;
         for (int i = 0; i < d; ++i) {
             float a = x[i] - p[0], b = x[i] - p[1];
-            q1 = q1 + a * a; q2 = q2 + b * b;
+            q1 = fmaf(a, a, q1); q2 = fmaf(b, b, q2);
         }
-        float s2 = 2.0f * p[2] * p[2];
-        return 0.5f * p[3] * (expf(-q1 / s2) + expf(-q2 / s2));
+        return 0.5f * p[3] * (expf(q1 * p[4]) + expf(q2 * p[4]));
     }
     case ORC_F_OSC: {
         float t = 2.0f * (float)ORC_PI * p[0];

@@ -80,6 +80,32 @@ def test_bin_of_matches_exact_rational_floor(orc, nbins, nstrat):
             assert 0 <= i < nbins
 
 
+def test_offset_single_conversion_rewrite_is_exact():
+    """The sampling kernel forms the in-bin offset with one integer-to-float
+    conversion when n_strat <= 256:
+      FP32  (float)(rem << 24 | low >> 9) * (2^-24 / ns)
+            == ((float)rem + (float)(low >> 9) * 2^-24) * (1 / ns)
+      FP64  fma((double)(rem << 1 | low >> 32), 2^32, (double)(low mod 2^32)) * (2^-33 / ns)
+            == ((double)rem + (double)low * 2^-33) * (1 / ns)
+    bit for bit, for every ns in 2..256, every rem < ns and any 33-bit low."""
+    rng = np.random.default_rng(5)
+    f32, f64 = np.float32, np.float64
+    edge = np.array([0, 1, 2**9 - 1, 2**9, 2**32 - 1, 2**32, 2**33 - 1, 2**33 - 2**9], dtype=np.uint64)
+    for ns in range(2, 257):
+        rem = np.concatenate([rng.integers(0, ns, 400), np.arange(ns), np.full(8, ns - 1)]).astype(np.uint64)
+        low = np.concatenate([rng.integers(0, 2**33, 400), rng.integers(0, 2**33, ns), edge]).astype(np.uint64)
+        inv32 = f32(1) / f32(ns)
+        ref32 = (rem.astype(f32) + (low >> np.uint64(9)).astype(f32) * f32(2.0**-24)) * inv32
+        one32 = ((rem << np.uint64(24)) | (low >> np.uint64(9))).astype(np.uint32).astype(f32) * (f32(2.0**-24) * inv32)
+        assert np.array_equal(ref32.view(np.uint32), one32.view(np.uint32)), ns
+        inv64 = 1.0 / ns
+        ref64 = (rem.astype(f64) + low.astype(f64) * 2.0**-33) * inv64
+        hi = ((rem << np.uint64(1)) | (low >> np.uint64(32))).astype(f64)
+        v = hi * 2.0**32 + (low & np.uint64(0xFFFFFFFF)).astype(f64)  # exact: < 2^41
+        one64 = v * (2.0**-33 * inv64)
+        assert np.array_equal(ref64.view(np.uint64), one64.view(np.uint64)), ns
+
+
 def test_tree_sum_exact_and_close(orc):
     rng = np.random.default_rng(3)
     for m in [1, 5, 1023, 1024, 1025, 4096 + 17, 1024 * 1024 + 3]:
@@ -207,7 +233,8 @@ def test_run_closed_form_gaussian(orc):
     d, n, ns, N, iters = 4, 256, 6, 20000, 10
     sigma, mu = 0.1, 0.5
     norm = (1.0 / (sigma * math.sqrt(2 * math.pi))) ** d
-    p, keep = orc.make_params(d, n, ns, N, iters, orc.F_GAUSS, (mu, sigma, norm), seed=0)
+    m = -1.0 / (2.0 * sigma * sigma)
+    p, keep = orc.make_params(d, n, ns, N, iters, orc.F_GAUSS, (mu, sigma, norm, m), seed=0)
     r = orc.run(p, keep, ns ** d)
     exact = (math.erf(0.5 / (sigma * math.sqrt(2)))) ** d
     assert abs(r["result"] - exact) < 4 * r["error"]
