@@ -18,8 +18,13 @@
 //
 // Compile-time shape (hipRTC -D options): VG_NDIM, VG_NBINS, VG_NSTRAT, VG_K,
 // VG_BLOCK, VG_FP32 (0: FP64 sampling and evaluation; 1: FP32 sampling and
-// evaluation with FP64 accumulation), VG_LDS_BINS, VG_LDS_MAP, VG_HAS_DOMAIN.
+// evaluation with FP64 accumulation), VG_LDS_BINS, VG_LDS_MAP, VG_HAS_DOMAIN,
+// VG_PAIR (two samples per lane step).
 #include "vg_philox.h"
+
+#ifndef VG_PAIR
+#define VG_PAIR 1
+#endif
 
 #if VG_FP32
 typedef float vg_real;
@@ -219,37 +224,72 @@ vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
         int hF = 0, npieces = 0;
         double f1 = 0.0, f2 = 0.0, a1 = 0.0, a2 = 0.0;
 
+        // The sample at cend opens cube h + 1: close cube h (or keep it as the
+        // run's first piece when it began before the run).
+        auto close_cube = [&]() {
+            if (npieces == 0 && sb) {
+                hasF = true; hF = h; f1 = a1; f2 = a2;
+            } else {
+                JF[h] = a1; JF2[h] = a2;
+            }
+            ++npieces;
+            ++h;
+            cstart = cend;
+            cend = nh_sum[h + 1];
+            a1 = 0.0; a2 = 0.0;
+        };
+        auto take = [&](const vg_sample& o) {
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+#if VG_LDS_BINS
+                atomicAdd(&lw[i * NB + o.ids[i]], (vg_bin_t)o.jf2);
+                atomicAdd(&lc[i * NB + o.ids[i]], 1u);
+#else
+                unsafeAtomicAdd(&weights[i * NB + o.ids[i]], o.jf2);
+                atomicAdd(&counts[i * NB + o.ids[i]], 1u);
+#endif
+            }
+            a1 = a1 + o.jf;
+            a2 = a2 + o.jf2;
+        };
+#if VG_K >= 2 && VG_PAIR
+        // Two samples per step, evaluated independently so that their map
+        // lookups and transcendental latencies overlap, then taken in sample
+        // order (the per-cube sums are unchanged).  Every cube holds at least
+        // 2 samples, so the pair crosses at most one cube boundary and both
+        // cubes are known before either evaluation.
+        for (int j = 0; j < K; j += 2) {
+            const long long s0 = rs + j;
+            if (s0 < re) {
+                const long long s1 = s0 + 1;
+                const bool v1 = s1 < re;
+                const bool c0 = s0 == cend;
+                const bool c1 = !c0 && v1 && s1 == cend;
+                const int h0 = h + (c0 ? 1 : 0);
+                const int h1 = h0 + (c1 ? 1 : 0);
+                vg_sample o0, o1;
+                vg_eval(o0, s0, h0, it, seed, stream, map, domain, domain_vol, theta);
+                // past the end of the sample range: evaluate s0 again, unused
+                vg_eval(o1, v1 ? s1 : s0, h1, it, seed, stream, map, domain, domain_vol, theta);
+                if (c0) close_cube();
+                take(o0);
+                if (v1) {
+                    if (c1) close_cube();
+                    take(o1);
+                }
+            }
+        }
+#else
         for (int j = 0; j < K; ++j) {
             const long long s = rs + j;
             if (s < re) {
-                if (s == cend) {
-                    if (npieces == 0 && sb) {
-                        hasF = true; hF = h; f1 = a1; f2 = a2;
-                    } else {
-                        JF[h] = a1; JF2[h] = a2;
-                    }
-                    ++npieces;
-                    ++h;
-                    cstart = cend;
-                    cend = nh_sum[h + 1];
-                    a1 = 0.0; a2 = 0.0;
-                }
+                if (s == cend) close_cube();
                 vg_sample o;
                 vg_eval(o, s, h, it, seed, stream, map, domain, domain_vol, theta);
-#pragma unroll
-                for (int i = 0; i < D; ++i) {
-#if VG_LDS_BINS
-                    atomicAdd(&lw[i * NB + o.ids[i]], (vg_bin_t)o.jf2);
-                    atomicAdd(&lc[i * NB + o.ids[i]], 1u);
-#else
-                    unsafeAtomicAdd(&weights[i * NB + o.ids[i]], o.jf2);
-                    atomicAdd(&counts[i * NB + o.ids[i]], 1u);
-#endif
-                }
-                a1 = a1 + o.jf;
-                a2 = a2 + o.jf2;
+                take(o);
             }
         }
+#endif
 
         // level 1: segmented scan of each run's last piece across the window
         const bool cont = cend > re;

@@ -87,7 +87,7 @@ int validate(const cvg_params* p) {
 struct cvg_program {
     int d = 0, nbins = 0, nstrat = 0, precision = 0, K = kDefaultRun, has_domain = 0;
     int block = kFillBlock, lds_bins = 0, lds_map = 0;
-    int has_grad = 0, n_theta = 0, bin32 = 0;
+    int has_grad = 0, n_theta = 0, bin32 = 0, pair = 1;
     int batch = 0, max_dim = 0, task_w = 8;
     std::string log;
     std::vector<char> code;
@@ -337,6 +337,11 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
     prog->nstrat = shape->n_strat;
     prog->precision = shape->precision;
     prog->K = shape->run_len > 0 ? shape->run_len : kDefaultRun;
+    // Measured on the 8-D double Gaussian (1e8 samples): the FP32 form runs
+    // best at 16 waves per CU, one sample per step; the FP64 form (map read
+    // from global memory) at 12 waves per CU with two samples in flight per lane.
+    prog->block = prog->precision == CVG_FP32C ? kFillBlock : 768;
+    prog->pair = prog->precision == CVG_FP32C ? 0 : 1;
     prog->has_domain = shape->has_domain ? 1 : 0;
     prog->n_theta = shape->n_theta;
     const long long dn = (long long)prog->d * prog->nbins;
@@ -346,7 +351,8 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
     prog->lds_map = prog->precision == CVG_FP32C && prog->lds_bins && bin_bytes + map_bytes <= kLdsBytes;
 
     // Launch-shape knobs in the options string (consumed here, not passed to
-    // hipRTC): lds_bins=0|1, lds_map=0|1, block=64..1024 (multiple of 64).
+    // hipRTC): lds_bins=0|1, lds_map=0|1, block=64..1024 (multiple of 64),
+    // pair=0|1 (two samples per lane step in the sampling loop).
     std::string rtc_extra;
     if (options) {
         std::istringstream ss(options);
@@ -375,6 +381,8 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
             } else if (key == "block") {
                 if (val < 64 || val > 1024 || val % 64) return fail(CVG_ERR_INVALID, "block must be 64..1024, multiple of 64");
                 prog->block = val;
+            } else if (key == "pair") {
+                prog->pair = val ? 1 : 0;
             } else {
                 rtc_extra += tok + " ";
             }
@@ -399,6 +407,7 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
         "-DVG_HAS_DOMAIN=" + std::to_string(prog->has_domain),
         "-DVG_GRAD=" + std::to_string(prog->has_grad),
         "-DVG_BIN32=" + std::to_string(prog->bin32),
+        "-DVG_PAIR=" + std::to_string(prog->pair),
         "-DVG_NTHETA=" + std::to_string(prog->n_theta)};
     if (options) {
         std::istringstream ss(options);
@@ -752,7 +761,9 @@ int cvg_state_set_nh(cvg_state* st, const int* nh) {
     if (!st || !nh) return fail(CVG_ERR_INVALID, "NULL argument");
     long long tot = 0;
     for (long long h = 0; h < st->C; ++h) {
-        if (nh[h] < 1) return fail(CVG_ERR_INVALID, "n_h must be >= 1");
+        // the allocation rule never gives fewer than 2 (cuda_wrapper.cu:130,477),
+        // and the sampling loop takes two samples per step on that guarantee
+        if (nh[h] < 2) return fail(CVG_ERR_INVALID, "n_h must be >= 2");
         tot += nh[h];
     }
     if (tot > st->s_cap) return fail(CVG_ERR_CAPACITY, "sum of n_h exceeds the state's sample buffers");

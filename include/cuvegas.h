@@ -143,7 +143,8 @@ long long cvg_state_n_cubes(const cvg_state* st);
 /* State I/O (host buffers).  Map edges: n_dim * (n_intervals + 1). */
 int cvg_state_get_map(cvg_state* st, double* x_edges);
 int cvg_state_set_map(cvg_state* st, const double* x_edges);
-/* Allocation n_h per cube (n_cubes ints). set_nh also rebuilds the sample layout. */
+/* Allocation n_h per cube (n_cubes ints). set_nh also rebuilds the sample layout;
+ * every n_h must be >= 2, as the allocation rule guarantees. */
 int cvg_state_get_nh(cvg_state* st, int* nh);
 int cvg_state_set_nh(cvg_state* st, const int* nh);
 /* d_h of the last iteration (n_cubes doubles). */
