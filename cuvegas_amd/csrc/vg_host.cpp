@@ -168,6 +168,7 @@ struct cvg_state {
     double* part = nullptr;
     long long* bsum = nullptr;
     long long* boff = nullptr;
+    unsigned* sync = nullptr;  // arrival counters of the finalize [0] and block-scan [1] launches
     long long* s_total = nullptr;
     int* it_p = nullptr;
     double* dsum = nullptr;
@@ -211,7 +212,7 @@ struct cvg_state {
         if (ev1) (void)hipEventDestroy(ev1);
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         void* bufs[] = {x_edges, map64, map32, weights, counts, refine_scratch, nh, nh_sum, dh,
-                        JF, JF2, run_cube, win_first, win_last, part, bsum, boff, s_total,
+                        JF, JF2, run_cube, win_first, win_last, part, bsum, boff, sync, s_total,
                         it_p, dsum, err, hist, domain, theta, fold_list, fold_count, bw_fold_list,
                         bw_fold_count, snap_x, snap_nh, bw_nh_sum,
                         bw_s_total, bw_x, bw_map64, bw_map32, bw_part};
@@ -253,13 +254,13 @@ struct cvg_state {
                                  6 + k_shift, JF, JF2));
         VG_HIP(vg_launch_results(stream, (int)C, V, p.beta, nh, nh_sum, JF, JF2, win_first, win_last,
                                  6 + k_shift, dh, part));
-        VG_HIP(vg_launch_finalize(stream, nb, part, s_total, it_p, p.max_it, hist, dsum, err));
+        VG_HIP(vg_launch_finalize(stream, nb, part, sync, s_total, it_p, p.max_it, hist, dsum, err));
         return CVG_OK;
     }
 
     int launch_realloc(int mode, int advance) {
         VG_HIP(vg_launch_alloc(stream, (int)C, mode, nh0, dh, dsum, p.n_eval_it, nh, bsum));
-        VG_HIP(vg_launch_scan(stream, (int)C, nh, bsum, boff, s_total, s_cap, it_p, advance, k_shift,
+        VG_HIP(vg_launch_scan(stream, (int)C, nh, bsum, boff, sync + 1, s_total, s_cap, it_p, advance, k_shift,
                               run_cap, nh_sum, run_cube, err, fold_list, fold_count, fold_cap));
         return CVG_OK;
     }
@@ -296,7 +297,7 @@ struct cvg_state {
 
 // Layout (nh_sum, run_cube) from the n_h already on the device.
 int cvg_state::launch_realloc_copy() {
-    VG_HIP(vg_launch_layout(stream, (int)C, nh, bsum, boff, s_total, s_cap, it_p, k_shift, run_cap, nh_sum,
+    VG_HIP(vg_launch_layout(stream, (int)C, nh, bsum, boff, sync + 1, s_total, s_cap, it_p, k_shift, run_cap, nh_sum,
                             run_cube, err, fold_list, fold_count, fold_cap));
     return CVG_OK;
 }
@@ -518,9 +519,12 @@ int cvg_state_create(cvg_program* prog, const cvg_params* params, cvg_state** ou
     if ((rc = dalloc(&st->run_cube, st->run_cap))) return rc;
     if ((rc = dalloc(&st->win_first, st->win_cap))) return rc;
     if ((rc = dalloc(&st->win_last, st->win_cap))) return rc;
-    if ((rc = dalloc(&st->part, 3 * st->nb))) return rc;
-    if ((rc = dalloc(&st->bsum, st->nb))) return rc;
-    if ((rc = dalloc(&st->boff, st->nb))) return rc;
+    // block partials, then one per 1024 blocks (the group level of the trees / scans)
+    const long long ng = (st->nb + 1023) / 1024;
+    if ((rc = dalloc(&st->part, 3 * (st->nb + ng)))) return rc;
+    if ((rc = dalloc(&st->bsum, st->nb + ng))) return rc;
+    if ((rc = dalloc(&st->boff, st->nb + ng))) return rc;
+    if ((rc = dalloc(&st->sync, 2))) return rc;
     if ((rc = dalloc(&st->s_total, 1))) return rc;
     if ((rc = dalloc(&st->it_p, 1))) return rc;
     if ((rc = dalloc(&st->dsum, 1))) return rc;
@@ -542,6 +546,7 @@ int cvg_state_create(cvg_program* prog, const cvg_params* params, cvg_state** ou
                               hipMemcpyHostToDevice, s));
     VG_HIP(hipMemsetAsync(st->it_p, 0, sizeof(int), s));
     VG_HIP(hipMemsetAsync(st->err, 0, sizeof(int), s));
+    VG_HIP(hipMemsetAsync(st->sync, 0, 2 * sizeof(unsigned), s));
     VG_HIP(hipMemsetAsync(st->theta, 0, 8LL * (params->n_theta > 0 ? params->n_theta : 1), s));
     VG_HIP(vg_launch_init_map(s, d, n, st->x_edges, st->map64, st->map32, st->weights, st->counts));
     if ((rc = st->launch_realloc(0, 0))) return rc;
@@ -953,7 +958,7 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         VG_HIP(hipMemcpyAsync(st->bw_x, st->snap_x + slot * nx, sizeof(double) * nx, hipMemcpyDeviceToDevice, st->stream));
         VG_HIP(vg_launch_tables(st->stream, d, n, st->bw_x, st->bw_map64, st->bw_map32));
         const int* snh = st->snap_nh + slot * st->C;
-        VG_HIP(vg_launch_layout(st->stream, C, snh, st->bsum, st->boff, st->bw_s_total, (1LL << 62), st->it_p,
+        VG_HIP(vg_launch_layout(st->stream, C, snh, st->bsum, st->boff, st->sync + 1, st->bw_s_total, (1LL << 62), st->it_p,
                                 st->k_shift, 0, st->bw_nh_sum, nullptr, st->err, st->bw_fold_list,
                                 st->bw_fold_count, 1));
         const void* map = st->p.precision == CVG_FP32C ? (const void*)st->bw_map32 : (const void*)st->bw_map64;

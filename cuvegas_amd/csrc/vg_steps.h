@@ -9,13 +9,18 @@ hipError_t vg_launch_results(hipStream_t st, int C, double V, double beta, const
                              const long long* nh_sum, double* JF, double* JF2,
                              const double2* win_first, const double2* win_last, int ws_shift,
                              double* dh, double* part);
-hipError_t vg_launch_finalize(hipStream_t st, long long nparts, double* part, const long long* s_total,
-                              const int* it_p, int hist_cap, double* hist, double* dsum, int* err);
+// part holds 3 * (nparts + ceil(nparts / 1024)) doubles; sync is one zeroed
+// counter that the launch leaves zeroed.
+hipError_t vg_launch_finalize(hipStream_t st, long long nparts, double* part, unsigned* sync,
+                              const long long* s_total, const int* it_p, int hist_cap, double* hist,
+                              double* dsum, int* err);
 hipError_t vg_launch_alloc(hipStream_t st, int C, int mode, int nh_const, const double* dh,
                            const double* dsum, long long N, int* nh, long long* bsum);
+// bsum and boff hold nb + ceil(nb / 1024) int64 (nb = ceil(C / 1024)); sync is
+// one zeroed counter that the launch leaves zeroed.
 hipError_t vg_launch_scan(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
-                          long long* s_total, long long s_cap, int* it_p, int advance, int k_shift,
-                          long long run_cap, long long* nh_sum, int* run_cube, int* err,
+                          unsigned* sync, long long* s_total, long long s_cap, int* it_p, int advance,
+                          int k_shift, long long run_cap, long long* nh_sum, int* run_cube, int* err,
                           int* fold_list, int* fold_count, int fold_cap);
 hipError_t vg_launch_bigfold(hipStream_t st, const int* list, const int* count, int cap,
                              const long long* nh_sum, const double2* win_first,
@@ -29,8 +34,8 @@ hipError_t vg_launch_tables(hipStream_t st, int d, int n, const double* x_edges,
                             float2* map32);
 long long vg_refine_scratch_doubles(int d, int n);
 hipError_t vg_launch_layout(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
-                            long long* s_total, long long s_cap, int* it_p, int k_shift, long long run_cap,
-                            long long* nh_sum, int* run_cube, int* err, int* fold_list,
+                            unsigned* sync, long long* s_total, long long s_cap, int* it_p, int k_shift,
+                            long long run_cap, long long* nh_sum, int* run_cube, int* err, int* fold_list,
                             int* fold_count, int fold_cap);
 hipError_t vg_launch_snapshot(hipStream_t st, const int* it_p, int skip, int cap, int C, int nx,
                               const double* x_edges, const int* nh, double* snap_x, int* snap_nh);

@@ -77,6 +77,27 @@ __device__ inline double vg_tree1024_256(double v0, double v1, double v2, double
     return x;
 }
 
+// True in every thread of the block that arrives last at *cnt among the
+// gridDim.x blocks of this launch, with the other blocks' global writes
+// visible to it (agent-scope release before the count, acquire after: on
+// gfx950 each fence writes back / invalidates this XCD's L2, so this is for
+// launches of a few dozen blocks).  The last block resets the counter, so
+// it is zero again for the next launch without a memset.
+__device__ inline bool vg_last_block(unsigned* cnt, int* flag) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned prev = atomicAdd(cnt, 1u);
+        const int last = prev == gridDim.x - 1;
+        if (last) atomicExch(cnt, 0u);
+        *flag = last;
+    }
+    __syncthreads();
+    const bool last = *flag != 0;
+    if (last) __threadfence();
+    return last;
+}
+
 // ---------------------------------------------------------------------------
 // Per-cube results (reference `results`, cuda_wrapper.cu:26-49).  Also folds
 // the window partials of cubes that crossed sampling windows and writes the
@@ -216,12 +237,33 @@ hipError_t vg_launch_bigfold(hipStream_t st, const int* list, const int* count, 
 
 // Remaining tree levels over the block partials, then the iteration record:
 // hist[4*it + {0,1,2,3}] = {I_k, sigma_k^2, D, samples}; *dsum = D.
+// Block g reduces chunk g (1024 block partials) into part2[3g + q]; the last
+// block to finish reduces part2 level by level, in place.  The sums are those
+// of the sequential definition (chunks of 1024, strided pairwise trees,
+// repeated over the partials).
 __global__ void __launch_bounds__(256)
-vg_finalize_kernel(long long nparts, double* __restrict__ part, const long long* __restrict__ s_total,
+vg_finalize_kernel(long long nparts, const double* __restrict__ part, double* __restrict__ part2,
+                   unsigned* __restrict__ sync, const long long* __restrict__ s_total,
                    const int* __restrict__ it_p, int hist_cap, double* __restrict__ hist,
                    double* __restrict__ dsum, int* __restrict__ err) {
     __shared__ double red[256];
-    long long cur = nparts;
+    __shared__ int last;
+    {
+        double v[4][3];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long long i = (long long)blockIdx.x * VG_TREE + threadIdx.x + 256 * j;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) v[j][q] = i < nparts ? part[3 * i + q] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double tot = vg_tree1024_256(v[0][q], v[1][q], v[2][q], v[3][q], red);
+            if (threadIdx.x == 0) part2[3 * blockIdx.x + q] = tot;
+        }
+    }
+    if (!vg_last_block(sync, &last)) return;
+    long long cur = gridDim.x;
     while (cur > 1) {
         const long long nch = (cur + VG_TREE - 1) / VG_TREE;
         for (long long c = 0; c < nch; ++c) {
@@ -230,13 +272,13 @@ vg_finalize_kernel(long long nparts, double* __restrict__ part, const long long*
             for (int j = 0; j < 4; ++j) {
                 const long long i = c * VG_TREE + threadIdx.x + 256 * j;
 #pragma unroll
-                for (int q = 0; q < 3; ++q) v[j][q] = i < cur ? part[3 * i + q] : 0.0;
+                for (int q = 0; q < 3; ++q) v[j][q] = i < cur ? part2[3 * i + q] : 0.0;
             }
             __syncthreads();  // every read of this chunk precedes the writes below
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 const double tot = vg_tree1024_256(v[0][q], v[1][q], v[2][q], v[3][q], red);
-                if (threadIdx.x == 0) part[3 * c + q] = tot;
+                if (threadIdx.x == 0) part2[3 * c + q] = tot;
             }
             __syncthreads();
         }
@@ -244,11 +286,11 @@ vg_finalize_kernel(long long nparts, double* __restrict__ part, const long long*
     }
     if (threadIdx.x == 0) {
         const int it = *it_p;
-        *dsum = part[2];
+        *dsum = part2[2];
         if (it < hist_cap) {
-            hist[4 * it + 0] = part[0];
-            hist[4 * it + 1] = part[1];
-            hist[4 * it + 2] = part[2];
+            hist[4 * it + 0] = part2[0];
+            hist[4 * it + 1] = part2[1];
+            hist[4 * it + 2] = part2[2];
             hist[4 * it + 3] = (double)*s_total;
         } else {
             atomicOr(err, VG_ERR_HIST);
@@ -293,22 +335,39 @@ vg_alloc_kernel(int C, int mode, int nh_const, const double* __restrict__ dh,
     if (threadIdx.x == 0) bsum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-// Exclusive scan of the block sums (one block); total -> *s_total, capped at
-// s_cap (error flag set if the allocation exceeds the sample buffers).
-// Optionally advances the iteration counter (end of an iteration).
+// Exclusive scan of the block sums.  Block g scans its 1024 block sums into
+// boff (offsets within group g) and writes the group total gsum[g]; the last
+// block to finish scans the group totals into goff.  The cube-level scan adds
+// boff[b] + goff[b / 1024].  Total -> *s_total, capped at s_cap (error flag
+// set if the allocation exceeds the sample buffers).  Optionally advances the
+// iteration counter (end of an iteration).
 __global__ void __launch_bounds__(VG_TREE)
 vg_scan_blocks_kernel(long long nb, const long long* __restrict__ bsum, long long* __restrict__ boff,
-                      long long* __restrict__ s_total, long long s_cap, int* __restrict__ it_p,
-                      int advance, int* __restrict__ err, int* __restrict__ fold_count) {
+                      long long* __restrict__ gsum, long long* __restrict__ goff,
+                      unsigned* __restrict__ sync, long long* __restrict__ s_total, long long s_cap,
+                      int* __restrict__ it_p, int advance, int* __restrict__ err,
+                      int* __restrict__ fold_count) {
     __shared__ long long wt[VG_TREE / 64];
-    const long long per = (nb + VG_TREE - 1) / VG_TREE;
-    const long long b0 = threadIdx.x * per;
-    long long local = 0;
-    for (long long b = b0; b < b0 + per && b < nb; ++b) local += bsum[b];
-    long long total;
-    long long acc = vg_block_scan_excl(local, wt, &total);
-    for (long long b = b0; b < b0 + per && b < nb; ++b) { boff[b] = acc; acc += bsum[b]; }
+    __shared__ int last;
+    {
+        const long long b = (long long)blockIdx.x * VG_TREE + threadIdx.x;
+        long long tot;
+        const long long ex = vg_block_scan_excl(b < nb ? bsum[b] : 0, wt, &tot);
+        if (b < nb) boff[b] = ex;
+        if (threadIdx.x == 0) gsum[blockIdx.x] = tot;
+    }
+    if (!vg_last_block(sync, &last)) return;
+    const long long ng = gridDim.x;
+    long long carry = 0;
+    for (long long g0 = 0; g0 < ng; g0 += VG_TREE) {
+        const long long g = g0 + threadIdx.x;
+        long long tot;
+        const long long ex = vg_block_scan_excl(g < ng ? gsum[g] : 0, wt, &tot);
+        if (g < ng) goff[g] = carry + ex;
+        carry += tot;
+    }
     if (threadIdx.x == 0) {
+        long long total = carry;
         if (total > s_cap) { atomicOr(err, VG_ERR_CAPACITY); total = s_cap; }
         *s_total = total;
         if (advance) *it_p = *it_p + 1;
@@ -322,7 +381,7 @@ vg_scan_blocks_kernel(long long nb, const long long* __restrict__ bsum, long lon
 // of cubes with more than 16 of them are written by the whole block.
 __global__ void __launch_bounds__(256)
 vg_scan_cubes_kernel(int C, const int* __restrict__ nh, const long long* __restrict__ boff,
-                     const long long* __restrict__ s_total, int k_shift, long long run_cap,
+                     const long long* __restrict__ goff, const long long* __restrict__ s_total, int k_shift, long long run_cap,
                      long long* __restrict__ nh_sum, int* __restrict__ run_cube,
                      int* __restrict__ fold_list, int* __restrict__ fold_count, int fold_cap) {
     __shared__ long long wt[4];
@@ -341,7 +400,7 @@ vg_scan_cubes_kernel(int C, const int* __restrict__ nh, const long long* __restr
     if (threadIdx.x == 0) nbig = 0;
     long long total;
     const long long S = *s_total;
-    long long ex = vg_block_scan_excl(own, wt, &total) + boff[blockIdx.x];
+    long long ex = vg_block_scan_excl(own, wt, &total) + boff[blockIdx.x] + goff[blockIdx.x / VG_TREE];
     const long long kmask = (1LL << k_shift) - 1;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -535,10 +594,12 @@ hipError_t vg_launch_results(hipStream_t st, int C, double V, double beta, const
     return hipGetLastError();
 }
 
-hipError_t vg_launch_finalize(hipStream_t st, long long nparts, double* part, const long long* s_total,
-                              const int* it_p, int hist_cap, double* hist, double* dsum, int* err) {
-    hipLaunchKernelGGL(vg_finalize_kernel, dim3(1), dim3(256), 0, st, nparts, part, s_total, it_p,
-                       hist_cap, hist, dsum, err);
+hipError_t vg_launch_finalize(hipStream_t st, long long nparts, double* part, unsigned* sync,
+                              const long long* s_total, const int* it_p, int hist_cap, double* hist,
+                              double* dsum, int* err) {
+    const long long ng = (nparts + VG_TREE - 1) / VG_TREE;
+    hipLaunchKernelGGL(vg_finalize_kernel, dim3((unsigned)ng), dim3(256), 0, st, nparts, part,
+                       part + 3 * nparts, sync, s_total, it_p, hist_cap, hist, dsum, err);
     return hipGetLastError();
 }
 
@@ -551,15 +612,16 @@ hipError_t vg_launch_alloc(hipStream_t st, int C, int mode, int nh_const, const 
 }
 
 hipError_t vg_launch_scan(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
-                          long long* s_total, long long s_cap, int* it_p, int advance, int k_shift,
-                          long long run_cap, long long* nh_sum, int* run_cube, int* err,
+                          unsigned* sync, long long* s_total, long long s_cap, int* it_p, int advance,
+                          int k_shift, long long run_cap, long long* nh_sum, int* run_cube, int* err,
                           int* fold_list, int* fold_count, int fold_cap) {
     const long long nb = (C + VG_TREE - 1) / VG_TREE;
-    hipLaunchKernelGGL(vg_scan_blocks_kernel, dim3(1), dim3(VG_TREE), 0, st, nb, bsum, boff, s_total,
-                       s_cap, it_p, advance, err, fold_count);
+    const long long ng = (nb + VG_TREE - 1) / VG_TREE;
+    hipLaunchKernelGGL(vg_scan_blocks_kernel, dim3((unsigned)ng), dim3(VG_TREE), 0, st, nb, bsum, boff,
+                       bsum + nb, boff + nb, sync, s_total, s_cap, it_p, advance, err, fold_count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(vg_scan_cubes_kernel, dim3((unsigned)nb), dim3(256), 0, st, C, nh, boff,
+    hipLaunchKernelGGL(vg_scan_cubes_kernel, dim3((unsigned)nb), dim3(256), 0, st, C, nh, boff, boff + nb,
                        s_total, k_shift, run_cap, nh_sum, run_cube, fold_list, fold_count, fold_cap);
     return hipGetLastError();
 }
@@ -617,14 +679,14 @@ vg_bsum_kernel(int C, const int* __restrict__ nh, long long* __restrict__ bsum) 
 }
 
 hipError_t vg_launch_layout(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
-                            long long* s_total, long long s_cap, int* it_p, int k_shift, long long run_cap,
-                            long long* nh_sum, int* run_cube, int* err, int* fold_list,
+                            unsigned* sync, long long* s_total, long long s_cap, int* it_p, int k_shift,
+                            long long run_cap, long long* nh_sum, int* run_cube, int* err, int* fold_list,
                             int* fold_count, int fold_cap) {
     const unsigned nb = (unsigned)((C + VG_TREE - 1) / VG_TREE);
     hipLaunchKernelGGL(vg_bsum_kernel, dim3(nb), dim3(VG_TREE), 0, st, C, nh, bsum);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return vg_launch_scan(st, C, nh, bsum, boff, s_total, s_cap, it_p, 0, k_shift, run_cap, nh_sum,
+    return vg_launch_scan(st, C, nh, bsum, boff, sync, s_total, s_cap, it_p, 0, k_shift, run_cap, nh_sum,
                           run_cube, err, fold_list, fold_count, fold_cap);
 }
 

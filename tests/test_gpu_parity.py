@@ -242,6 +242,47 @@ def test_fp32_form_double_gaussian(cv):
         assert r["error"] / itg.exact < 5e-3
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_many_blocks_bit_exact(cv, orc, prec):
+    """1.68 M cubes: more than 1024 blocks of 1024 cubes, so the block-sum scan
+    and the tree over block partials each take a second (group) level.  The
+    sample layout (prefix sums and run table), per-cube sums, d_h, n_h and the
+    iteration record match the oracle bit for bit."""
+    from cuvegas_amd import integrands as I
+    d, n, ns, N, beta = 4, 32, 36, 4_000_000, 0.75
+    itg = I.poly_peak(d)
+    _say("many-blocks", prec)
+    rng = np.random.default_rng(8)
+    st = _integrator(cv, itg, d, n, ns, N, 2, prec)
+    assert st.n_cubes == ns ** d and (st.n_cubes + 1023) // 1024 > 1024
+    nh0 = rng.integers(2, 6, st.n_cubes).astype(np.int32)
+    st.set_nh(nh0)
+    x0 = st.get_map()
+    S = int(nh0.sum())
+    p, keep = _orc_params(orc, itg, d, n, ns, N, 1, 0 if prec == "fp64" else 1)
+    _, _, _, ojf = orc.sample_dump(p, keep, x0, nh0, 0, S)
+    oJF, oJF2 = orc.cube_sums(nh0, ojf, ojf * ojf, 16)
+    # one full iteration: sampling over the layout set_nh built, then the
+    # window folds, results, tree over block partials, allocation and scan
+    st.run(1, use_graph=False)
+    JF, JF2 = st.get_cube_sums()
+    assert np.array_equal(JF, oJF) and np.array_equal(JF2, oJF2)
+    V = math.pow(1.0 / ns, d)
+    odh, sums = orc.results(V, beta, nh0, oJF, oJF2)
+    assert np.array_equal(st.get_dh(), odh)
+    onh = orc.alloc(odh, sums[2], N)
+    assert np.array_equal(st.get_nh(), onh)
+    I_k, S2, ne = st.history()
+    assert I_k[0] == sums[0] and S2[0] == sums[1] and ne[0] == S
+    # the new allocation's layout: every sample of the next pass lands in the
+    # cube the oracle's prefix sums assign it to
+    S1 = int(onh.sum())
+    cube, _, _, _ = st.dump(1, S1)
+    ocube, _, _, _ = orc.sample_dump(p, keep, st.get_map(), onh, 1, S1)
+    assert np.array_equal(cube, ocube)
+    st.close()
+
+
 def test_large_shape_properties(cv):
     """Config-2 shape (8-D, 1024 bins, 8 strata, 1e7 per iteration): sizes and
     invariants that hold at any size."""
