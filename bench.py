@@ -60,7 +60,7 @@ def run_form(precision, itg, n_eval_it, warmup, steps, n_strat, nbins, seed, use
         mean = float((I_k[sl] * w).sum() / w.sum())
         K = k + 1 - warmup
         chi2 = float((((I_k[sl] - mean) ** 2) * w).sum() / (K - 1)) if K > 1 else 0.0
-        if abs(mean - itg.exact) / itg.exact <= 1e-4 and chi2 < 2.0:
+        if abs(mean - itg.exact) / abs(itg.exact) <= 1e-4 and chi2 < 2.0:
             t_to, it_to = cum, k
             break
     return {
@@ -71,7 +71,7 @@ def run_form(precision, itg, n_eval_it, warmup, steps, n_strat, nbins, seed, use
         "fill_ms_avg": float(np.mean(fill_ms[warmup:])),
         "evals_per_fill_avg": float(np.mean(ne[warmup:])),
         "result": r["result"], "error": r["error"], "chi2_dof": r["chi2_dof"],
-        "rel_err_vs_exact": abs(r["result"] - itg.exact) / itg.exact,
+        "rel_err_vs_exact": abs(r["result"] - itg.exact) / abs(itg.exact),
         "pull": (r["result"] - itg.exact) / r["error"],
         "time_to_1e-4_ms": t_to, "iteration_reached_1e-4": it_to,
         "tot_nevals": int(r["tot_nevals"]), "compile_ms": r["compile_ms"],
@@ -79,16 +79,18 @@ def run_form(precision, itg, n_eval_it, warmup, steps, n_strat, nbins, seed, use
     }
 
 
-def cpu_baseline(itg, nbins, n_strat, n_eval_it, seed):
+def cpu_baseline(itg, family, nbins, n_strat, n_eval_it, seed):
     """The CPU oracle (FP64 restatement of the reference algorithm) on a bounded
-    sample: the first iteration of the same workload, on every host core this
-    process may use (OpenMP over sample windows), and on one core."""
+    sample: the first iteration of the same workload (n_eval_it samples, the
+    caller may scale it down to keep the CPU time bounded), on every host core
+    this process may use (OpenMP over sample windows), and on one core."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
+    fid = {"double_gaussian": orc.F_DGAUSS, "oscillatory": orc.F_OSC}[family]
     cores = min(len(os.sched_getaffinity(0)), 16)
     out = {}
     for threads in (cores, 1):
-        p, keep = orc.make_params(itg.n_dim, nbins, n_strat, n_eval_it, 1, orc.F_DGAUSS,
+        p, keep = orc.make_params(itg.n_dim, nbins, n_strat, n_eval_it, 1, fid,
                                   list(itg.params), skip=0, seed=seed, threads=threads)
         t0 = time.perf_counter()
         r = orc.run(p, keep, n_strat ** itg.n_dim, snapshots=False)
@@ -196,9 +198,9 @@ def copy_probe():
     print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
 
 
-def _traffic_from_profile():
+def _traffic_from_profile(workload):
     """HBM bytes per vg_fill launch (FP32 form) from the committed PMC summary."""
-    path = os.path.join(ROOT, "profiles", "vg_fill_dg8_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"vg_fill_{workload}_hbm_traffic.md")
     try:
         with open(path) as f:
             for line in f:
@@ -214,16 +216,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=15)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--neval", type=float, default=1e8, help="evaluations per iteration")
+    ap.add_argument("--neval", type=float, default=None,
+                    help="evaluations per iteration (default: the workload's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
     ap.add_argument("--batch-config", default="",
                     help="batch512: 'block,run_len,task_windows' instead of autotuning")
     ap.add_argument("--skip-sequential", action="store_true",
                     help="batch512: time the batched run only (for profiling)")
-    ap.add_argument("--workload", choices=["dg8", "batch512", "copy"], default="dg8",
-                    help="dg8: the 8-D double Gaussian headline; batch512: 512-integral batch; "
-                         "copy: 1 GiB device copies for counter calibration")
+    ap.add_argument("--workload", choices=["dg8", "osc16", "batch512", "copy"], default="dg8",
+                    help="dg8: the 8-D double Gaussian headline; osc16: 16-D oscillatory; "
+                         "batch512: 512-integral batch; copy: 1 GiB device copies for counter "
+                         "calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -243,9 +247,23 @@ def main():
     from cuvegas_amd.roofline import slots_per_eval, valu_peak
     import ctypes as C
 
-    d, nbins, n_strat, seed = 8, 1024, 8, 0
-    n_eval_it = int(args.neval)
-    itg = I.double_gaussian(d, 0.33, 0.67, 0.05)
+    seed = 0
+    if args.workload == "dg8":
+        d, nbins, n_strat, family = 8, 1024, 8, "double_gaussian"
+        n_eval_it = int(args.neval or 1e8)
+        itg = I.double_gaussian(d, 0.33, 0.67, 0.05)
+        desc = (f"8-D double Gaussian peak sigma=0.05, nbins=1024, n_strat=8 (16,777,216 cubes), "
+                f"{n_eval_it:.0e} evals/iteration")
+        cpu_neval = n_eval_it
+    else:  # osc16: parameters drawn once (numpy default_rng(1)), committed as a fixture
+        with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
+            osc = json.load(f)["oscillatory_16d"]
+        d, nbins, n_strat, family = 16, 512, 2, "oscillatory"
+        n_eval_it = int(args.neval or 2e8)
+        itg = I.oscillatory(d, osc["u"], osc["a"])
+        desc = (f"16-D oscillatory cos(2 pi u + sum a_i x_i), a_i ~ U(0,5), nbins=512, n_strat=2 "
+                f"(65,536 cubes), {n_eval_it:.0e} evals/iteration")
+        cpu_neval = n_eval_it // 10  # bounded CPU sample: ~1/10 of an iteration
 
     n_cu, clk = C.c_int(0), C.c_int(0)
     arch = C.create_string_buffer(64)
@@ -262,18 +280,18 @@ def main():
         _log(prec, json.dumps(forms[prec]))
     head = forms.get("fp32") or forms["fp64"]
 
-    S = slots_per_eval(d, "double_gaussian")
+    S = slots_per_eval(d, family)
     peak = valu_peak(n_cu.value, clock_hz)
     achieved = S * head["evals_per_fill_avg"] / (head["fill_ms_avg"] * 1e-3)
     roof = {"bound": "valu", "kernel": "vg_fill",
             "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tslot/s",
-            "frac": achieved / peak, "slots_per_eval": S, "traffic": _traffic_from_profile(),
+            "frac": achieved / peak, "slots_per_eval": S, "traffic": _traffic_from_profile(args.workload),
             "fill_ms_avg": head["fill_ms_avg"], "evals_per_fill": head["evals_per_fill_avg"]}
 
     cpu = None
     if not args.no_cpu:
         _log("cpu baseline")
-        cpu = cpu_baseline(itg, nbins, n_strat, n_eval_it, seed)
+        cpu = cpu_baseline(itg, family, nbins, n_strat, cpu_neval, seed)
 
     out = {
         "metric": "integrand evals/s + time to 1e-4 rel. error, FP32-comp vs plain FP64, 1 GPU",
@@ -282,9 +300,7 @@ def main():
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if head is forms.get("fp32") else "f64",
         "data": "synthetic (closed-form integrand, Philox4x32-10 seed 0)",
-        "config": {"workload": "8-D double Gaussian peak sigma=0.05, nbins=1024, n_strat=8 "
-                               "(16,777,216 cubes), 1e8 evals/iteration, alpha=0.5, beta=0.75, "
-                               f"{args.warmup}+{args.steps} iterations",
+        "config": {"workload": f"{desc}, alpha=0.5, beta=0.75, {args.warmup}+{args.steps} iterations",
                    "n_eval_it": n_eval_it, "exact": itg.exact},
         "forms": forms,
         "fp32_over_fp64": (forms["fp32"]["evals_per_s"] / forms["fp64"]["evals_per_s"]

@@ -121,6 +121,7 @@ SIGNATURES = {
     "cvg_state_get_bins": (C.c_int, [_P, _PD, _PU]),
     "cvg_state_set_bins": (C.c_int, [_P, _PD, _PU]),
     "cvg_state_set_iteration": (C.c_int, [_P, C.c_int]),
+    "cvg_state_set_history": (C.c_int, [_P, _PD, _PD, _PLL, C.c_int]),
     "cvg_state_fill": (C.c_int, [_P]),
     "cvg_state_refine": (C.c_int, [_P]),
     "cvg_state_reallocate": (C.c_int, [_P, _PD, C.c_double]),

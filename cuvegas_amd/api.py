@@ -21,6 +21,8 @@ import numpy as np
 from . import integrands as _ints
 from ._lib import (CVG_FP32C, CVG_FP64, CuvegasError, check, cvg_params, cvg_result, lib)
 
+STATE_FORMAT = "cuvegas_amd.state.v1"
+
 PRECISIONS = {"fp64": CVG_FP64, "fp32": CVG_FP32C, "fp32c": CVG_FP32C}
 
 
@@ -210,6 +212,69 @@ class Integrator:
 
     def set_iteration(self, it: int) -> None:
         check(lib.cvg_state_set_iteration(self._h, int(it)))
+
+    def set_history(self, I, sigma2, nevals) -> None:
+        """Restore the per-iteration record; the next iteration is len(I)."""
+        I = np.ascontiguousarray(I, dtype=np.float64)
+        S2 = np.ascontiguousarray(sigma2, dtype=np.float64)
+        ne = np.ascontiguousarray(nevals, dtype=np.int64)
+        if not (I.shape == S2.shape == ne.shape and I.ndim == 1):
+            raise ValueError("I, sigma2 and nevals must be 1-D arrays of one length")
+        check(lib.cvg_state_set_history(self._h, _dptr(I), _dptr(S2), _llptr(ne), int(I.size)))
+
+    # -- on-disk state -------------------------------------------------------
+    # An .npz archive of plain arrays (np.load(path, allow_pickle=False) reads
+    # it): the shape and run parameters, the map edges, the allocation n_h and
+    # the per-iteration (I_k, sigma_k^2, samples).  Restoring it reproduces the
+    # iteration the saved run would have run next: same map, same allocation
+    # and sample layout, same Philox counter block.  Not saved: the compiled
+    # integrand (the caller passes the source again), the bin accumulators
+    # (cleared at the end of every iteration) and the autograd record.
+
+    def save(self, path) -> None:
+        """Write the run's state (see above) to `path` (.npz)."""
+        I, S2, ne = self.history()
+        p = self.params
+        dom = self._dom if self._dom is not None else np.zeros(0)
+        np.savez(path, format=np.array(STATE_FORMAT), n_dim=p.n_dim, n_intervals=p.n_intervals,
+                 n_strat=p.n_strat, n_eval_it=p.n_eval_it, max_it=p.max_it, skip=p.skip,
+                 alpha=p.alpha, beta=p.beta, seed=p.seed, stream_id=p.stream_id,
+                 precision=p.precision, run_len=p.run_len, domain=dom, x_edges=self.get_map(),
+                 nh=self.get_nh(), I=I, sigma2=S2, nevals=ne)
+
+    def restore(self, path) -> "Integrator":
+        """Load a saved state into this integrator (same n_dim, n_intervals, n_strat)."""
+        with np.load(path, allow_pickle=False) as z:
+            if "format" not in z or str(z["format"]) != STATE_FORMAT:
+                raise ValueError(f"{path}: not a {STATE_FORMAT} archive")
+            shape = (int(z["n_dim"]), int(z["n_intervals"]), int(z["n_strat"]))
+            if shape != (self.n_dim, self.n_intervals, self.n_strat):
+                raise ValueError(f"saved shape (n_dim, n_intervals, n_strat) = {shape} differs from "
+                                 f"{(self.n_dim, self.n_intervals, self.n_strat)}")
+            if z["I"].size > self.max_it:
+                raise ValueError(f"saved run has {z['I'].size} iterations; max_it is {self.max_it}")
+            self.set_map(z["x_edges"])
+            self.set_nh(z["nh"])
+            self.set_history(z["I"], z["sigma2"], z["nevals"])
+        return self
+
+    @classmethod
+    def from_state(cls, path, source: str, *, max_it: Optional[int] = None,
+                   options: Optional[str] = None, program: Optional[Program] = None) -> "Integrator":
+        """A new integrator with the saved parameters, restored from `path`;
+        max_it (history capacity) may be raised to continue the run."""
+        with np.load(path, allow_pickle=False) as z:
+            if "format" not in z or str(z["format"]) != STATE_FORMAT:
+                raise ValueError(f"{path}: not a {STATE_FORMAT} archive")
+            dom = z["domain"]
+            kw = dict(n_intervals=int(z["n_intervals"]), n_strat=int(z["n_strat"]),
+                      max_it=int(max_it if max_it is not None else z["max_it"]), skip=int(z["skip"]),
+                      alpha=float(z["alpha"]), beta=float(z["beta"]), seed=int(z["seed"]),
+                      precision=int(z["precision"]), stream_id=int(z["stream_id"]),
+                      run_len=int(z["run_len"]), domain=dom.reshape(2, -1) if dom.size else None)
+            n_dim, n_eval_it = int(z["n_dim"]), int(z["n_eval_it"])
+        itg = cls(source, n_dim, n_eval_it, options=options, program=program, **kw)
+        return itg.restore(path)
 
     def set_theta(self, theta) -> None:
         t = np.ascontiguousarray(theta, dtype=np.float64)

@@ -250,10 +250,9 @@ struct cvg_state {
     }
 
     int launch_results() {
-        VG_HIP(vg_launch_bigfold(stream, fold_list, fold_count, fold_cap, nh_sum, win_first, win_last,
-                                 6 + k_shift, JF, JF2));
-        VG_HIP(vg_launch_results(stream, (int)C, V, p.beta, nh, nh_sum, JF, JF2, win_first, win_last,
-                                 6 + k_shift, dh, part));
+        VG_HIP(vg_launch_bigfold(stream, fold_list, fold_count, fold_cap, nh_sum, run_cube, s_total,
+                                 win_first, win_last, 6 + k_shift, JF, JF2));
+        VG_HIP(vg_launch_results(stream, (int)C, V, p.beta, nh, JF, JF2, dh, part));
         VG_HIP(vg_launch_finalize(stream, nb, part, sync, s_total, it_p, p.max_it, hist, dsum, err));
         return CVG_OK;
     }
@@ -805,6 +804,26 @@ int cvg_state_set_bins(cvg_state* st, const double* weights, const unsigned int*
     int rc = copy_h2d(st, st->weights, weights, sizeof(double) * dn);
     if (rc) return rc;
     return copy_h2d(st, st->counts, counts, sizeof(unsigned) * dn);
+}
+
+int cvg_state_set_history(cvg_state* st, const double* I, const double* sigma2, const long long* nevals,
+                          int count) {
+    if (!st || count < 0) return fail(CVG_ERR_INVALID, "bad argument");
+    if (count > 0 && (!I || !sigma2 || !nevals)) return fail(CVG_ERR_INVALID, "NULL argument");
+    if (count > st->p.max_it) return fail(CVG_ERR_INVALID, "count exceeds the state's max_it");
+    if (count > 0) {
+        std::vector<double> h(4 * (size_t)count);
+        for (int i = 0; i < count; ++i) {
+            h[4 * i + 0] = I[i];
+            h[4 * i + 1] = sigma2[i];
+            h[4 * i + 2] = 0.0;  // D of a restored iteration is not kept
+            h[4 * i + 3] = (double)nevals[i];
+        }
+        int rc = copy_h2d(st, st->hist, h.data(), sizeof(double) * h.size());
+        if (rc) return rc;
+    }
+    st->iterations_done = count;
+    return copy_h2d(st, st->it_p, &count, sizeof(int));
 }
 
 int cvg_state_set_iteration(cvg_state* st, int it) {

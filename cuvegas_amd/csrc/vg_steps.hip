@@ -151,19 +151,36 @@ __device__ inline void vg_cube_result(long long h, int C, double V, double beta,
     dh[h] = dd;
 }
 
+// Results of cube h from complete per-cube sums (the window folds ran before).
+__device__ inline void vg_cube_result_sums(long long h, int C, double V, double beta,
+                                           const int* __restrict__ nh, const double* __restrict__ JF,
+                                           const double* __restrict__ JF2, double* __restrict__ dh,
+                                           double& sr, double& ss, double& dd) {
+    sr = 0.0; ss = 0.0; dd = 0.0;
+    if (h >= C) return;
+    const int neval = nh[h];
+    const double j1 = JF[h], j2 = JF2[h];
+    const double Ih = j1 / neval * V;
+    const double Sig2 = (j2 / neval * V * V) - (Ih * Ih);
+    sr = Ih;
+    ss = Sig2 / neval;
+    const double u = V * j1 / neval;
+    double t = V * V * j2 / neval - u * u;
+    if (t < 0) t = 0;
+    dd = vg_pow(t, beta);
+    dh[h] = dd;
+}
+
 __global__ void __launch_bounds__(256)
 vg_results_kernel(int C, double V, double beta, const int* __restrict__ nh,
-                  const long long* __restrict__ nh_sum, double* __restrict__ JF,
-                  double* __restrict__ JF2, const double2* __restrict__ win_first,
-                  const double2* __restrict__ win_last, int ws_shift, double* __restrict__ dh,
-                  double* __restrict__ part) {
+                  const double* __restrict__ JF, const double* __restrict__ JF2,
+                  double* __restrict__ dh, double* __restrict__ part) {
     __shared__ double red[256];
     const long long base = (long long)blockIdx.x * VG_TREE + threadIdx.x;
     double r[4][3];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-        vg_cube_result(base + 256 * j, C, V, beta, nh, nh_sum, JF, JF2, win_first, win_last, ws_shift,
-                       dh, r[j][0], r[j][1], r[j][2]);
+        vg_cube_result_sums(base + 256 * j, C, V, beta, nh, JF, JF2, dh, r[j][0], r[j][1], r[j][2]);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const double tot = vg_tree1024_256(r[0][q], r[1][q], r[2][q], r[3][q], red);
@@ -209,15 +226,44 @@ __device__ void vg_fold_tree(long long wa, long long m, const double2* __restric
     __syncthreads();
 }
 
+// Blocks [0, VG_BIGFOLD_BLOCKS): the tree folds above.  The other blocks:
+// cubes spanning 2..VG_FOLD_MAX windows, found at window starts -- the cube
+// holding the first sample of window w (run_cube[64 w]) crosses from window
+// w - 1 when it starts there; its partials are left-folded in window order,
+// win_last[w - 1] + win_first[w] + ... + win_first[wb], into JF / JF2.
+#define VG_BIGFOLD_BLOCKS 128
 __global__ void __launch_bounds__(256)
 vg_bigfold_kernel(const int* __restrict__ list, const int* __restrict__ count, int cap,
-                  const long long* __restrict__ nh_sum, const double2* __restrict__ win_first,
+                  const long long* __restrict__ nh_sum, const int* __restrict__ run_cube,
+                  const long long* __restrict__ s_total, const double2* __restrict__ win_first,
                   const double2* __restrict__ win_last, int ws_shift, double* __restrict__ JF,
                   double* __restrict__ JF2) {
     __shared__ double red[256];
     __shared__ double part[2 * 2048];
+    if (blockIdx.x >= VG_BIGFOLD_BLOCKS) {
+        const long long nwin = (*s_total + (1LL << ws_shift) - 1) >> ws_shift;
+        const long long stride = (long long)(gridDim.x - VG_BIGFOLD_BLOCKS) * blockDim.x;
+        for (long long w = 1 + (long long)(blockIdx.x - VG_BIGFOLD_BLOCKS) * blockDim.x + threadIdx.x;
+             w < nwin; w += stride) {
+            const int h = run_cube[w << 6];
+            const long long st = nh_sum[h];
+            if ((st >> ws_shift) != w - 1) continue;  // starts at window w, or folded from an earlier one
+            const long long wb = (nh_sum[h + 1] - 1) >> ws_shift;
+            if (wb - (w - 1) + 1 > VG_FOLD_MAX) continue;  // tree fold
+            double2 v = win_last[w - 1];
+            double j1 = v.x, j2 = v.y;
+            for (long long u = w; u <= wb; ++u) {
+                v = win_first[u];
+                j1 = j1 + v.x;
+                j2 = j2 + v.y;
+            }
+            JF[h] = j1;
+            JF2[h] = j2;
+        }
+        return;
+    }
     const int n = *count < cap ? *count : cap;
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    for (int i = blockIdx.x; i < n; i += VG_BIGFOLD_BLOCKS) {
         const int h = list[i];
         const long long st = nh_sum[h], en = nh_sum[h + 1];
         const long long wa = st >> ws_shift, wb = (en - 1) >> ws_shift;
@@ -228,10 +274,11 @@ vg_bigfold_kernel(const int* __restrict__ list, const int* __restrict__ count, i
 }
 
 hipError_t vg_launch_bigfold(hipStream_t st, const int* list, const int* count, int cap,
-                             const long long* nh_sum, const double2* win_first,
-                             const double2* win_last, int ws_shift, double* JF, double* JF2) {
-    hipLaunchKernelGGL(vg_bigfold_kernel, dim3(128), dim3(256), 0, st, list, count, cap, nh_sum,
-                       win_first, win_last, ws_shift, JF, JF2);
+                             const long long* nh_sum, const int* run_cube, const long long* s_total,
+                             const double2* win_first, const double2* win_last, int ws_shift,
+                             double* JF, double* JF2) {
+    hipLaunchKernelGGL(vg_bigfold_kernel, dim3(VG_BIGFOLD_BLOCKS + 256), dim3(256), 0, st, list, count,
+                       cap, nh_sum, run_cube, s_total, win_first, win_last, ws_shift, JF, JF2);
     return hipGetLastError();
 }
 
@@ -390,25 +437,41 @@ vg_scan_cubes_kernel(int C, const int* __restrict__ nh, const long long* __restr
     __shared__ int nbig;
     const long long base = (long long)blockIdx.x * VG_TREE;
     const long long h0 = base + 4 * threadIdx.x;
+    const bool full = h0 + 4 <= C;  // the 4 cubes in range: 16-byte load, 2 x 16-byte stores
     long long v[4];
     long long own = 0;
+    if (full) {
+        const int4 q = *reinterpret_cast<const int4*>(nh + h0);  // h0 % 4 == 0: aligned
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        v[j] = h0 + j < C ? nh[h0 + j] : 0;
-        own += v[j];
+        for (int j = 0; j < 4; ++j) v[j] = h0 + j < C ? nh[h0 + j] : 0;
     }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) own += v[j];
     if (threadIdx.x == 0) nbig = 0;
     long long total;
     const long long S = *s_total;
     long long ex = vg_block_scan_excl(own, wt, &total) + boff[blockIdx.x] + goff[blockIdx.x / VG_TREE];
     const long long kmask = (1LL << k_shift) - 1;
+    if (full) {
+        long long e[5];
+        e[0] = ex;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) e[j + 1] = e[j] + v[j];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) e[j] = e[j] < S ? e[j] : S;
+        longlong2* o = reinterpret_cast<longlong2*>(nh_sum + h0);
+        o[0] = make_longlong2(e[0], e[1]);
+        o[1] = make_longlong2(e[2], e[3]);
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const long long h = h0 + j;
         if (h < C) {
             const long long st = ex < S ? ex : S;
             const long long en = ex + v[j] < S ? ex + v[j] : S;
-            nh_sum[h] = st;
+            if (!full) nh_sum[h] = st;
             if (h == C - 1) nh_sum[C] = S;
             const long long r0 = (st + kmask) >> k_shift;
             long long r1 = (en + kmask) >> k_shift;
@@ -585,12 +648,9 @@ long long vg_refine_scratch_doubles(int d, int n) {
 // ---------------------------------------------------------------------------
 
 hipError_t vg_launch_results(hipStream_t st, int C, double V, double beta, const int* nh,
-                             const long long* nh_sum, double* JF, double* JF2,
-                             const double2* win_first, const double2* win_last, int ws_shift,
-                             double* dh, double* part) {
+                             const double* JF, const double* JF2, double* dh, double* part) {
     const unsigned nb = (unsigned)((C + VG_TREE - 1) / VG_TREE);
-    hipLaunchKernelGGL(vg_results_kernel, dim3(nb), dim3(256), 0, st, C, V, beta, nh, nh_sum,
-                       JF, JF2, win_first, win_last, ws_shift, dh, part);
+    hipLaunchKernelGGL(vg_results_kernel, dim3(nb), dim3(256), 0, st, C, V, beta, nh, JF, JF2, dh, part);
     return hipGetLastError();
 }
 

@@ -156,6 +156,14 @@ int cvg_state_get_bins(cvg_state* st, double* weights, unsigned int* counts);
 int cvg_state_set_bins(cvg_state* st, const double* weights, const unsigned int* counts);
 /* Iteration counter (selects the Philox counter block). */
 int cvg_state_set_iteration(cvg_state* st, int it);
+/* Restore the per-iteration record of a saved run (no reference counterpart;
+ * the reference keeps no state between calls): iterations [0, count) get
+ * I_k, sigma_k^2 and samples, and both the device iteration counter and the
+ * count of iterations done become count, so the next run continues with
+ * iteration count and result() averages the restored iterations >= skip.
+ * Together with set_map / set_nh this resumes or warm-starts a run. */
+int cvg_state_set_history(cvg_state* st, const double* I, const double* sigma2, const long long* nevals,
+                          int count);
 
 /* Single steps, for callers that drive their own loop. */
 int cvg_state_fill(cvg_state* st);      /* one sampling pass with the current map */

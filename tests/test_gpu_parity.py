@@ -283,6 +283,59 @@ def test_many_blocks_bit_exact(cv, orc, prec):
     st.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_oscillatory_16d_closed_form(cv, prec):
+    """The 16-D oscillatory workload (n = 512, n_strat = 2: 65,536 cubes of
+    ~300 samples, each spanning windows) at 2e7 samples per iteration: the
+    averaged result (5 iterations after 3 skipped) is within 4 sigma of
+    cos(2 pi u + sum a_i / 2) prod sin(a_i / 2) / (a_i / 2), with chi2/dof < 3."""
+    import json
+    import os
+    from cuvegas_amd import integrands as I
+    with open(os.path.join(os.path.dirname(__file__), "golden", "workloads.json")) as f:
+        osc = json.load(f)["oscillatory_16d"]
+    itg = I.oscillatory(16, osc["u"], osc["a"])
+    _say("oscillatory-16d", prec)
+    st = _integrator(cv, itg, 16, 512, 2, 20_000_000, 8, prec, skip=3)
+    st.run(8)
+    r = st.result()
+    st.close()
+    assert abs(r["result"] - itg.exact) < 4 * r["error"], (r["result"], itg.exact, r["error"])
+    assert r["chi2_dof"] < 3.0
+    # |f| <= 1: the error is at most the plain Monte Carlo error of 5 x 2e7 samples
+    assert 0 < r["error"] < 1.0 / math.sqrt(5 * 20_000_000)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_state_save_restore_resumes(cv, tmp_path, prec):
+    """A run saved after 4 iterations and restored into a fresh integrator runs
+    the same 5th iteration as the original continuing: same I_k and sigma_k^2,
+    the same new allocation, the same combined result."""
+    from cuvegas_amd import Integrator
+    from cuvegas_amd import integrands as I
+    itg = I.gaussian_peak(5, 0.5, 0.1)
+    _say("save-restore", prec)
+    a = Integrator(itg.source, 5, 200_000, n_intervals=128, max_it=6, skip=2, seed=9, precision=prec)
+    a.run(4)
+    path = str(tmp_path / "state.npz")
+    a.save(path)
+    a.run(1)
+    b = Integrator.from_state(path, itg.source)
+    assert np.array_equal(b.get_map(), np.load(path)["x_edges"])
+    b.run(1)
+    Ia, Sa, na = a.history()
+    Ib, Sb, nb = b.history()
+    assert len(Ib) == 5
+    assert np.array_equal(Ia, Ib) and np.array_equal(Sa, Sb) and np.array_equal(na, nb)
+    assert np.array_equal(a.get_nh(), b.get_nh())
+    ra, rb = a.result(), b.result()
+    assert ra["result"] == rb["result"] and ra["error"] == rb["error"]
+    with pytest.raises(ValueError):
+        Integrator(itg.source, 4, 200_000, n_intervals=128, max_it=6, precision=prec).restore(path)
+    a.close()
+    b.close()
+
+
 def test_large_shape_properties(cv):
     """Config-2 shape (8-D, 1024 bins, 8 strata, 1e7 per iteration): sizes and
     invariants that hold at any size."""
