@@ -177,6 +177,67 @@ def batch_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def mix6_bench(args):
+    """6-D four-component Gaussian mixture (32 parameters from the committed
+    fixture, numpy default_rng(2)): 5e7 evaluations over 10 iterations (5e6 per
+    iteration, 5 averaged), forward with recording, then the backward replay of
+    the 5 recorded iterations, d value / d theta.  Reports forward evals/s per
+    form, backward time, and the gradient's pulls against the closed-form erf
+    gradient."""
+    import numpy as np
+    from cuvegas_amd import Integrator
+    from cuvegas_amd import integrands as I
+    with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
+        theta = np.array(json.load(f)["mixture_6d"]["theta"])
+    d, nbins, iters, skip = 6, 1024, 10, 5
+    n_eval_it = int(args.neval or 5e6)
+    itg = I.gaussian_mixture_param(d, theta)
+    exact_grad = np.array(itg.exact_grad(theta))
+    forms = {}
+    for prec in (["fp64", "fp32"] if args.only is None else [args.only]):
+        _log("mixture", prec)
+        st = Integrator(itg.source, d, n_eval_it, n_intervals=nbins, max_it=iters, skip=skip, seed=0,
+                        precision=prec, theta=theta, options="grad")
+        st.set_record(True)
+        st.set_timing(True)
+        st.run(iters, use_graph=False)
+        r = st.result()
+        I_k, S2, ne = st.history()
+        fill_ms, iter_ms = st.timing()
+        st.backward()  # first call allocates the replay buffers
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            v, vv, g, gv = st.backward()
+        bw_ms = (time.perf_counter() - t0) * 1e3 / reps
+        replayed = int(ne[skip:].sum())
+        pull = (g - exact_grad) / np.sqrt(gv)
+        forms[prec] = {
+            "precision": prec, "n_strat": st.n_strat, "n_cubes": st.n_cubes,
+            "forward_evals_per_s": float(ne.sum() / (iter_ms.sum() * 1e-3)),
+            "forward_ms_per_iteration": float(iter_ms.mean()), "fill_ms_avg": float(fill_ms.mean()),
+            "result": r["result"], "error": r["error"], "chi2_dof": r["chi2_dof"],
+            "pull_value": (r["result"] - itg.exact) / r["error"],
+            "backward_ms": bw_ms, "backward_replayed_evals": replayed,
+            "backward_evals_per_s": replayed / (bw_ms * 1e-3),
+            "grad_pull_rms": float(np.sqrt(np.mean(pull ** 2))),
+            "grad_pull_max": float(np.max(np.abs(pull))), "n_theta": int(theta.size),
+        }
+        _log(prec, json.dumps(forms[prec]))
+        st.close()
+    head = forms.get("fp32") or forms["fp64"]
+    print(json.dumps({
+        "metric": "6-D Gaussian mixture: forward evals/s, backward (32-parameter gradient) time",
+        "value": head["forward_evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": iters,
+        "warmup": 0, "ms_per_step": head["forward_ms_per_iteration"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if head is forms.get("fp32") else "f64",
+        "data": "synthetic (mixture parameters from numpy default_rng(2), committed fixture)",
+        "config": {"workload": f"6-D 4-component Gaussian mixture, nbins={nbins}, {n_eval_it:.0e} "
+                               f"evals/iteration, {iters} iterations ({iters - skip} averaged and "
+                               "replayed by the backward)", "exact": itg.exact},
+        "forms": forms}), flush=True)
+
+
 def copy_probe():
     """Three device-to-device copies of 1 GiB: the calibration kernel for the
     FETCH_SIZE / WRITE_SIZE counters (bytes known exactly)."""
@@ -224,10 +285,10 @@ def main():
                     help="batch512: 'block,run_len,task_windows' instead of autotuning")
     ap.add_argument("--skip-sequential", action="store_true",
                     help="batch512: time the batched run only (for profiling)")
-    ap.add_argument("--workload", choices=["dg8", "osc16", "batch512", "copy"], default="dg8",
+    ap.add_argument("--workload", choices=["dg8", "osc16", "mix6", "batch512", "copy"], default="dg8",
                     help="dg8: the 8-D double Gaussian headline; osc16: 16-D oscillatory; "
-                         "batch512: 512-integral batch; copy: 1 GiB device copies for counter "
-                         "calibration")
+                         "mix6: 6-D mixture forward + gradient; batch512: 512-integral batch; "
+                         "copy: 1 GiB device copies for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -240,6 +301,9 @@ def main():
         return
     if args.workload == "copy":
         copy_probe()
+        return
+    if args.workload == "mix6":
+        mix6_bench(args)
         return
 
     from cuvegas_amd import integrands as I

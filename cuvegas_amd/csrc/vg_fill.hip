@@ -367,8 +367,9 @@ vg_dump(const vg_map_t* __restrict__ gmap, const long long* __restrict__ nh_sum,
 // Philox counters of forward iteration it).  The integrand supplies
 //   vg_real integrand_grad(vg_real* x, const vg_real* theta, vg_real* grad)
 // returning f and writing df/dtheta[0..VG_NTHETA).  Samples are regenerated,
-// not stored.  One lane walks whole cubes (grid-stride), so per-cube sums of
-// J df/dtheta and their squares give the stratified variance of the gradient:
+// not stored.  Per-cube sums of J df/dtheta and their squares (each cube
+// replayed by one lane, or by a whole wave when large, see vg_grad) give the
+// stratified variance of the gradient:
 //   G_p   = sum_h V/n_h sum_s J df_p
 //   Var_p = sum_h (V^2 sum_s (J df_p)^2 / n_h - (V sum_s J df_p / n_h)^2) / (n_h - 1)
 // (unbiased per-cube sample variance; n_h >= 2 always)
@@ -377,7 +378,93 @@ vg_dump(const vg_map_t* __restrict__ gmap, const long long* __restrict__ nh_sum,
 // differences.  part[block][2 + 2P]: fixed-order block partials.
 // ---------------------------------------------------------------------------
 constexpr int NT = VG_NTHETA;
+// Cubes with more samples than this are replayed by the whole wave.
+constexpr int VG_GRAD_LANE_MAX = 16;
 
+// J f and J df/dtheta (FP64) of sample s of cube h under the frozen map.
+__device__ inline void vg_grad_sample(long long h, long long s, unsigned it, const vg_map_t* __restrict__ gmap,
+                                      const double* __restrict__ domain, double domain_vol,
+                                      const vg_real* __restrict__ theta, unsigned seed, unsigned stream,
+                                      double& jf, double* jg) {
+    unsigned words[4 * NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const vg_u4 v = vg_sample_words((unsigned long long)s, it, (unsigned)g, seed, stream);
+        words[4 * g + 0] = v.x; words[4 * g + 1] = v.y;
+        words[4 * g + 2] = v.z; words[4 * g + 3] = v.w;
+    }
+    unsigned tmp = (unsigned)h;
+    vg_real x[D], dxv[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        const unsigned q = tmp / NS;
+        const unsigned k = tmp - q * NS;
+        tmp = q;
+        const vg_binpos b = vg_bin_of(words[i], k * (unsigned)NB, (unsigned)NB, NS);
+#if VG_FP32
+        constexpr float inv_ns = 1.0f / (float)NS;
+        const float frac = ((float)b.rem + (float)(unsigned)(b.low >> 9) * 0x1p-24f) * inv_ns;
+        const float2 m = gmap[i * NB + b.id];
+        x[i] = fmaf(m.y, frac, m.x);
+#else
+        constexpr double inv_ns = 1.0 / (double)NS;
+        const double frac = ((double)b.rem + (double)b.low * 0x1p-33) * inv_ns;
+        const double2 m = gmap[i * NB + b.id];
+        x[i] = fma(m.y, frac, m.x);
+#endif
+        dxv[i] = m.y;
+    }
+    vg_real jac = (vg_real)1;
+#pragma unroll
+    for (int i = 0; i < D; ++i) jac = jac * ((vg_real)NB * dxv[i]);
+#if VG_HAS_DOMAIN
+#pragma unroll
+    for (int i = 0; i < D; ++i) x[i] = x[i] * (vg_real)domain[D + i] + (vg_real)domain[i];
+#endif
+    vg_real gr[NT];
+    vg_real f = integrand_grad(x, theta, gr);
+#if VG_HAS_DOMAIN
+    f = (vg_real)domain_vol * f;
+#pragma unroll
+    for (int p = 0; p < NT; ++p) gr[p] = (vg_real)domain_vol * gr[p];
+#endif
+    jf = (double)(f * jac);
+#pragma unroll
+    for (int p = 0; p < NT; ++p) jg[p] = (double)(gr[p] * jac);
+}
+
+// Adds the contributions of one cube (n samples, sums sf, sf2, g1, g2) to acc,
+// with the forward's results expressions for I_h and sigma_h^2 / n_h.
+__device__ inline void vg_grad_cube(int n, double V, double sf, double sf2, const double* g1,
+                                    const double* g2, double* acc) {
+    const double Ih = sf / n * V;
+    const double Sig2 = (sf2 / n * V * V) - (Ih * Ih);
+    acc[0] = acc[0] + Ih;
+    acc[1] = acc[1] + Sig2 / n;
+#pragma unroll
+    for (int p = 0; p < NT; ++p) {
+        const double Gh = g1[p] / n * V;
+        const double Gv = (g2[p] / n * V * V) - (Gh * Gh);
+        acc[2 + p] = acc[2 + p] + Gh;
+        acc[2 + NT + p] = acc[2 + NT + p] + Gv / (n - 1);
+    }
+}
+
+// Wave 64 sum, the same value in every lane: fixed butterfly order, so lane 0
+// (and every lane) gets a result independent of timing.
+__device__ inline double vg_wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Each wave takes groups of 64 consecutive cubes (grid-stride), lane l owning
+// cube group + l, whose contributions it accumulates in cube order.  A cube of
+// at most VG_GRAD_LANE_MAX samples is replayed by its own lane; larger cubes
+// of the group are replayed by the whole wave, one after another, samples
+// strided over the lanes, and summed in a fixed order.  The allocation puts
+// most cubes at n_h = 2 and a few at thousands of samples, which one lane per
+// cube would leave mostly idle.
 extern "C" __global__ void __launch_bounds__(256)
 vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
         const long long* __restrict__ nh_sum, int C, unsigned it, double V,
@@ -388,76 +475,51 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
     double acc[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-    for (long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x; h < C;
-         h += (long long)gridDim.x * blockDim.x) {
-        const int n = nh[h];
-        const long long s0 = nh_sum[h];
-        double sf = 0.0, sf2 = 0.0, g1[NT], g2[NT];
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long g0 = wave * 64; g0 < C; g0 += nwaves * 64) {
+        const long long h = g0 + lane;
+        const int n = h < C ? nh[h] : 0;
+        const bool own = h < C && n <= VG_GRAD_LANE_MAX;
+        if (own) {
+            const long long s0 = nh_sum[h];
+            double sf = 0.0, sf2 = 0.0, g1[NT], g2[NT];
 #pragma unroll
-        for (int p = 0; p < NT; ++p) { g1[p] = 0.0; g2[p] = 0.0; }
-        for (int e = 0; e < n; ++e) {
-            unsigned words[4 * NG];
+            for (int p = 0; p < NT; ++p) { g1[p] = 0.0; g2[p] = 0.0; }
+            for (int e = 0; e < n; ++e) {
+                double jf, jg[NT];
+                vg_grad_sample(h, s0 + e, it, gmap, domain, domain_vol, theta, seed, stream, jf, jg);
+                sf = sf + jf;
+                sf2 = sf2 + jf * jf;
 #pragma unroll
-            for (int g = 0; g < NG; ++g) {
-                const vg_u4 v = vg_sample_words((unsigned long long)(s0 + e), it, (unsigned)g, seed, stream);
-                words[4 * g + 0] = v.x; words[4 * g + 1] = v.y;
-                words[4 * g + 2] = v.z; words[4 * g + 3] = v.w;
+                for (int p = 0; p < NT; ++p) { g1[p] = g1[p] + jg[p]; g2[p] = g2[p] + jg[p] * jg[p]; }
             }
-            unsigned tmp = (unsigned)h;
-            vg_real x[D], dxv[D];
-#pragma unroll
-            for (int i = 0; i < D; ++i) {
-                const unsigned q = tmp / NS;
-                const unsigned k = tmp - q * NS;
-                tmp = q;
-                const vg_binpos b = vg_bin_of(words[i], k * (unsigned)NB, (unsigned)NB, NS);
-#if VG_FP32
-                constexpr float inv_ns = 1.0f / (float)NS;
-                const float frac = ((float)b.rem + (float)(unsigned)(b.low >> 9) * 0x1p-24f) * inv_ns;
-                const float2 m = gmap[i * NB + b.id];
-                x[i] = fmaf(m.y, frac, m.x);
-#else
-                constexpr double inv_ns = 1.0 / (double)NS;
-                const double frac = ((double)b.rem + (double)b.low * 0x1p-33) * inv_ns;
-                const double2 m = gmap[i * NB + b.id];
-                x[i] = fma(m.y, frac, m.x);
-#endif
-                dxv[i] = m.y;
-            }
-            vg_real jac = (vg_real)1;
-#pragma unroll
-            for (int i = 0; i < D; ++i) jac = jac * ((vg_real)NB * dxv[i]);
-#if VG_HAS_DOMAIN
-#pragma unroll
-            for (int i = 0; i < D; ++i) x[i] = x[i] * (vg_real)domain[D + i] + (vg_real)domain[i];
-#endif
-            vg_real gr[NT];
-            vg_real f = integrand_grad(x, theta, gr);
-#if VG_HAS_DOMAIN
-            f = (vg_real)domain_vol * f;
-#pragma unroll
-            for (int p = 0; p < NT; ++p) gr[p] = (vg_real)domain_vol * gr[p];
-#endif
-            const double jf = (double)(f * jac);
-            sf = sf + jf;
-            sf2 = sf2 + jf * jf;
-#pragma unroll
-            for (int p = 0; p < NT; ++p) {
-                const double gj = (double)(gr[p] * jac);
-                g1[p] = g1[p] + gj;
-                g2[p] = g2[p] + gj * gj;
-            }
+            vg_grad_cube(n, V, sf, sf2, g1, g2, acc);
         }
-        const double Ih = sf / n * V;
-        const double Sig2 = (sf2 / n * V * V) - (Ih * Ih);
-        acc[0] = acc[0] + Ih;
-        acc[1] = acc[1] + Sig2 / n;
+        unsigned long long big = __ballot(h < C && !own);
+        while (big) {
+            const int j = __ffsll((long long)big) - 1;
+            big &= big - 1;
+            const long long hb = g0 + j;
+            const int nb = __shfl(n, j, 64);
+            const long long s0 = nh_sum[hb];
+            double sf = 0.0, sf2 = 0.0, g1[NT], g2[NT];
 #pragma unroll
-        for (int p = 0; p < NT; ++p) {
-            const double Gh = g1[p] / n * V;
-            const double Gv = (g2[p] / n * V * V) - (Gh * Gh);
-            acc[2 + p] = acc[2 + p] + Gh;
-            acc[2 + NT + p] = acc[2 + NT + p] + Gv / (n - 1);
+            for (int p = 0; p < NT; ++p) { g1[p] = 0.0; g2[p] = 0.0; }
+            for (int e = lane; e < nb; e += 64) {
+                double jf, jg[NT];
+                vg_grad_sample(hb, s0 + e, it, gmap, domain, domain_vol, theta, seed, stream, jf, jg);
+                sf = sf + jf;
+                sf2 = sf2 + jf * jf;
+#pragma unroll
+                for (int p = 0; p < NT; ++p) { g1[p] = g1[p] + jg[p]; g2[p] = g2[p] + jg[p] * jg[p]; }
+            }
+            sf = vg_wave_sum(sf);
+            sf2 = vg_wave_sum(sf2);
+#pragma unroll
+            for (int p = 0; p < NT; ++p) { g1[p] = vg_wave_sum(g1[p]); g2[p] = vg_wave_sum(g2[p]); }
+            if (lane == j) vg_grad_cube(nb, V, sf, sf2, g1, g2, acc);
         }
     }
     // fixed-order block reduction, one quantity at a time
