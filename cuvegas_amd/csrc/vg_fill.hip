@@ -433,70 +433,76 @@ __device__ inline void vg_grad_sample(long long h, long long s, unsigned it, con
     for (int p = 0; p < NT; ++p) jg[p] = (double)(gr[p] * jac);
 }
 
-// Adds the contributions of one cube (n samples, sums sf, sf2, g1, g2) to acc,
-// with the forward's results expressions for I_h and sigma_h^2 / n_h.
-__device__ inline void vg_grad_cube(int n, double V, double sf, double sf2, const double* g1,
-                                    const double* g2, double* acc) {
+// Contributions of one cube (n samples, sums sf, sf2, g1, g2), with the
+// forward's results expressions for I_h and sigma_h^2 / n_h, written over the
+// sums: sf <- I_h, sf2 <- sigma_h^2 / n_h, g1[p] <- G_h,p, g2[p] <- Var_h,p.
+// The value terms keep the forward's division order (they replay I_k); the
+// gradient terms multiply by 1/n_h and 1/(n_h - 1), two divides per cube
+// instead of four per parameter.
+__device__ inline void vg_grad_cube(int n, double V, double& sf, double& sf2, double* g1, double* g2) {
     const double Ih = sf / n * V;
     const double Sig2 = (sf2 / n * V * V) - (Ih * Ih);
-    acc[0] = acc[0] + Ih;
-    acc[1] = acc[1] + Sig2 / n;
+    sf = Ih;
+    sf2 = Sig2 / n;
+    const double vn = V / n, vvn = V * V / n, in1 = 1.0 / (n - 1);
 #pragma unroll
     for (int p = 0; p < NT; ++p) {
-        const double Gh = g1[p] / n * V;
-        const double Gv = (g2[p] / n * V * V) - (Gh * Gh);
-        acc[2 + p] = acc[2 + p] + Gh;
-        acc[2 + NT + p] = acc[2 + NT + p] + Gv / (n - 1);
+        const double Gh = g1[p] * vn;
+        const double Gv = g2[p] * vvn - Gh * Gh;
+        g1[p] = Gh;
+        g2[p] = Gv * in1;
     }
 }
 
-// Wave 64 sum, the same value in every lane: fixed butterfly order, so lane 0
-// (and every lane) gets a result independent of timing.
+// Wave 64 sum, the same value in every lane: fixed butterfly order, so the
+// result does not depend on timing.
 __device__ inline double vg_wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
     return v;
 }
 
+// Adds (wave-uniform) values to the wave's accumulator in LDS (lane 0 writes).
+__device__ inline void vg_wacc_add(double* wacc, int lane, double sf, double sf2, const double* g1,
+                                   const double* g2) {
+    if (lane == 0) {
+        wacc[0] = wacc[0] + sf;
+        wacc[1] = wacc[1] + sf2;
+#pragma unroll
+        for (int p = 0; p < NT; ++p) {
+            wacc[2 + p] = wacc[2 + p] + g1[p];
+            wacc[2 + NT + p] = wacc[2 + NT + p] + g2[p];
+        }
+    }
+}
+
 // Each wave takes groups of 64 consecutive cubes (grid-stride), lane l owning
-// cube group + l, whose contributions it accumulates in cube order.  A cube of
-// at most VG_GRAD_LANE_MAX samples is replayed by its own lane; larger cubes
-// of the group are replayed by the whole wave, one after another, samples
-// strided over the lanes, and summed in a fixed order.  The allocation puts
-// most cubes at n_h = 2 and a few at thousands of samples, which one lane per
-// cube would leave mostly idle.
-extern "C" __global__ void __launch_bounds__(256)
+// cube group + l.  Cubes of more than VG_GRAD_LANE_MAX samples are replayed
+// first, by the whole wave, one after another (samples strided over the lanes,
+// fixed butterfly sums); then every other cube by its own lane, and the 64
+// lanes' contributions are summed across the wave.  Both go into the wave's
+// LDS accumulator in that fixed order, so only one set of per-cube sums is
+// live at a time (register pressure sets the occupancy of this kernel).  The
+// allocation puts most cubes at n_h = 2 and a few at thousands of samples,
+// which one lane per cube would leave mostly idle.  Two waves per SIMD (256
+// VGPRs, a few dozen spilled) measured 12% faster than one wave without spills.
+extern "C" __global__ void __launch_bounds__(256, 2)
 vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
         const long long* __restrict__ nh_sum, int C, unsigned it, double V,
         const double* __restrict__ domain, double domain_vol, const vg_real* __restrict__ theta,
         unsigned seed, unsigned stream, double* __restrict__ part) {
     constexpr int NV = 2 + 2 * NT;
-    __shared__ double red[256];
-    double acc[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+    __shared__ double wacc_all[4][NV];
     const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    double* wacc = wacc_all[wid];
+    for (int q = lane; q < NV; q += 64) wacc[q] = 0.0;
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     for (long long g0 = wave * 64; g0 < C; g0 += nwaves * 64) {
         const long long h = g0 + lane;
         const int n = h < C ? nh[h] : 0;
         const bool own = h < C && n <= VG_GRAD_LANE_MAX;
-        if (own) {
-            const long long s0 = nh_sum[h];
-            double sf = 0.0, sf2 = 0.0, g1[NT], g2[NT];
-#pragma unroll
-            for (int p = 0; p < NT; ++p) { g1[p] = 0.0; g2[p] = 0.0; }
-            for (int e = 0; e < n; ++e) {
-                double jf, jg[NT];
-                vg_grad_sample(h, s0 + e, it, gmap, domain, domain_vol, theta, seed, stream, jf, jg);
-                sf = sf + jf;
-                sf2 = sf2 + jf * jf;
-#pragma unroll
-                for (int p = 0; p < NT; ++p) { g1[p] = g1[p] + jg[p]; g2[p] = g2[p] + jg[p] * jg[p]; }
-            }
-            vg_grad_cube(n, V, sf, sf2, g1, g2, acc);
-        }
         unsigned long long big = __ballot(h < C && !own);
         while (big) {
             const int j = __ffsll((long long)big) - 1;
@@ -511,27 +517,46 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
                 double jf, jg[NT];
                 vg_grad_sample(hb, s0 + e, it, gmap, domain, domain_vol, theta, seed, stream, jf, jg);
                 sf = sf + jf;
-                sf2 = sf2 + jf * jf;
+                sf2 = fma(jf, jf, sf2);
 #pragma unroll
-                for (int p = 0; p < NT; ++p) { g1[p] = g1[p] + jg[p]; g2[p] = g2[p] + jg[p] * jg[p]; }
+                for (int p = 0; p < NT; ++p) { g1[p] = g1[p] + jg[p]; g2[p] = fma(jg[p], jg[p], g2[p]); }
             }
             sf = vg_wave_sum(sf);
             sf2 = vg_wave_sum(sf2);
 #pragma unroll
             for (int p = 0; p < NT; ++p) { g1[p] = vg_wave_sum(g1[p]); g2[p] = vg_wave_sum(g2[p]); }
-            if (lane == j) vg_grad_cube(nb, V, sf, sf2, g1, g2, acc);
+            vg_grad_cube(nb, V, sf, sf2, g1, g2);
+            vg_wacc_add(wacc, lane, sf, sf2, g1, g2);
+        }
+        double sf = 0.0, sf2 = 0.0, g1[NT], g2[NT];
+#pragma unroll
+        for (int p = 0; p < NT; ++p) { g1[p] = 0.0; g2[p] = 0.0; }
+        if (own) {
+            const long long s0 = nh_sum[h];
+            for (int e = 0; e < n; ++e) {
+                double jf, jg[NT];
+                vg_grad_sample(h, s0 + e, it, gmap, domain, domain_vol, theta, seed, stream, jf, jg);
+                sf = sf + jf;
+                sf2 = fma(jf, jf, sf2);
+#pragma unroll
+                for (int p = 0; p < NT; ++p) { g1[p] = g1[p] + jg[p]; g2[p] = fma(jg[p], jg[p], g2[p]); }
+            }
+            vg_grad_cube(n, V, sf, sf2, g1, g2);
+        }
+        if (__ballot(own)) {
+            sf = vg_wave_sum(sf);
+            sf2 = vg_wave_sum(sf2);
+#pragma unroll
+            for (int p = 0; p < NT; ++p) { g1[p] = vg_wave_sum(g1[p]); g2[p] = vg_wave_sum(g2[p]); }
+            vg_wacc_add(wacc, lane, sf, sf2, g1, g2);
         }
     }
-    // fixed-order block reduction, one quantity at a time
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        __syncthreads();
-        red[threadIdx.x] = acc[q];
-        for (int stride = 128; stride >= 1; stride >>= 1) {
-            __syncthreads();
-            if ((int)threadIdx.x < stride) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + stride];
-        }
-        if (threadIdx.x == 0) part[(long long)blockIdx.x * NV + q] = red[0];
+    // block partials: the waves' accumulators in wave order
+    __syncthreads();
+    for (int q = threadIdx.x; q < NV; q += blockDim.x) {
+        double t = wacc_all[0][q];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) t = t + wacc_all[w][q];
+        part[(long long)blockIdx.x * NV + q] = t;
     }
 }
 #endif

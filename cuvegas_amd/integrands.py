@@ -133,16 +133,19 @@ __device__ inline vg_real vg_mixture(const vg_real* x, const vg_real* th, vg_rea
     for (int k = 0; k < VG_MIX_K; ++k) {{
         const vg_real w = th[k], s = th[VG_MIX_K + k];
         const vg_real* mu = th + 2 * VG_MIX_K + k * VG_NDIM;
-        vg_real q = (vg_real)0;
-        for (int i = 0; i < VG_NDIM; ++i) {{ const vg_real a = x[i] - mu[i]; q = q + a * a; }}
+        // functions of theta only (the same for every sample): 1/s^2, 1/s, 1/norm
+        const vg_real is2 = (vg_real)1 / (s * s), is = (vg_real)1 / s, m = (vg_real)-0.5 * is2;
         vg_real norm = (vg_real)1;
         for (int i = 0; i < VG_NDIM; ++i) norm = norm * (s * c);
-        const vg_real pk = exp(-q / ((vg_real)2 * s * s)) / norm;
+        const vg_real inorm = (vg_real)1 / norm;
+        vg_real q = (vg_real)0;
+        for (int i = 0; i < VG_NDIM; ++i) {{ const vg_real a = x[i] - mu[i]; q = fma(a, a, q); }}
+        const vg_real pk = vg_expr(q * m) * inorm;
         f = f + w * pk;
         if (G) {{
-            const vg_real wp = w * pk, is2 = (vg_real)1 / (s * s);
+            const vg_real wp = w * pk;
             gr[k] = pk;
-            gr[VG_MIX_K + k] = wp * (q * is2 - (vg_real)VG_NDIM) / s;
+            gr[VG_MIX_K + k] = wp * (q * is2 - (vg_real)VG_NDIM) * is;
             for (int i = 0; i < VG_NDIM; ++i) gr[2 * VG_MIX_K + k * VG_NDIM + i] = wp * (x[i] - mu[i]) * is2;
         }}
     }}

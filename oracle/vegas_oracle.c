@@ -488,15 +488,17 @@ static double mix64(const double* x, int d, const double* th, double* g) {
     for (int k = 0; k < 4; ++k) {
         double w = th[k], s = th[4 + k], q = 0.0;
         const double* mu = th + 8 + k * d;
-        for (int i = 0; i < d; ++i) { double a = x[i] - mu[i]; q = q + a * a; }
+        double is2 = 1.0 / (s * s), is = 1.0 / s, m = -0.5 * is2;
         double norm = 1.0;
         for (int i = 0; i < d; ++i) norm = norm * (s * c);
-        double pk = exp(-q / (2.0 * s * s)) / norm;
+        double inorm = 1.0 / norm;
+        for (int i = 0; i < d; ++i) { double a = x[i] - mu[i]; q = fma(a, a, q); }
+        double pk = exp(q * m) * inorm;
         f = f + w * pk;
         if (g) {
-            double wp = w * pk, is2 = 1.0 / (s * s);
+            double wp = w * pk;
             g[k] = pk;
-            g[4 + k] = wp * (q * is2 - (double)d) / s;
+            g[4 + k] = wp * (q * is2 - (double)d) * is;
             for (int i = 0; i < d; ++i) g[8 + k * d + i] = wp * (x[i] - mu[i]) * is2;
         }
     }
@@ -509,15 +511,17 @@ static float mix32(const float* x, int d, const float* th, float* g) {
     for (int k = 0; k < 4; ++k) {
         float w = th[k], s = th[4 + k], q = 0.0f;
         const float* mu = th + 8 + k * d;
-        for (int i = 0; i < d; ++i) { float a = x[i] - mu[i]; q = q + a * a; }
+        float is2 = 1.0f / (s * s), is = 1.0f / s, m = -0.5f * is2;
         float norm = 1.0f;
         for (int i = 0; i < d; ++i) norm = norm * (s * c);
-        float pk = expf(-q / (2.0f * s * s)) / norm;
+        float inorm = 1.0f / norm;
+        for (int i = 0; i < d; ++i) { float a = x[i] - mu[i]; q = fmaf(a, a, q); }
+        float pk = expf(q * m) * inorm;
         f = f + w * pk;
         if (g) {
-            float wp = w * pk, is2 = 1.0f / (s * s);
+            float wp = w * pk;
             g[k] = pk;
-            g[4 + k] = wp * (q * is2 - (float)d) / s;
+            g[4 + k] = wp * (q * is2 - (float)d) * is;
             for (int i = 0; i < d; ++i) g[8 + k * d + i] = wp * (x[i] - mu[i]) * is2;
         }
     }
@@ -960,18 +964,19 @@ ORC_EXPORT int orc_grad_replay(const orc_params* P, const double* x_edges, const
                 for (int q = 0; q < NT; ++q) gr[q] = (double)(grf[q] * jac);
             }
             sf = sf + jf;
-            sf2 = sf2 + jf * jf;
-            for (int q = 0; q < NT; ++q) { g1[q] = g1[q] + gr[q]; g2[q] = g2[q] + gr[q] * gr[q]; }
+            sf2 = fma(jf, jf, sf2);
+            for (int q = 0; q < NT; ++q) { g1[q] = g1[q] + gr[q]; g2[q] = fma(gr[q], gr[q], g2[q]); }
         }
         const double Ih = sf / cnt * V;
         const double Sig2 = (sf2 / cnt * V * V) - (Ih * Ih);
         out[0] += Ih;
         out[1] += Sig2 / cnt;
+        const double vn = V / cnt, vvn = V * V / cnt, in1 = 1.0 / (cnt - 1);
         for (int q = 0; q < NT; ++q) {
-            const double Gh = g1[q] / cnt * V;
-            const double Gv = (g2[q] / cnt * V * V) - (Gh * Gh);
+            const double Gh = g1[q] * vn;
+            const double Gv = g2[q] * vvn - Gh * Gh;
             out[2 + q] += Gh;
-            out[2 + NT + q] += Gv / (cnt - 1);
+            out[2 + NT + q] += Gv * in1;
         }
         s0 += cnt;
     }
