@@ -379,7 +379,13 @@ vg_dump(const vg_map_t* __restrict__ gmap, const long long* __restrict__ nh_sum,
 // ---------------------------------------------------------------------------
 constexpr int NT = VG_NTHETA;
 // Cubes with more samples than this are replayed by the whole wave.
-constexpr int VG_GRAD_LANE_MAX = 16;
+// Measured on the 6-D mixture (1e6 cubes, n_h from 2 to ~3,000): 16 -> 8.8 ms,
+// 32 -> 7.4, 128 -> 6.0, 256 -> 5.8, 512 -> 6.9, no wave path -> 18 ms (FP32
+// backward); each wave-replayed cube pays 66 wave sums.
+constexpr int VG_GRAD_LANE_MAX = 128;
+// FP32 tables of up to 64 KB are staged in LDS (two 256-thread blocks per CU
+// still fit); FP64 tables are read through L1/L2.
+constexpr bool kGradLdsMap = VG_FP32 && (long long)D * NB * 8 <= 65536;
 
 // J f and J df/dtheta (FP64) of sample s of cube h under the frozen map.
 __device__ inline void vg_grad_sample(long long h, long long s, unsigned it, const vg_map_t* __restrict__ gmap,
@@ -493,6 +499,13 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
         unsigned seed, unsigned stream, double* __restrict__ part) {
     constexpr int NV = 2 + 2 * NT;
     __shared__ double wacc_all[4][NV];
+    __shared__ vg_map_t lmap[kGradLdsMap ? D * NB : 1];
+    const vg_map_t* map = gmap;
+    if (kGradLdsMap) {
+        for (int i = threadIdx.x; i < D * NB; i += blockDim.x) lmap[i] = gmap[i];
+        __syncthreads();
+        map = lmap;
+    }
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     double* wacc = wacc_all[wid];
@@ -515,7 +528,7 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
             for (int p = 0; p < NT; ++p) { g1[p] = 0.0; g2[p] = 0.0; }
             for (int e = lane; e < nb; e += 64) {
                 double jf, jg[NT];
-                vg_grad_sample(hb, s0 + e, it, gmap, domain, domain_vol, theta, seed, stream, jf, jg);
+                vg_grad_sample(hb, s0 + e, it, map, domain, domain_vol, theta, seed, stream, jf, jg);
                 sf = sf + jf;
                 sf2 = fma(jf, jf, sf2);
 #pragma unroll
@@ -535,7 +548,7 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
             const long long s0 = nh_sum[h];
             for (int e = 0; e < n; ++e) {
                 double jf, jg[NT];
-                vg_grad_sample(h, s0 + e, it, gmap, domain, domain_vol, theta, seed, stream, jf, jg);
+                vg_grad_sample(h, s0 + e, it, map, domain, domain_vol, theta, seed, stream, jf, jg);
                 sf = sf + jf;
                 sf2 = fma(jf, jf, sf2);
 #pragma unroll
