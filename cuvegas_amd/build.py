@@ -1,7 +1,7 @@
 """Build libcuvegas_amd.so in-tree with hipcc for gfx950.
 
 Steps:
-  1. embed vg_fill.hip and vg_philox.h as C string literals (csrc/vg_embed.inc),
+  1. embed vg_fill.hip, vg_philox.h and vg_math.h as C string literals (csrc/vg_embed.inc),
      so the library carries the sampling-kernel source it hands to hipRTC;
   2. compile the ahead-of-time step kernels (vg_steps.hip) and the host driver
      (vg_host.cpp);
@@ -59,7 +59,8 @@ def _run(cmd):
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     _embed([("kVgFillSrc", os.path.join(CSRC, "vg_fill.hip")),
-            ("kVgPhiloxSrc", os.path.join(CSRC, "vg_philox.h"))],
+            ("kVgPhiloxSrc", os.path.join(CSRC, "vg_philox.h")),
+            ("kVgMathSrc", os.path.join(CSRC, "vg_math.h"))],
            os.path.join(CSRC, "vg_embed.inc"))
     steps_o = os.path.join(BUILD, "vg_steps.o")
     host_o = os.path.join(BUILD, "vg_host.o")

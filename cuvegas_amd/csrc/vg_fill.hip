@@ -21,6 +21,7 @@
 // evaluation with FP64 accumulation), VG_LDS_BINS, VG_LDS_MAP, VG_HAS_DOMAIN,
 // VG_PAIR (two samples per lane step).
 #include "vg_philox.h"
+#include "vg_math.h"
 
 #ifndef VG_PAIR
 #define VG_PAIR 1
@@ -32,13 +33,16 @@ typedef float vg_real;
 typedef double vg_real;
 #endif
 
-// exp in the working precision, for integrands.  FP64: the library exp.
+// exp in the working precision, for integrands.  FP64: vg_exp, the project's
+// Cody-Waite + Taylor exp that the CPU oracle restates step for step, so FP64
+// integrand values (and with them d_h and the integer allocation) are the same
+// on the GPU and the CPU.
 // FP32: e^x = 2^(x log2 e) with the product carried as hi + lo (the fma gives
 // the rounding error of hi exactly, plus x times the tail of log2 e), then
 // 2^(hi + lo) = 2^hi (1 + lo ln 2) to second order in lo; v_exp_f32 gives 2^hi
 // to 1 ulp, so the result is within about 2 ulp.  The argument is clamped to
 // [-150, 88.7]: below, e^x underflows to 0; 88.72 is the overflow threshold.
-__device__ inline double vg_expr(double x) { return exp(x); }
+__device__ inline double vg_expr(double x) { return vg_exp(x); }
 __device__ inline float vg_expr(float x) {
     x = __builtin_amdgcn_fmed3f(x, -150.0f, 88.7f);
     const float l2e_hi = 0x1.715476p+0f, l2e_lo = 0x1.4ae0c0p-26f;

@@ -21,7 +21,7 @@
 #include "../../include/cuvegas.h"
 #include "vg_steps.h"
 
-#include "vg_embed.inc"  // kVgFillSrc, kVgPhiloxSrc (generated from vg_fill.hip, vg_philox.h)
+#include "vg_embed.inc"  // kVgFillSrc, kVgPhiloxSrc, kVgMathSrc (generated from vg_fill.hip, vg_philox.h, vg_math.h)
 
 namespace {
 
@@ -391,9 +391,9 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
     options = rtc_extra.c_str();
     const double t0 = now_ms();
     hiprtcProgram rtc;
-    const char* hdr_src[2] = {kVgPhiloxSrc, integrand_src};
-    const char* hdr_name[2] = {"vg_philox.h", "vg_user.h"};
-    if (hiprtcCreateProgram(&rtc, kVgFillSrc, "vg_fill.hip", 2, hdr_src, hdr_name) != HIPRTC_SUCCESS)
+    const char* hdr_src[3] = {kVgPhiloxSrc, kVgMathSrc, integrand_src};
+    const char* hdr_name[3] = {"vg_philox.h", "vg_math.h", "vg_user.h"};
+    if (hiprtcCreateProgram(&rtc, kVgFillSrc, "vg_fill.hip", 3, hdr_src, hdr_name) != HIPRTC_SUCCESS)
         return fail(CVG_ERR_COMPILE, "hiprtcCreateProgram failed");
     std::vector<std::string> opts = {
         "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
@@ -1242,9 +1242,9 @@ int cvg_batch_program_create(const char* integrand_src, int max_dim, const cvg_p
     prog->block = block;
     const double t0 = now_ms();
     hiprtcProgram rtc;
-    const char* hdr_src[2] = {kVgPhiloxSrc, integrand_src};
-    const char* hdr_name[2] = {"vg_philox.h", "vg_user.h"};
-    if (hiprtcCreateProgram(&rtc, kVgFillSrc, "vg_fill.hip", 2, hdr_src, hdr_name) != HIPRTC_SUCCESS)
+    const char* hdr_src[3] = {kVgPhiloxSrc, kVgMathSrc, integrand_src};
+    const char* hdr_name[3] = {"vg_philox.h", "vg_math.h", "vg_user.h"};
+    if (hiprtcCreateProgram(&rtc, kVgFillSrc, "vg_fill.hip", 3, hdr_src, hdr_name) != HIPRTC_SUCCESS)
         return fail(CVG_ERR_COMPILE, "hiprtcCreateProgram failed");
     std::vector<std::string> opts = {
         "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",

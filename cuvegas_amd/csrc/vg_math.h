@@ -31,9 +31,6 @@
 VG_HD double vg_from_bits(unsigned long long b) { return __builtin_bit_cast(double, b); }
 VG_HD unsigned long long vg_to_bits(double x) { return __builtin_bit_cast(unsigned long long, x); }
 
-// 2^e as a double for -1022 <= e <= 1023.
-VG_HD double vg_pow2i(int e) { return vg_from_bits((unsigned long long)(e + 1023) << 52); }
-
 VG_HD double vg_log(double x) {
     if (x != x) return x;
     if (x < 0.0) return vg_from_bits(0x7ff8000000000000ull);
@@ -64,13 +61,15 @@ VG_HD double vg_log(double x) {
     return fma(de, VG_LN2_HI, fma(de, VG_LN2_LO, logm));
 }
 
+// The argument is clamped to [-746, 710], where 2^k still lands in ldexp's
+// underflow-to-0 / overflow-to-inf range, and the result is scaled by one
+// ldexp (exact in the normal range, one rounding below it): no branch but the
+// NaN test.
 VG_HD double vg_exp(double y) {
     if (y != y) return y;
-    if (y > 709.782712893384) return vg_from_bits(0x7ff0000000000000ull);
-    if (y < -745.1332191019412) return 0.0;
-    const double kd = floor(fma(y, VG_INV_LN2, 0.5));
-    const int k = (int)kd;
-    const double r = fma(-kd, VG_LN2_LO, fma(-kd, VG_LN2_HI, y));
+    const double yc = y < -746.0 ? -746.0 : (y > 710.0 ? 710.0 : y);
+    const double kd = floor(fma(yc, VG_INV_LN2, 0.5));
+    const double r = fma(-kd, VG_LN2_LO, fma(-kd, VG_LN2_HI, yc));
     double p = 1.0 / 6227020800.0;            // 1/13!
     p = fma(p, r, 1.0 / 479001600.0);         // 1/12!
     p = fma(p, r, 1.0 / 39916800.0);
@@ -85,9 +84,7 @@ VG_HD double vg_exp(double y) {
     p = fma(p, r, 0.5);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
-    if (k > 1023) return (p * vg_pow2i(1023)) * vg_pow2i(k - 1023);
-    if (k < -1022) return (p * vg_pow2i(k + 1000)) * vg_pow2i(-1000);
-    return p * vg_pow2i(k);
+    return ldexp(p, (int)kd);
 }
 
 // x^a for x >= 0 (the only case the integrator needs).

@@ -79,7 +79,6 @@ static const double SQRT2 = 1.41421356237309514547e+00;
 
 static double from_bits(uint64_t b) { double x; memcpy(&x, &b, 8); return x; }
 static uint64_t to_bits(double x) { uint64_t b; memcpy(&b, &x, 8); return b; }
-static double pow2i(int e) { return from_bits((uint64_t)(e + 1023) << 52); }
 
 ORC_EXPORT double orc_log(double x) {
     if (x != x) return x;
@@ -113,11 +112,9 @@ ORC_EXPORT double orc_log(double x) {
 
 ORC_EXPORT double orc_exp(double y) {
     if (y != y) return y;
-    if (y > 709.782712893384) return INFINITY;
-    if (y < -745.1332191019412) return 0.0;
-    double kd = floor(fma(y, INV_LN2, 0.5));
-    int k = (int)kd;
-    double r = fma(-kd, LN2_LO, fma(-kd, LN2_HI, y));
+    double yc = y < -746.0 ? -746.0 : (y > 710.0 ? 710.0 : y);  /* 2^k then under/overflows in ldexp */
+    double kd = floor(fma(yc, INV_LN2, 0.5));
+    double r = fma(-kd, LN2_LO, fma(-kd, LN2_HI, yc));
     double p = 1.0 / 6227020800.0;
     p = fma(p, r, 1.0 / 479001600.0);
     p = fma(p, r, 1.0 / 39916800.0);
@@ -132,9 +129,7 @@ ORC_EXPORT double orc_exp(double y) {
     p = fma(p, r, 0.5);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
-    if (k > 1023) return (p * pow2i(1023)) * pow2i(k - 1023);
-    if (k < -1022) return (p * pow2i(k + 1000)) * pow2i(-1000);
-    return p * pow2i(k);
+    return ldexp(p, (int)kd);
 }
 
 ORC_EXPORT double orc_pow(double x, double a) {
@@ -493,7 +488,7 @@ static double mix64(const double* x, int d, const double* th, double* g) {
         for (int i = 0; i < d; ++i) norm = norm * (s * c);
         double inorm = 1.0 / norm;
         for (int i = 0; i < d; ++i) { double a = x[i] - mu[i]; q = fma(a, a, q); }
-        double pk = exp(q * m) * inorm;
+        double pk = orc_exp(q * m) * inorm;
         f = f + w * pk;
         if (g) {
             double wp = w * pk;
@@ -536,7 +531,7 @@ static double f64_eval(int id, const double* x, int d, const double* p) {
     case ORC_F_GAUSS: { /* p = {mu, sigma, norm, m}: norm * exp(m * sum (x-mu)^2), m = -1/(2 sigma^2) */
         double q = 0.0;
         for (int i = 0; i < d; ++i) { double a = x[i] - p[0]; q = fma(a, a, q); }
-        return p[2] * exp(q * p[3]);
+        return p[2] * orc_exp(q * p[3]);
     }
     case ORC_F_DGAUSS: { /* p = {c1, c2, sigma, norm, m} */
         double q1 = 0.0, q2 = 0.0;
@@ -544,7 +539,7 @@ static double f64_eval(int id, const double* x, int d, const double* p) {
             double a = x[i] - p[0], b = x[i] - p[1];
             q1 = fma(a, a, q1); q2 = fma(b, b, q2);
         }
-        return 0.5 * p[3] * (exp(q1 * p[4]) + exp(q2 * p[4]));
+        return 0.5 * p[3] * (orc_exp(q1 * p[4]) + orc_exp(q2 * p[4]));
     }
     case ORC_F_OSC: { /* p = {u, a_0..a_{d-1}} */
         double t = 2.0 * ORC_PI * p[0];
