@@ -4,7 +4,8 @@
 //
 // One launch evaluates every sample of one iteration (reference vegasFill,
 // cuda_kernel.cu:14-76) and produces
-//   * per-cube sums of J f and (J f)^2 in a fixed order, without atomics:
+//   * per-cube sums of J f and (J f)^2 in a fixed order, without atomics,
+//     stored interleaved (one 16-byte store per cube):
 //       level 0  each lane walks a run of VG_K consecutive samples (the
 //                sample -> cube map comes from the nh prefix sums, so there is
 //                no evals[] array) and sums each cube's samples in order;
@@ -184,7 +185,7 @@ vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
         const int* __restrict__ run_cube,       // cube holding the first sample of each run
         const long long* __restrict__ s_total,  // samples this iteration
         const int* __restrict__ it_p,           // iteration counter
-        double* __restrict__ JF, double* __restrict__ JF2,
+        double2* __restrict__ JS,                 // [C] per-cube (sum J f, sum (J f)^2)
         double2* __restrict__ win_first, double2* __restrict__ win_last,
         double* __restrict__ weights, unsigned* __restrict__ counts,
         const double* __restrict__ domain, double domain_vol,
@@ -234,7 +235,7 @@ vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
             if (npieces == 0 && sb) {
                 hasF = true; hF = h; f1 = a1; f2 = a2;
             } else {
-                JF[h] = a1; JF2[h] = a2;
+                JS[h] = make_double2(a1, a2);
             }
             ++npieces;
             ++h;
@@ -313,11 +314,11 @@ vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
                 double t1 = f1, t2 = f2;
                 if (lane > 0) { t1 = q1 + t1; t2 = q2 + t2; }
                 if (startF < wstart) win_first[w] = make_double2(t1, t2);
-                else { JF[hF] = t1; JF2[hF] = t2; }
+                else JS[hF] = make_double2(t1, t2);
             }
             if (!cont) {
                 if (cstart < wstart) win_first[w] = make_double2(s1, s2);
-                else { JF[h] = s1; JF2[h] = s2; }
+                else JS[h] = make_double2(s1, s2);
             } else if (lane == 63) {
                 win_last[w] = make_double2(s1, s2);
                 if (cstart < wstart) win_first[w] = make_double2(s1, s2);

@@ -160,8 +160,7 @@ struct cvg_state {
     int* nh = nullptr;
     long long* nh_sum = nullptr;
     double* dh = nullptr;
-    double* JF = nullptr;
-    double* JF2 = nullptr;
+    double2* JS = nullptr;  // per-cube (sum J f, sum (J f)^2)
     int* run_cube = nullptr;
     double2* win_first = nullptr;
     double2* win_last = nullptr;
@@ -212,7 +211,7 @@ struct cvg_state {
         if (ev1) (void)hipEventDestroy(ev1);
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         void* bufs[] = {x_edges, map64, map32, weights, counts, refine_scratch, nh, nh_sum, dh,
-                        JF, JF2, run_cube, win_first, win_last, part, bsum, boff, sync, s_total,
+                        JS, run_cube, win_first, win_last, part, bsum, boff, sync, s_total,
                         it_p, dsum, err, hist, domain, theta, fold_list, fold_count, bw_fold_list,
                         bw_fold_count, snap_x, snap_nh, bw_nh_sum,
                         bw_s_total, bw_x, bw_map64, bw_map32, bw_part};
@@ -235,7 +234,7 @@ struct cvg_state {
             for (int i = 0; i < p.n_dim; ++i) vol *= domain_host[p.n_dim + i];
         unsigned seed = p.seed, sid = p.stream_id;
         void* args[] = {(void*)&map,       (void*)&nh_sum,  (void*)&run_cube, (void*)&s_total,
-                        (void*)&it_p,      (void*)&JF,      (void*)&JF2,      (void*)&win_first,
+                        (void*)&it_p,      (void*)&JS,      (void*)&win_first,
                         (void*)&win_last,  (void*)&weights, (void*)&counts,   (void*)&domain,
                         (void*)&vol,       (void*)&theta,   (void*)&seed,     (void*)&sid};
         VG_HIP(hipModuleLaunchKernel(prog->fill, grid, 1, 1, (unsigned)prog->block, 1, 1, 0, stream,
@@ -251,8 +250,8 @@ struct cvg_state {
 
     int launch_results() {
         VG_HIP(vg_launch_bigfold(stream, fold_list, fold_count, fold_cap, nh_sum, run_cube, s_total,
-                                 win_first, win_last, 6 + k_shift, JF, JF2));
-        VG_HIP(vg_launch_results(stream, (int)C, V, p.beta, nh, JF, JF2, dh, part));
+                                 win_first, win_last, 6 + k_shift, JS));
+        VG_HIP(vg_launch_results(stream, (int)C, V, p.beta, nh, JS, dh, part));
         VG_HIP(vg_launch_finalize(stream, nb, part, sync, s_total, it_p, p.max_it, hist, dsum, err));
         return CVG_OK;
     }
@@ -513,8 +512,7 @@ int cvg_state_create(cvg_program* prog, const cvg_params* params, cvg_state** ou
     if ((rc = dalloc(&st->nh, C))) return rc;
     if ((rc = dalloc(&st->nh_sum, C + 1))) return rc;
     if ((rc = dalloc(&st->dh, C))) return rc;
-    if ((rc = dalloc(&st->JF, C))) return rc;
-    if ((rc = dalloc(&st->JF2, C))) return rc;
+    if ((rc = dalloc(&st->JS, C))) return rc;
     if ((rc = dalloc(&st->run_cube, st->run_cap))) return rc;
     if ((rc = dalloc(&st->win_first, st->win_cap))) return rc;
     if ((rc = dalloc(&st->win_last, st->win_cap))) return rc;
@@ -785,9 +783,11 @@ int cvg_state_get_dh(cvg_state* st, double* dh) {
 
 int cvg_state_get_cube_sums(cvg_state* st, double* jf, double* jf2) {
     if (!st || !jf || !jf2) return fail(CVG_ERR_INVALID, "NULL argument");
-    int rc = copy_d2h(st, jf, st->JF, sizeof(double) * st->C);
+    std::vector<double2> js((size_t)st->C);
+    int rc = copy_d2h(st, js.data(), st->JS, sizeof(double2) * st->C);
     if (rc) return rc;
-    return copy_d2h(st, jf2, st->JF2, sizeof(double) * st->C);
+    for (long long h = 0; h < st->C; ++h) { jf[h] = js[h].x; jf2[h] = js[h].y; }
+    return CVG_OK;
 }
 
 int cvg_state_get_bins(cvg_state* st, double* weights, unsigned int* counts) {

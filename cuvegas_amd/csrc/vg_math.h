@@ -13,7 +13,8 @@
 //     split into a 32-bit head and a tail (Cody & Waite, "Software Manual for
 //     the Elementary Functions", 1980).
 //   * exp: y = k ln2 + r with the same split, |r| <= ln2/2, exp(r) by its
-//     Taylor series to r^13/13! (truncation < 5e-18), then scaled by 2^k.
+//     Taylor series to r^13/13! (truncation < 5e-18, Estrin's evaluation
+//     order), then scaled by 2^k.
 //   * pow(x, a) = exp(a log x) for x >= 0.
 // Relative error is a few ulp for log/exp and below 1e-13 for pow over the
 // range used; the oracle restates the same steps, so results agree bit for bit.
@@ -70,17 +71,19 @@ VG_HD double vg_exp(double y) {
     const double yc = y < -746.0 ? -746.0 : (y > 710.0 ? 710.0 : y);
     const double kd = floor(fma(yc, VG_INV_LN2, 0.5));
     const double r = fma(-kd, VG_LN2_LO, fma(-kd, VG_LN2_HI, yc));
-    double p = 1.0 / 6227020800.0;            // 1/13!
-    p = fma(p, r, 1.0 / 479001600.0);         // 1/12!
-    p = fma(p, r, 1.0 / 39916800.0);
-    p = fma(p, r, 1.0 / 3628800.0);
-    p = fma(p, r, 1.0 / 362880.0);
-    p = fma(p, r, 1.0 / 40320.0);
-    p = fma(p, r, 1.0 / 5040.0);
-    p = fma(p, r, 1.0 / 720.0);
-    p = fma(p, r, 1.0 / 120.0);
-    p = fma(p, r, 1.0 / 24.0);
-    p = fma(p, r, 1.0 / 6.0);
+    // Taylor polynomial sum_{i<=13} r^i / i!: the r^4..r^13 tail by Estrin's
+    // scheme (pairs, then r^2, r^4 levels), the four leading terms by Horner
+    // steps last, where the rounding of the large terms matters most.
+    const double r2 = r * r, r4 = r2 * r2;
+    const double q0 = fma(r, 1.0 / 120.0, 1.0 / 24.0);
+    const double q1 = fma(r, 1.0 / 5040.0, 1.0 / 720.0);
+    const double q2 = fma(r, 1.0 / 362880.0, 1.0 / 40320.0);
+    const double q3 = fma(r, 1.0 / 39916800.0, 1.0 / 3628800.0);
+    const double q4 = fma(r, 1.0 / 6227020800.0, 1.0 / 479001600.0);
+    const double s0 = fma(q1, r2, q0);
+    const double s1 = fma(q3, r2, q2);
+    const double t = fma(fma(q4, r4, s1), r4, s0);
+    double p = fma(t, r, 1.0 / 6.0);
     p = fma(p, r, 0.5);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);

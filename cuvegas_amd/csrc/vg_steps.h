@@ -8,7 +8,7 @@
 // Per-cube results from complete per-cube sums: vg_launch_bigfold must have
 // folded the window partials of every cube that crosses a sampling window.
 hipError_t vg_launch_results(hipStream_t st, int C, double V, double beta, const int* nh,
-                             const double* JF, const double* JF2, double* dh, double* part);
+                             const double2* JS, double* dh, double* part);
 // part holds 3 * (nparts + ceil(nparts / 1024)) doubles; sync is one zeroed
 // counter that the launch leaves zeroed.
 hipError_t vg_launch_finalize(hipStream_t st, long long nparts, double* part, unsigned* sync,
@@ -25,7 +25,7 @@ hipError_t vg_launch_scan(hipStream_t st, int C, const int* nh, long long* bsum,
 hipError_t vg_launch_bigfold(hipStream_t st, const int* list, const int* count, int cap,
                              const long long* nh_sum, const int* run_cube, const long long* s_total,
                              const double2* win_first, const double2* win_last, int ws_shift,
-                             double* JF, double* JF2);
+                             double2* JS);
 hipError_t vg_launch_init_map(hipStream_t st, int d, int n, double* x_edges, double2* map64,
                               float2* map32, double* weights, unsigned* counts);
 hipError_t vg_launch_refine(hipStream_t st, int d, int n, double alpha, double* x_edges,

@@ -153,13 +153,13 @@ __device__ inline void vg_cube_result(long long h, int C, double V, double beta,
 
 // Results of cube h from complete per-cube sums (the window folds ran before).
 __device__ inline void vg_cube_result_sums(long long h, int C, double V, double beta,
-                                           const int* __restrict__ nh, const double* __restrict__ JF,
-                                           const double* __restrict__ JF2, double* __restrict__ dh,
-                                           double& sr, double& ss, double& dd) {
+                                           const int* __restrict__ nh, const double2* __restrict__ JS,
+                                           double* __restrict__ dh, double& sr, double& ss, double& dd) {
     sr = 0.0; ss = 0.0; dd = 0.0;
     if (h >= C) return;
     const int neval = nh[h];
-    const double j1 = JF[h], j2 = JF2[h];
+    const double2 js = JS[h];
+    const double j1 = js.x, j2 = js.y;
     const double Ih = j1 / neval * V;
     const double Sig2 = (j2 / neval * V * V) - (Ih * Ih);
     sr = Ih;
@@ -173,14 +173,13 @@ __device__ inline void vg_cube_result_sums(long long h, int C, double V, double 
 
 __global__ void __launch_bounds__(256)
 vg_results_kernel(int C, double V, double beta, const int* __restrict__ nh,
-                  const double* __restrict__ JF, const double* __restrict__ JF2,
-                  double* __restrict__ dh, double* __restrict__ part) {
+                  const double2* __restrict__ JS, double* __restrict__ dh, double* __restrict__ part) {
     __shared__ double red[256];
     const long long base = (long long)blockIdx.x * VG_TREE + threadIdx.x;
     double r[4][3];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-        vg_cube_result_sums(base + 256 * j, C, V, beta, nh, JF, JF2, dh, r[j][0], r[j][1], r[j][2]);
+        vg_cube_result_sums(base + 256 * j, C, V, beta, nh, JS, dh, r[j][0], r[j][1], r[j][2]);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const double tot = vg_tree1024_256(r[0][q], r[1][q], r[2][q], r[3][q], red);
@@ -236,8 +235,7 @@ __global__ void __launch_bounds__(256)
 vg_bigfold_kernel(const int* __restrict__ list, const int* __restrict__ count, int cap,
                   const long long* __restrict__ nh_sum, const int* __restrict__ run_cube,
                   const long long* __restrict__ s_total, const double2* __restrict__ win_first,
-                  const double2* __restrict__ win_last, int ws_shift, double* __restrict__ JF,
-                  double* __restrict__ JF2) {
+                  const double2* __restrict__ win_last, int ws_shift, double2* __restrict__ JS) {
     __shared__ double red[256];
     __shared__ double part[2 * 2048];
     if (blockIdx.x >= VG_BIGFOLD_BLOCKS) {
@@ -257,8 +255,7 @@ vg_bigfold_kernel(const int* __restrict__ list, const int* __restrict__ count, i
                 j1 = j1 + v.x;
                 j2 = j2 + v.y;
             }
-            JF[h] = j1;
-            JF2[h] = j2;
+            JS[h] = make_double2(j1, j2);
         }
         return;
     }
@@ -269,16 +266,16 @@ vg_bigfold_kernel(const int* __restrict__ list, const int* __restrict__ count, i
         const long long wa = st >> ws_shift, wb = (en - 1) >> ws_shift;
         double o1, o2;
         vg_fold_tree(wa, wb - wa + 1, win_first, win_last, part, red, o1, o2);
-        if (threadIdx.x == 0) { JF[h] = o1; JF2[h] = o2; }
+        if (threadIdx.x == 0) JS[h] = make_double2(o1, o2);
     }
 }
 
 hipError_t vg_launch_bigfold(hipStream_t st, const int* list, const int* count, int cap,
                              const long long* nh_sum, const int* run_cube, const long long* s_total,
                              const double2* win_first, const double2* win_last, int ws_shift,
-                             double* JF, double* JF2) {
+                             double2* JS) {
     hipLaunchKernelGGL(vg_bigfold_kernel, dim3(VG_BIGFOLD_BLOCKS + 256), dim3(256), 0, st, list, count,
-                       cap, nh_sum, run_cube, s_total, win_first, win_last, ws_shift, JF, JF2);
+                       cap, nh_sum, run_cube, s_total, win_first, win_last, ws_shift, JS);
     return hipGetLastError();
 }
 
@@ -648,9 +645,9 @@ long long vg_refine_scratch_doubles(int d, int n) {
 // ---------------------------------------------------------------------------
 
 hipError_t vg_launch_results(hipStream_t st, int C, double V, double beta, const int* nh,
-                             const double* JF, const double* JF2, double* dh, double* part) {
+                             const double2* JS, double* dh, double* part) {
     const unsigned nb = (unsigned)((C + VG_TREE - 1) / VG_TREE);
-    hipLaunchKernelGGL(vg_results_kernel, dim3(nb), dim3(256), 0, st, C, V, beta, nh, JF, JF2, dh, part);
+    hipLaunchKernelGGL(vg_results_kernel, dim3(nb), dim3(256), 0, st, C, V, beta, nh, JS, dh, part);
     return hipGetLastError();
 }
 

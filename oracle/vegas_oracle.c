@@ -115,17 +115,17 @@ ORC_EXPORT double orc_exp(double y) {
     double yc = y < -746.0 ? -746.0 : (y > 710.0 ? 710.0 : y);  /* 2^k then under/overflows in ldexp */
     double kd = floor(fma(yc, INV_LN2, 0.5));
     double r = fma(-kd, LN2_LO, fma(-kd, LN2_HI, yc));
-    double p = 1.0 / 6227020800.0;
-    p = fma(p, r, 1.0 / 479001600.0);
-    p = fma(p, r, 1.0 / 39916800.0);
-    p = fma(p, r, 1.0 / 3628800.0);
-    p = fma(p, r, 1.0 / 362880.0);
-    p = fma(p, r, 1.0 / 40320.0);
-    p = fma(p, r, 1.0 / 5040.0);
-    p = fma(p, r, 1.0 / 720.0);
-    p = fma(p, r, 1.0 / 120.0);
-    p = fma(p, r, 1.0 / 24.0);
-    p = fma(p, r, 1.0 / 6.0);
+    /* Taylor polynomial sum_{i<=13} r^i / i!: r^4..r^13 tail by Estrin, leading terms by Horner (as vg_exp) */
+    double r2 = r * r, r4 = r2 * r2;
+    double q0 = fma(r, 1.0 / 120.0, 1.0 / 24.0);
+    double q1 = fma(r, 1.0 / 5040.0, 1.0 / 720.0);
+    double q2 = fma(r, 1.0 / 362880.0, 1.0 / 40320.0);
+    double q3 = fma(r, 1.0 / 39916800.0, 1.0 / 3628800.0);
+    double q4 = fma(r, 1.0 / 6227020800.0, 1.0 / 479001600.0);
+    double s0 = fma(q1, r2, q0);
+    double s1 = fma(q3, r2, q2);
+    double t = fma(fma(q4, r4, s1), r4, s0);
+    double p = fma(t, r, 1.0 / 6.0);
     p = fma(p, r, 0.5);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
