@@ -243,6 +243,37 @@ def test_fp32_form_double_gaussian(cv):
         assert r["error"] / itg.exact < 5e-3
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("shape", [(2, 100, 300, 200_000), (1, 1000, 1000, 50_000), (3, 48, 7, 30_000)])
+def test_first_iteration_bit_exact_shapes(cv, orc, prec, shape):
+    """Shapes off the common path: n_strat > 256 (the in-bin offset's general
+    form), bin counts that are not powers of two, one dimension.  Sample dump
+    and first iteration (per-cube sums, d_h, n_h, I_0, sigma_0^2) bit-exact."""
+    from cuvegas_amd import integrands as I
+    d, n, ns, N = shape
+    itg = I.poly_peak(d)
+    _say("shapes", prec, shape)
+    st = _integrator(cv, itg, d, n, ns, N, 2, prec)
+    nh0 = st.get_nh()
+    x0 = st.get_map()
+    S = int(nh0.sum())
+    p, keep = _orc_params(orc, itg, d, n, ns, N, 1, 0 if prec == "fp64" else 1)
+    cube, ids, xs, jf = st.dump(0, S)
+    ocube, oids, ox, ojf = orc.sample_dump(p, keep, x0, nh0, 0, S)
+    assert np.array_equal(cube, ocube) and np.array_equal(ids, oids)
+    assert np.array_equal(xs, ox) and np.array_equal(jf, ojf)
+    oJF, oJF2 = orc.cube_sums(nh0, ojf, ojf * ojf, 16)
+    st.run(1, use_graph=False)
+    JF, JF2 = st.get_cube_sums()
+    assert np.array_equal(JF, oJF) and np.array_equal(JF2, oJF2)
+    odh, sums = orc.results(math.pow(1.0 / ns, d), 0.75, nh0, oJF, oJF2)
+    assert np.array_equal(st.get_dh(), odh)
+    assert np.array_equal(st.get_nh(), orc.alloc(odh, sums[2], N))
+    I_k, S2, ne = st.history()
+    assert I_k[0] == sums[0] and S2[0] == sums[1] and ne[0] == S
+    st.close()
+
+
 @pytest.mark.parametrize("case", ["gauss4_config", "dgauss8_small"])
 def test_gaussian_first_iteration_bit_exact_fp64(cv, orc, case):
     """FP64 form with the Gaussian integrands of the benchmark configurations
