@@ -7,7 +7,7 @@ Steps:
      (vg_host.cpp);
   3. link the shared library against libhiprtc.
 
-Usage: python -m cuvegas_amd.build   (or python cuvegas_amd/build.py)
+Usage: python cuvegas_amd/build.py   (run as a file: importing the package needs the library)
 """
 from __future__ import annotations
 
