@@ -20,12 +20,16 @@
 // Compile-time shape (hipRTC -D options): VG_NDIM, VG_NBINS, VG_NSTRAT, VG_K,
 // VG_BLOCK, VG_FP32 (0: FP64 sampling and evaluation; 1: FP32 sampling and
 // evaluation with FP64 accumulation), VG_LDS_BINS, VG_LDS_MAP, VG_HAS_DOMAIN,
-// VG_PAIR (two samples per lane step).
+// VG_PAIR (two samples per lane step), VG_PIPE (one sample per step, the next
+// sample's map entries requested before the current one is evaluated).
 #include "vg_philox.h"
 #include "vg_math.h"
 
 #ifndef VG_PAIR
 #define VG_PAIR 1
+#endif
+#ifndef VG_PIPE
+#define VG_PIPE 0
 #endif
 
 #if VG_FP32
@@ -95,14 +99,15 @@ struct vg_sample {
     vg_real x[D];
 };
 
-// Evaluate global sample s (cube h) of iteration it.  map is the packed
-// (edge, width) table [D][NB] in the working precision.
-template <typename MapT>
-__device__ inline void vg_eval(vg_sample& o, long long s, int h, unsigned it, unsigned seed,
-                               unsigned stream, const MapT* __restrict__ map,
-                               const double* __restrict__ domain, double domain_vol,
-                               const vg_real* __restrict__ theta) {
-    unsigned words[4 * NG];
+#if VG_FP32
+typedef float2 vg_map_t;
+#else
+typedef double2 vg_map_t;
+#endif
+
+// Random words of global sample s of iteration it (all dimension groups).
+__device__ inline void vg_words(unsigned (&words)[4 * NG], long long s, unsigned it, unsigned seed,
+                                unsigned stream) {
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
         const vg_u4 v = vg_sample_words((unsigned long long)s, it, (unsigned)g, seed, stream);
@@ -111,8 +116,25 @@ __device__ inline void vg_eval(vg_sample& o, long long s, int h, unsigned it, un
         words[4 * g + 2] = v.z;
         words[4 * g + 3] = v.w;
     }
+}
+
+// A sample located in the map: bin ids, in-bin offsets and the (edge, width)
+// entries of its bins.  Splitting location from evaluation lets the sampling
+// loop request the map entries of the next sample before it evaluates the
+// current one.
+template <typename MapT>
+struct vg_pre {
+    unsigned ids[D];
+    vg_real frac[D];
+    MapT m[D];
+};
+
+// Bins and offsets of a sample of cube h from its random words; map is the
+// packed (edge, width) table [D][NB] in the working precision.
+template <typename MapT>
+__device__ inline void vg_locate(vg_pre<MapT>& p, const unsigned (&words)[4 * NG], int h,
+                                 const MapT* __restrict__ map) {
     unsigned tmp = (unsigned)h;
-    vg_real dxv[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) {
         const unsigned q = tmp / NS;
@@ -124,31 +146,43 @@ __device__ inline void vg_eval(vg_sample& o, long long s, int h, unsigned it, un
 #if VG_NSTRAT <= 256
         // rem * 2^24 + (low >> 9) < 2^32: one conversion rounds the same real
         // number as (float)rem + (float)(low >> 9) * 2^-24 does, times 2^24.
-        const float frac = (float)((b.rem << 24) | (unsigned)(b.low >> 9)) * (0x1p-24f * inv_ns);
+        p.frac[i] = (float)((b.rem << 24) | (unsigned)(b.low >> 9)) * (0x1p-24f * inv_ns);
 #else
-        const float frac = ((float)b.rem + (float)(unsigned)(b.low >> 9) * 0x1p-24f) * inv_ns;
+        p.frac[i] = ((float)b.rem + (float)(unsigned)(b.low >> 9) * 0x1p-24f) * inv_ns;
 #endif
-        const float2 m = map[i * NB + b.id];
-        o.x[i] = fmaf(m.y, frac, m.x);
 #else
         constexpr double inv_ns = 1.0 / (double)NS;
 #if VG_NSTRAT <= 256
         // rem * 2^33 + low < 2^41 is exact in FP64 (as is rem + low * 2^-33),
         // so scaling it by 2^-33 / n_strat rounds exactly as the sum does.
         const unsigned hi = (b.rem << 1) | (unsigned)(b.low >> 32);
-        const double frac = fma((double)hi, 0x1p32, (double)(unsigned)b.low) * (0x1p-33 * inv_ns);
+        p.frac[i] = fma((double)hi, 0x1p32, (double)(unsigned)b.low) * (0x1p-33 * inv_ns);
 #else
-        const double frac = ((double)b.rem + (double)b.low * 0x1p-33) * inv_ns;
+        p.frac[i] = ((double)b.rem + (double)b.low * 0x1p-33) * inv_ns;
 #endif
-        const double2 m = map[i * NB + b.id];
-        o.x[i] = fma(m.y, frac, m.x);
 #endif
-        dxv[i] = m.y;
-        o.ids[i] = b.id;
+        p.m[i] = map[i * NB + b.id];
+        p.ids[i] = b.id;
+    }
+}
+
+// Mapped point, Jacobian and J f of a located sample.
+template <typename MapT>
+__device__ inline void vg_finish(vg_sample& o, const vg_pre<MapT>& p,
+                                 const double* __restrict__ domain, double domain_vol,
+                                 const vg_real* __restrict__ theta) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+#if VG_FP32
+        o.x[i] = fmaf(p.m[i].y, p.frac[i], p.m[i].x);
+#else
+        o.x[i] = fma(p.m[i].y, p.frac[i], p.m[i].x);
+#endif
+        o.ids[i] = p.ids[i];
     }
     vg_real jac = (vg_real)1;
 #pragma unroll
-    for (int i = 0; i < D; ++i) jac = jac * ((vg_real)NB * dxv[i]);
+    for (int i = 0; i < D; ++i) jac = jac * ((vg_real)NB * p.m[i].y);
     vg_real xe[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -167,11 +201,19 @@ __device__ inline void vg_eval(vg_sample& o, long long s, int h, unsigned it, un
     o.jf2 = o.jf * o.jf;
 }
 
-#if VG_FP32
-typedef float2 vg_map_t;
-#else
-typedef double2 vg_map_t;
-#endif
+// Evaluate global sample s (cube h) of iteration it.
+template <typename MapT>
+__device__ inline void vg_eval(vg_sample& o, long long s, int h, unsigned it, unsigned seed,
+                               unsigned stream, const MapT* __restrict__ map,
+                               const double* __restrict__ domain, double domain_vol,
+                               const vg_real* __restrict__ theta) {
+    unsigned words[4 * NG];
+    vg_words(words, s, it, seed, stream);
+    vg_pre<MapT> p;
+    vg_locate(p, words, h, map);
+    vg_finish(o, p, domain, domain_vol, theta);
+}
+
 // Per-block bin partials: FP64, or FP32 when VG_BIN32 is set (FP32 form only).
 #if VG_BIN32
 typedef float vg_bin_t;
@@ -282,6 +324,32 @@ vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
                     if (c1) close_cube();
                     take(o1);
                 }
+            }
+        }
+#elif VG_PIPE
+        // Software-pipelined: the map entries of sample s + 1 are requested
+        // (after its Philox words are formed) before sample s is evaluated and
+        // its 2 D bin atomics are issued, so the LDS (or cache) latency of the
+        // lookups is covered by the evaluation instead of being waited for
+        // behind the previous sample's atomics.  Sample s + 1 lies in cube
+        // h + (s + 1 == cend), cend being the end of the cube holding s (as
+        // updated by close_cube); it is compared only after s has been
+        // evaluated, which covers the load of a new cend.  Inactive lanes
+        // locate sample rs in cube 0: in bounds, never taken.
+        vg_pre<vg_map_t> cur;
+        unsigned wn[4 * NG];
+        vg_words(wn, rs, it, seed, stream);
+        vg_locate(cur, wn, h, map);
+        for (int j = 0; j < K; ++j) {
+            const long long s = rs + j;
+            if (s < re) {
+                if (s == cend) close_cube();
+                const bool more = s + 1 < re;
+                if (more) vg_words(wn, s + 1, it, seed, stream);
+                vg_sample o;
+                vg_finish(o, cur, domain, domain_vol, theta);
+                take(o);
+                if (more) vg_locate(cur, wn, h + (s + 1 == cend ? 1 : 0), map);
             }
         }
 #else

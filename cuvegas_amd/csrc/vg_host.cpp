@@ -87,7 +87,7 @@ int validate(const cvg_params* p) {
 struct cvg_program {
     int d = 0, nbins = 0, nstrat = 0, precision = 0, K = kDefaultRun, has_domain = 0;
     int block = kFillBlock, lds_bins = 0, lds_map = 0;
-    int has_grad = 0, n_theta = 0, bin32 = 0, pair = 1;
+    int has_grad = 0, n_theta = 0, bin32 = 0, pair = 1, pipe = 0;
     int batch = 0, max_dim = 0, task_w = 8;
     std::string log;
     std::vector<char> code;
@@ -336,22 +336,29 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
     prog->nstrat = shape->n_strat;
     prog->precision = shape->precision;
     prog->K = shape->run_len > 0 ? shape->run_len : kDefaultRun;
-    // Measured on the 8-D double Gaussian (1e8 samples): the FP32 form runs
-    // best at 16 waves per CU, one sample per step; the FP64 form (map read
-    // from global memory) at 12 waves per CU with two samples in flight per lane.
+    // Measured on the 8-D double Gaussian (1e8 samples, fill kernel): one
+    // sample per lane step with the next sample's map entries requested before
+    // the current one is evaluated, the map read through the caches (FP32
+    // 1.29 ms, against 1.35 ms with the map in LDS and no prefetch; FP64 1.66
+    // ms, against 1.70 ms with two samples per step); 16 waves per CU for the
+    // FP32 form, 12 for the FP64 form (168 VGPRs).  With the map out of LDS,
+    // the LDS serves only the bin atomics, which bound the FP32 kernel.
     prog->block = prog->precision == CVG_FP32C ? kFillBlock : 768;
-    prog->pair = prog->precision == CVG_FP32C ? 0 : 1;
+    prog->pair = 0;
+    prog->pipe = 1;
     prog->has_domain = shape->has_domain ? 1 : 0;
     prog->n_theta = shape->n_theta;
     const long long dn = (long long)prog->d * prog->nbins;
     long long bin_bytes = dn * 12;
     const long long map_bytes = prog->precision == CVG_FP32C ? dn * 8 : 0;
     prog->lds_bins = bin_bytes <= kLdsBytes;
-    prog->lds_map = prog->precision == CVG_FP32C && prog->lds_bins && bin_bytes + map_bytes <= kLdsBytes;
+    prog->lds_map = 0;
 
     // Launch-shape knobs in the options string (consumed here, not passed to
     // hipRTC): lds_bins=0|1, lds_map=0|1, block=64..1024 (multiple of 64),
-    // pair=0|1 (two samples per lane step in the sampling loop).
+    // pair=0|1 (two samples per lane step in the sampling loop), pipe=0|1
+    // (one sample per step; 1: the next sample's map entries are requested
+    // before the current sample is evaluated).
     std::string rtc_extra;
     if (options) {
         std::istringstream ss(options);
@@ -373,7 +380,7 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
                 prog->bin32 = val ? 1 : 0;
                 bin_bytes = dn * (prog->bin32 ? 8 : 12);
                 prog->lds_bins = bin_bytes <= kLdsBytes;
-                prog->lds_map = prog->precision == CVG_FP32C && prog->lds_bins && bin_bytes + map_bytes <= kLdsBytes;
+                if (!prog->lds_bins || bin_bytes + map_bytes > kLdsBytes) prog->lds_map = 0;
             } else if (key == "grad") {
                 if (shape->n_theta < 1) return fail(CVG_ERR_INVALID, "grad needs n_theta >= 1");
                 prog->has_grad = val ? 1 : 0;
@@ -382,6 +389,11 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
                 prog->block = val;
             } else if (key == "pair") {
                 prog->pair = val ? 1 : 0;
+                if (prog->pair) prog->pipe = 0;
+            } else if (key == "pipe") {
+                if (val < 0 || val > 1) return fail(CVG_ERR_INVALID, "pipe must be 0 or 1");
+                prog->pipe = val;
+                if (prog->pipe) prog->pair = 0;
             } else {
                 rtc_extra += tok + " ";
             }
@@ -407,6 +419,7 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
         "-DVG_GRAD=" + std::to_string(prog->has_grad),
         "-DVG_BIN32=" + std::to_string(prog->bin32),
         "-DVG_PAIR=" + std::to_string(prog->pair),
+        "-DVG_PIPE=" + std::to_string(prog->pipe),
         "-DVG_NTHETA=" + std::to_string(prog->n_theta)};
     if (options) {
         std::istringstream ss(options);

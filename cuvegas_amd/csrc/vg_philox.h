@@ -35,21 +35,28 @@ VG_HD unsigned int vg_xor3(unsigned int a, unsigned int b, unsigned int c) {
 #endif
 }
 
+// One Philox4x32 round with round key (k0, k1).
+VG_HD vg_u4 vg_philox_round(vg_u4 c, unsigned int k0, unsigned int k1) {
+    const unsigned long long p0 = (unsigned long long)0xD2511F53u * c.x;
+    const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c.z;
+    const unsigned int hi0 = (unsigned int)(p0 >> 32), lo0 = (unsigned int)p0;
+    const unsigned int hi1 = (unsigned int)(p1 >> 32), lo1 = (unsigned int)p1;
+    vg_u4 n;
+    n.x = vg_xor3(hi1, c.y, k0);
+    n.y = lo1;
+    n.z = vg_xor3(hi0, c.w, k1);
+    n.w = lo0;
+    return n;
+}
+
+// Key of round r (0-based): the Weyl sequence k + r * (0x9E3779B9, 0xBB67AE85).
+VG_HD unsigned int vg_philox_key0(unsigned int k0, int r) { return k0 + (unsigned int)r * 0x9E3779B9u; }
+VG_HD unsigned int vg_philox_key1(unsigned int k1, int r) { return k1 + (unsigned int)r * 0xBB67AE85u; }
+
 VG_HD vg_u4 vg_philox4x32_10(vg_u4 c, unsigned int k0, unsigned int k1) {
 #pragma unroll
-    for (int round = 0; round < 10; ++round) {
-        if (round > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        const unsigned long long p0 = (unsigned long long)0xD2511F53u * c.x;
-        const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c.z;
-        const unsigned int hi0 = (unsigned int)(p0 >> 32), lo0 = (unsigned int)p0;
-        const unsigned int hi1 = (unsigned int)(p1 >> 32), lo1 = (unsigned int)p1;
-        vg_u4 n;
-        n.x = vg_xor3(hi1, c.y, k0);
-        n.y = lo1;
-        n.z = vg_xor3(hi0, c.w, k1);
-        n.w = lo0;
-        c = n;
-    }
+    for (int round = 0; round < 10; ++round)
+        c = vg_philox_round(c, vg_philox_key0(k0, round), vg_philox_key1(k1, round));
     return c;
 }
 

@@ -36,9 +36,9 @@ def _orc_params(orc, itg, d, n, ns, N, iters, prec, K=16, seed=0, skip=1, domain
                            seed=seed, precision=prec, domain=domain)
 
 
-def _integrator(cv, itg, d, n, ns, N, iters, prec, K=16, seed=0, skip=1, domain=None):
+def _integrator(cv, itg, d, n, ns, N, iters, prec, K=16, seed=0, skip=1, domain=None, options=None):
     return cv.Integrator(itg.source, d, N, n_intervals=n, n_strat=ns, max_it=iters, skip=skip,
-                         seed=seed, precision=prec, run_len=K, domain=domain)
+                         seed=seed, precision=prec, run_len=K, domain=domain, options=options)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
@@ -127,6 +127,49 @@ def test_first_iteration_bit_exact(cv, orc, prec, layout, K):
     if layout != "mixed":  # end to end against the oracle integrator from scratch
         r = orc.run(p, keep, st.n_cubes)
         assert np.array_equal(JF, r["JF_first"]) and np.array_equal(st2.get_nh(), r["nh_first"])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("options", ["pair=1", "pipe=0", "pipe=1", "pipe=0 lds_map=1", "pipe=1 lds_map=1"])
+@pytest.mark.parametrize("K", [16, 2])
+def test_sampling_loop_variants_bit_exact(cv, orc, prec, options, K):
+    """Every sampling-loop schedule (two samples per step; one per step; the
+    next sample's map entries requested early; FP32 map read from LDS or
+    through the caches) gives the oracle's per-cube sums and bin counts bit for
+    bit: cubes of 2 samples next to cubes spanning windows, runs
+    of 2 (the peeled last sample is every other sample), a partial last run."""
+    from cuvegas_amd import integrands as I
+    if "lds_map" in options and prec == "fp64":
+        pytest.skip("the FP64 map is always read through the caches")
+    d, n, ns, N = 3, 32, 5, 100000
+    itg = I.poly_peak(d)
+    _say("loop-variant", prec, options, K)
+    p, keep = _orc_params(orc, itg, d, n, ns, N, 1, 0 if prec == "fp64" else 1, K=K)
+    st = _integrator(cv, itg, d, n, ns, N, 2, prec, K=K, options=options)
+    rng = np.random.default_rng(5)
+    st.set_nh(np.where(rng.random(st.n_cubes) < 0.2, rng.integers(500, 3000, st.n_cubes), 2))
+    nh0 = st.get_nh()
+    x0 = st.get_map()
+    S = int(nh0.sum())
+    _, ids, _, ojf = orc.sample_dump(p, keep, x0, nh0, 0, S)
+    oJF, oJF2 = orc.cube_sums(nh0, ojf, ojf * ojf, K)
+    oc = np.zeros((d, n), dtype=np.uint32)
+    ow = np.zeros((d, n))
+    for i in range(d):
+        oc[i] = np.bincount(ids[:, i], minlength=n)
+        np.add.at(ow[i], ids[:, i], ojf * ojf)
+    st.fill()
+    w, c = st.get_bins()
+    assert np.array_equal(c, oc)
+    assert np.max(np.abs(w - ow) / np.maximum(ow, 1e-300)) < 1e-12
+    # per-cube sums need the window folds of a full iteration: fresh state, same program
+    st2 = cv.Integrator(itg.source, d, N, n_intervals=n, n_strat=ns, max_it=2, skip=1, seed=0,
+                        precision=prec, run_len=K, program=st.program)
+    st2.set_nh(nh0)
+    st2.run(1, use_graph=False)
+    JF, JF2 = st2.get_cube_sums()
+    assert np.array_equal(JF, oJF)
+    assert np.array_equal(JF2, oJF2)
 
 
 def test_refine_pure_function_bit_exact(cv, orc):
