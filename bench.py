@@ -79,6 +79,20 @@ def run_form(precision, itg, n_eval_it, warmup, steps, n_strat, nbins, seed, use
     }
 
 
+def _form_difference(forms):
+    """The FP32-compensated result against the FP64 form of the same run: the
+    difference, relative to the FP64 value, and in units of the two forms'
+    combined error (the forms draw the same Philox stream, but their maps and
+    allocations diverge after the first refinement, so they are compared
+    statistically)."""
+    if "fp32" not in forms or "fp64" not in forms:
+        return None
+    a, b = forms["fp32"], forms["fp64"]
+    diff = a["result"] - b["result"]
+    return {"difference": diff, "rel_difference": diff / abs(b["result"]),
+            "pull": diff / math.hypot(a["error"], b["error"])}
+
+
 def cpu_baseline(itg, family, nbins, n_strat, n_eval_it, seed):
     """The CPU oracle (FP64 restatement of the reference algorithm) on a bounded
     sample: the first iteration of the same workload (n_eval_it samples, the
@@ -369,6 +383,7 @@ def main():
         "forms": forms,
         "fp32_over_fp64": (forms["fp32"]["evals_per_s"] / forms["fp64"]["evals_per_s"]
                            if "fp32" in forms and "fp64" in forms else None),
+        "fp32_vs_fp64": _form_difference(forms),
         "roofline": roof,
         "cpu_baseline": cpu,
         "device": {"arch": arch.value.decode(), "n_cu": n_cu.value, "clock_hz": clock_hz},

@@ -12,6 +12,7 @@ from __future__ import annotations
 import hashlib
 import json
 import os
+import sys
 import time
 from typing import Dict, Sequence, Tuple
 
@@ -76,7 +77,7 @@ def autotune_batch(source: str, items: Sequence, theta: Sequence[float], *, max_
                "evals_per_s": float(r["nevals"].sum()) / (r["loop_ms"] * 1e-3), "wall_s": wall}
         table.append(row)
         if verbose:
-            print("[autotune]", row, flush=True)
+            print("[autotune]", row, file=sys.stderr, flush=True)
     best = min(table, key=lambda r: r["ms_per_iter"])
     out = {"block": best["block"], "run_len": best["run_len"], "task_windows": best["task_windows"],
            "ms_per_iter": best["ms_per_iter"], "table": table}

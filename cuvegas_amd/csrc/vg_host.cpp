@@ -176,8 +176,10 @@ struct cvg_state {
     double* domain = nullptr;
     void* theta = nullptr;
     int* fold_list = nullptr;   // cubes whose samples span more than 64 windows
-    int* fold_count = nullptr;
+    int* fold_count = nullptr;  // [0] fold list length, [1] big-run list length
     int fold_cap = 0;
+    long long* big_list = nullptr;  // run-table chunks of cubes holding many runs
+    int big_cap = 0;
     int* bw_fold_list = nullptr;
     int* bw_fold_count = nullptr;
     int iterations_done = 0;
@@ -212,7 +214,7 @@ struct cvg_state {
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         void* bufs[] = {x_edges, map64, map32, weights, counts, refine_scratch, nh, nh_sum, dh,
                         JS, run_cube, win_first, win_last, part, bsum, boff, sync, s_total,
-                        it_p, dsum, err, hist, domain, theta, fold_list, fold_count, bw_fold_list,
+                        it_p, dsum, err, hist, domain, theta, fold_list, fold_count, big_list, bw_fold_list,
                         bw_fold_count, snap_x, snap_nh, bw_nh_sum,
                         bw_s_total, bw_x, bw_map64, bw_map32, bw_part};
         for (void* b : bufs)
@@ -259,7 +261,8 @@ struct cvg_state {
     int launch_realloc(int mode, int advance) {
         VG_HIP(vg_launch_alloc(stream, (int)C, mode, nh0, dh, dsum, p.n_eval_it, nh, bsum));
         VG_HIP(vg_launch_scan(stream, (int)C, nh, bsum, boff, sync + 1, s_total, s_cap, it_p, advance, k_shift,
-                              run_cap, nh_sum, run_cube, err, fold_list, fold_count, fold_cap));
+                              run_cap, nh_sum, run_cube, err, fold_list, fold_count, fold_cap, big_list,
+                              big_cap));
         return CVG_OK;
     }
 
@@ -296,7 +299,7 @@ struct cvg_state {
 // Layout (nh_sum, run_cube) from the n_h already on the device.
 int cvg_state::launch_realloc_copy() {
     VG_HIP(vg_launch_layout(stream, (int)C, nh, bsum, boff, sync + 1, s_total, s_cap, it_p, k_shift, run_cap, nh_sum,
-                            run_cube, err, fold_list, fold_count, fold_cap));
+                            run_cube, err, fold_list, fold_count, fold_cap, big_list, big_cap));
     return CVG_OK;
 }
 
@@ -336,23 +339,29 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
     prog->nstrat = shape->n_strat;
     prog->precision = shape->precision;
     prog->K = shape->run_len > 0 ? shape->run_len : kDefaultRun;
-    // Measured on the 8-D double Gaussian (1e8 samples, fill kernel): one
-    // sample per lane step with the next sample's map entries requested before
-    // the current one is evaluated, the map read through the caches (FP32
-    // 1.29 ms, against 1.35 ms with the map in LDS and no prefetch; FP64 1.66
-    // ms, against 1.70 ms with two samples per step); 16 waves per CU for the
-    // FP32 form, 12 for the FP64 form (168 VGPRs).  With the map out of LDS,
-    // the LDS serves only the bin atomics, which bound the FP32 kernel.
+    // Sampling-loop schedule, measured (fill kernel, MI355X):
+    //  * d <= 8: one sample per lane step with the next sample's map entries
+    //    requested before the current one is evaluated, FP32 map read through
+    //    the caches, so the LDS serves only the bin atomics that bound the
+    //    FP32 kernel.  8-D double Gaussian, 1e8 samples: FP32 1.29 ms against
+    //    1.35 ms with the map in LDS and no prefetch; FP64 1.66 ms against
+    //    1.70 ms with two samples per step.
+    //  * d > 8: the prefetched sample's state does not fit next to the
+    //    current one (16-D oscillatory, FP64: 85 VGPRs spilled, fill 8.8 ms
+    //    against 4.8 ms without the prefetch; FP32 3.42 against 3.33 ms), so
+    //    one sample per step, the FP32 map in LDS when it fits.
+    // 16 waves per CU for the FP32 form, 12 for the FP64 form (168 VGPRs).
     prog->block = prog->precision == CVG_FP32C ? kFillBlock : 768;
     prog->pair = 0;
-    prog->pipe = 1;
+    prog->pipe = prog->d <= 8 ? 1 : 0;
     prog->has_domain = shape->has_domain ? 1 : 0;
     prog->n_theta = shape->n_theta;
     const long long dn = (long long)prog->d * prog->nbins;
     long long bin_bytes = dn * 12;
     const long long map_bytes = prog->precision == CVG_FP32C ? dn * 8 : 0;
     prog->lds_bins = bin_bytes <= kLdsBytes;
-    prog->lds_map = 0;
+    prog->lds_map = prog->precision == CVG_FP32C && !prog->pipe && prog->lds_bins &&
+                    bin_bytes + map_bytes <= kLdsBytes;
 
     // Launch-shape knobs in the options string (consumed here, not passed to
     // hipRTC): lds_bins=0|1, lds_map=0|1, block=64..1024 (multiple of 64),
@@ -543,7 +552,9 @@ int cvg_state_create(cvg_program* prog, const cvg_params* params, cvg_state** ou
     if ((rc = dalloc(&st->domain, 2LL * d))) return rc;
     st->fold_cap = (int)(st->s_cap / (64LL * 64 * K) + 2);
     if ((rc = dalloc(&st->fold_list, st->fold_cap))) return rc;
-    if ((rc = dalloc(&st->fold_count, 1))) return rc;
+    if ((rc = dalloc(&st->fold_count, 2))) return rc;
+    st->big_cap = (int)vg_big_list_cap(st->run_cap);
+    if ((rc = dalloc(&st->big_list, 2LL * st->big_cap))) return rc;
     if ((rc = dalloc((char**)&st->theta, 8LL * (params->n_theta > 0 ? params->n_theta : 1)))) return rc;
 
     VG_HIP(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
@@ -958,7 +969,7 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         if ((rc = dalloc(&st->bw_map64, (long long)d * n))) return rc;
         if ((rc = dalloc(&st->bw_map32, (long long)d * n))) return rc;
         if ((rc = dalloc(&st->bw_fold_list, 1))) return rc;
-        if ((rc = dalloc(&st->bw_fold_count, 1))) return rc;
+        if ((rc = dalloc(&st->bw_fold_count, 2))) return rc;
     }
     // parameters for the replay: given theta, or the forward's
     void* th = st->theta;
@@ -992,7 +1003,7 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         const int* snh = st->snap_nh + slot * st->C;
         VG_HIP(vg_launch_layout(st->stream, C, snh, st->bsum, st->boff, st->sync + 1, st->bw_s_total, (1LL << 62), st->it_p,
                                 st->k_shift, 0, st->bw_nh_sum, nullptr, st->err, st->bw_fold_list,
-                                st->bw_fold_count, 1));
+                                st->bw_fold_count, 1, nullptr, 0));
         const void* map = st->p.precision == CVG_FP32C ? (const void*)st->bw_map32 : (const void*)st->bw_map64;
         unsigned itu = (unsigned)k;
         double V = st->V;

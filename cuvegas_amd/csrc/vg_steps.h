@@ -17,11 +17,14 @@ hipError_t vg_launch_finalize(hipStream_t st, long long nparts, double* part, un
 hipError_t vg_launch_alloc(hipStream_t st, int C, int mode, int nh_const, const double* dh,
                            const double* dsum, long long N, int* nh, long long* bsum);
 // bsum and boff hold nb + ceil(nb / 1024) int64 (nb = ceil(C / 1024)); sync is
-// one zeroed counter that the launch leaves zeroed.
+// one zeroed counter that the launch leaves zeroed.  counts: 2 ints (lengths
+// of the fold list and of big_list, 2 * big_cap int64 with big_cap =
+// vg_big_list_cap(run_cap)).  run_cube may be NULL (no run table).
 hipError_t vg_launch_scan(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
                           unsigned* sync, long long* s_total, long long s_cap, int* it_p, int advance,
                           int k_shift, long long run_cap, long long* nh_sum, int* run_cube, int* err,
-                          int* fold_list, int* fold_count, int fold_cap);
+                          int* fold_list, int* counts, int fold_cap, long long* big_list, int big_cap);
+long long vg_big_list_cap(long long run_cap);
 hipError_t vg_launch_bigfold(hipStream_t st, const int* list, const int* count, int cap,
                              const long long* nh_sum, const int* run_cube, const long long* s_total,
                              const double2* win_first, const double2* win_last, int ws_shift,
@@ -37,7 +40,7 @@ long long vg_refine_scratch_doubles(int d, int n);
 hipError_t vg_launch_layout(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
                             unsigned* sync, long long* s_total, long long s_cap, int* it_p, int k_shift,
                             long long run_cap, long long* nh_sum, int* run_cube, int* err, int* fold_list,
-                            int* fold_count, int fold_cap);
+                            int* counts, int fold_cap, long long* big_list, int big_cap);
 hipError_t vg_launch_snapshot(hipStream_t st, const int* it_p, int skip, int cap, int C, int nx,
                               const double* x_edges, const int* nh, double* snap_x, int* snap_nh);
 

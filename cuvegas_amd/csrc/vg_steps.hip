@@ -390,7 +390,7 @@ vg_scan_blocks_kernel(long long nb, const long long* __restrict__ bsum, long lon
                       long long* __restrict__ gsum, long long* __restrict__ goff,
                       unsigned* __restrict__ sync, long long* __restrict__ s_total, long long s_cap,
                       int* __restrict__ it_p, int advance, int* __restrict__ err,
-                      int* __restrict__ fold_count) {
+                      int* __restrict__ counts) {
     __shared__ long long wt[VG_TREE / 64];
     __shared__ int last;
     {
@@ -415,26 +415,33 @@ vg_scan_blocks_kernel(long long nb, const long long* __restrict__ bsum, long lon
         if (total > s_cap) { atomicOr(err, VG_ERR_CAPACITY); total = s_cap; }
         *s_total = total;
         if (advance) *it_p = *it_p + 1;
-        *fold_count = 0;
+        counts[0] = 0;
+        counts[1] = 0;
     }
 }
 
-// Per-block local scan of n_h -> nh_sum (exclusive, nh_sum[C] = total), and
-// the cube holding the first sample of every run of K = 2^k_shift samples.
-// 256 threads own 4 consecutive cubes each.  A cube writes its own runs; runs
-// of cubes with more than 16 of them are written by the whole block.
+// Per-block local scan of n_h -> nh_sum (exclusive, nh_sum[C] = total, every
+// entry capped at the sample total), the run table -- run_cube[r] = the cube
+// holding sample r * K (K = 2^k_shift) -- and the list of cubes whose samples
+// span more than VG_FOLD_MAX sampling windows.  256 threads own 4 consecutive
+// cubes each (16-byte load, 2 x 16-byte stores).  A cube writes its own runs
+// when it holds at most VG_OWN_RUNS of them; larger cubes go to big_list in
+// chunks of at most VG_BIG_CHUNK runs (r0, n << 32 | h), which
+// vg_big_runs_kernel spreads over the whole grid, so a cube holding millions
+// of samples does not hold up one block.  counts[0]: fold list length,
+// counts[1]: big list length (both cleared by vg_scan_blocks_kernel).
+#define VG_OWN_RUNS 16
+#define VG_BIG_CHUNK 8192
 __global__ void __launch_bounds__(256)
 vg_scan_cubes_kernel(int C, const int* __restrict__ nh, const long long* __restrict__ boff,
-                     const long long* __restrict__ goff, const long long* __restrict__ s_total, int k_shift, long long run_cap,
-                     long long* __restrict__ nh_sum, int* __restrict__ run_cube,
-                     int* __restrict__ fold_list, int* __restrict__ fold_count, int fold_cap) {
+                     const long long* __restrict__ goff, const long long* __restrict__ s_total, int k_shift,
+                     long long run_cap, long long* __restrict__ nh_sum, int* __restrict__ run_cube,
+                     int* __restrict__ fold_list, int* __restrict__ counts, int fold_cap,
+                     long long* __restrict__ big_list, int big_cap, int* __restrict__ err) {
     __shared__ long long wt[4];
-    __shared__ int big[VG_TREE];
-    __shared__ long long big_r0[VG_TREE], big_r1[VG_TREE];
-    __shared__ int nbig;
     const long long base = (long long)blockIdx.x * VG_TREE;
     const long long h0 = base + 4 * threadIdx.x;
-    const bool full = h0 + 4 <= C;  // the 4 cubes in range: 16-byte load, 2 x 16-byte stores
+    const bool full = h0 + 4 <= C;
     long long v[4];
     long long own = 0;
     if (full) {
@@ -446,54 +453,87 @@ vg_scan_cubes_kernel(int C, const int* __restrict__ nh, const long long* __restr
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) own += v[j];
-    if (threadIdx.x == 0) nbig = 0;
     long long total;
     const long long S = *s_total;
-    long long ex = vg_block_scan_excl(own, wt, &total) + boff[blockIdx.x] + goff[blockIdx.x / VG_TREE];
-    const long long kmask = (1LL << k_shift) - 1;
+    const long long ex = vg_block_scan_excl(own, wt, &total) + boff[blockIdx.x] + goff[blockIdx.x / VG_TREE];
+    long long e[5];
+    e[0] = ex;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) e[j + 1] = e[j] + v[j];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) e[j] = e[j] < S ? e[j] : S;
     if (full) {
-        long long e[5];
-        e[0] = ex;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) e[j + 1] = e[j] + v[j];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) e[j] = e[j] < S ? e[j] : S;
         longlong2* o = reinterpret_cast<longlong2*>(nh_sum + h0);
         o[0] = make_longlong2(e[0], e[1]);
         o[1] = make_longlong2(e[2], e[3]);
+    }
+    const long long kmask = (1LL << k_shift) - 1;
+    long long r0[4], nr[4];
+    long long nchunk = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        r0[j] = (e[j] + kmask) >> k_shift;
+        long long r1 = (e[j + 1] + kmask) >> k_shift;
+        if (r1 > run_cap) r1 = run_cap;
+        nr[j] = h0 + j < C ? r1 - r0[j] : 0;
+        if (nr[j] > VG_OWN_RUNS) nchunk += (nr[j] + VG_BIG_CHUNK - 1) / VG_BIG_CHUNK;
+    }
+    // list slots: one global atomic per block that has big cubes
+    __shared__ long long slot0;
+    long long nblock;
+    long long slot = vg_block_scan_excl(nchunk, wt, &nblock);
+    if (nblock > 0) {
+        if (threadIdx.x == 0) slot0 = atomicAdd(counts + 1, (int)nblock);
+        __syncthreads();
+        slot += slot0;
+        if (threadIdx.x == 0 && slot0 + nblock > big_cap) atomicOr(err, VG_ERR_CAPACITY);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const long long h = h0 + j;
         if (h < C) {
-            const long long st = ex < S ? ex : S;
-            const long long en = ex + v[j] < S ? ex + v[j] : S;
-            if (!full) nh_sum[h] = st;
+            if (!full) nh_sum[h] = e[j];
             if (h == C - 1) nh_sum[C] = S;
-            const long long r0 = (st + kmask) >> k_shift;
-            long long r1 = (en + kmask) >> k_shift;
-            if (r1 > run_cap) r1 = run_cap;
-            if (r1 - r0 > 16) {
-                const int slot = atomicAdd(&nbig, 1);
-                big[slot] = (int)(h - base);
-                big_r0[slot] = r0;
-                big_r1[slot] = r1;
+            const long long n = nr[j];
+            if (n > VG_OWN_RUNS) {
+                for (long long c = 0; c * VG_BIG_CHUNK < n; ++c, ++slot) {
+                    if (slot >= big_cap) continue;
+                    const long long m = n - c * VG_BIG_CHUNK < VG_BIG_CHUNK ? n - c * VG_BIG_CHUNK : VG_BIG_CHUNK;
+                    big_list[2 * slot] = r0[j] + c * VG_BIG_CHUNK;
+                    big_list[2 * slot + 1] = (m << 32) | h;
+                }
             } else {
-                for (long long r = r0; r < r1; ++r) run_cube[r] = (int)h;
+                for (long long r = r0[j]; r < r0[j] + n; ++r) run_cube[r] = (int)h;
             }
+            const long long st = e[j], en = e[j + 1];
             if (en > st && ((en - 1) >> (k_shift + 6)) - (st >> (k_shift + 6)) + 1 > VG_FOLD_MAX) {
-                const int slot = atomicAdd(fold_count, 1);
-                if (slot < fold_cap) fold_list[slot] = (int)h;
+                const int fs = atomicAdd(counts, 1);
+                if (fs < fold_cap) fold_list[fs] = (int)h;
             }
         }
-        ex += v[j];
     }
-    __syncthreads();
-    const int nb = nbig;
-    for (int i = 0; i < nb; ++i) {
-        const int h = (int)(base + big[i]);
-        for (long long r = big_r0[i] + threadIdx.x; r < big_r1[i]; r += 256) run_cube[r] = h;
+}
+
+// Runs of the cubes in big_list: one wave per chunk (grid-stride over waves).
+__global__ void __launch_bounds__(256)
+vg_big_runs_kernel(const long long* __restrict__ big_list, const int* __restrict__ counts, int big_cap,
+                   int* __restrict__ run_cube) {
+    const int n = counts[1] < big_cap ? counts[1] : big_cap;
+    const int lane = threadIdx.x & 63;
+    const int nw = gridDim.x * 4;
+    for (int e = blockIdx.x * 4 + (threadIdx.x >> 6); e < n; e += nw) {
+        const long long r0 = big_list[2LL * e];
+        const long long w = big_list[2LL * e + 1];
+        const int h = (int)(w & 0xffffffffLL);
+        const int m = (int)(w >> 32);
+        for (int i = lane; i < m; i += 64) run_cube[r0 + i] = h;
     }
+}
+
+long long vg_big_list_cap(long long run_cap) {
+    // every listed cube holds > VG_OWN_RUNS runs, and chunking adds at most
+    // one entry per VG_BIG_CHUNK runs
+    return run_cap / (VG_OWN_RUNS + 1) + run_cap / VG_BIG_CHUNK + 2;
 }
 
 // ---------------------------------------------------------------------------
@@ -671,15 +711,20 @@ hipError_t vg_launch_alloc(hipStream_t st, int C, int mode, int nh_const, const 
 hipError_t vg_launch_scan(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
                           unsigned* sync, long long* s_total, long long s_cap, int* it_p, int advance,
                           int k_shift, long long run_cap, long long* nh_sum, int* run_cube, int* err,
-                          int* fold_list, int* fold_count, int fold_cap) {
+                          int* fold_list, int* counts, int fold_cap, long long* big_list, int big_cap) {
     const long long nb = (C + VG_TREE - 1) / VG_TREE;
     const long long ng = (nb + VG_TREE - 1) / VG_TREE;
     hipLaunchKernelGGL(vg_scan_blocks_kernel, dim3((unsigned)ng), dim3(VG_TREE), 0, st, nb, bsum, boff,
-                       bsum + nb, boff + nb, sync, s_total, s_cap, it_p, advance, err, fold_count);
+                       bsum + nb, boff + nb, sync, s_total, s_cap, it_p, advance, err, counts);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (!run_cube) run_cap = 0;
     hipLaunchKernelGGL(vg_scan_cubes_kernel, dim3((unsigned)nb), dim3(256), 0, st, C, nh, boff, boff + nb,
-                       s_total, k_shift, run_cap, nh_sum, run_cube, fold_list, fold_count, fold_cap);
+                       s_total, k_shift, run_cap, nh_sum, run_cube, fold_list, counts, fold_cap, big_list,
+                       big_cap, err);
+    e = hipGetLastError();
+    if (e != hipSuccess || run_cap <= 0) return e;
+    hipLaunchKernelGGL(vg_big_runs_kernel, dim3(2048), dim3(256), 0, st, big_list, counts, big_cap, run_cube);
     return hipGetLastError();
 }
 
@@ -738,13 +783,13 @@ vg_bsum_kernel(int C, const int* __restrict__ nh, long long* __restrict__ bsum) 
 hipError_t vg_launch_layout(hipStream_t st, int C, const int* nh, long long* bsum, long long* boff,
                             unsigned* sync, long long* s_total, long long s_cap, int* it_p, int k_shift,
                             long long run_cap, long long* nh_sum, int* run_cube, int* err, int* fold_list,
-                            int* fold_count, int fold_cap) {
+                            int* counts, int fold_cap, long long* big_list, int big_cap) {
     const unsigned nb = (unsigned)((C + VG_TREE - 1) / VG_TREE);
     hipLaunchKernelGGL(vg_bsum_kernel, dim3(nb), dim3(VG_TREE), 0, st, C, nh, bsum);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return vg_launch_scan(st, C, nh, bsum, boff, sync, s_total, s_cap, it_p, 0, k_shift, run_cap, nh_sum,
-                          run_cube, err, fold_list, fold_count, fold_cap);
+                          run_cube, err, fold_list, counts, fold_cap, big_list, big_cap);
 }
 
 // Copy the map edges and allocation that the coming sampling pass will use
@@ -933,6 +978,8 @@ vg_bbigfold_kernel(const vg_bitem* __restrict__ items, const long long* __restri
     }
 }
 
+// Batched counterpart of vg_scan_cubes_kernel: block b scans 1024 cubes of
+// item blk_item[b]; the item's run table is filled run-centrically as there.
 __global__ void __launch_bounds__(256)
 vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ blk_item,
                       const int* __restrict__ nh_all, const long long* __restrict__ boff,
@@ -940,9 +987,7 @@ vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict_
                       long long* __restrict__ nh_sum_all, int* __restrict__ run_cube_all,
                       long long* __restrict__ fold_list, int* __restrict__ fold_count, int fold_cap) {
     __shared__ long long wt[4];
-    __shared__ int big[VG_TREE];
-    __shared__ long long big_r0[VG_TREE], big_r1[VG_TREE];
-    __shared__ int nbig;
+    __shared__ long long es[VG_TREE + 1];  // capped start of the block's cubes; es[nc] = end
     const int item = blk_item[blockIdx.x];
     const vg_bitem I = items[item];
     const int* nh = nh_all + I.cube_off;
@@ -951,6 +996,7 @@ vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict_
     const long long run_cap = I.wcap * 64;
     const int C = I.C;
     const long long base = (blockIdx.x - I.blk_off) * VG_TREE;
+    const int nc = C - base < VG_TREE ? (int)(C - base) : VG_TREE;
     const long long h0 = base + 4 * threadIdx.x;
     long long v[4];
     long long own = 0;
@@ -959,42 +1005,52 @@ vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict_
         v[j] = h0 + j < C ? nh[h0 + j] : 0;
         own += v[j];
     }
-    if (threadIdx.x == 0) nbig = 0;
     long long total;
     const long long S = s_item[item];
-    long long ex = vg_block_scan_excl(own, wt, &total) + boff[blockIdx.x];
-    const long long kmask = (1LL << k_shift) - 1;
+    const long long ex = vg_block_scan_excl(own, wt, &total) + boff[blockIdx.x];
+    long long e[5];
+    e[0] = ex;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) e[j + 1] = e[j] + v[j];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) e[j] = e[j] < S ? e[j] : S;
+    const int i0 = 4 * threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const long long h = h0 + j;
         if (h < C) {
-            const long long st = ex < S ? ex : S;
-            const long long en = ex + v[j] < S ? ex + v[j] : S;
-            nh_sum[h] = st;
+            nh_sum[h] = e[j];
             if (h == C - 1) nh_sum[C] = S;
-            const long long r0 = (st + kmask) >> k_shift;
-            long long r1 = (en + kmask) >> k_shift;
-            if (r1 > run_cap) r1 = run_cap;
-            if (r1 - r0 > 16) {
-                const int slot = atomicAdd(&nbig, 1);
-                big[slot] = (int)(h - base);
-                big_r0[slot] = r0;
-                big_r1[slot] = r1;
-            } else {
-                for (long long r = r0; r < r1; ++r) run_cube[r] = (int)h;
-            }
+            es[i0 + j] = e[j];
+            if (i0 + j == nc - 1) es[nc] = e[j + 1];
+            const long long st = e[j], en = e[j + 1];
             if (en > st && ((en - 1) >> (k_shift + 6)) - (st >> (k_shift + 6)) + 1 > VG_FOLD_MAX) {
                 const int slot = atomicAdd(fold_count, 1);
                 if (slot < fold_cap) fold_list[slot] = ((long long)item << 32) | (long long)h;
             }
         }
-        ex += v[j];
     }
     __syncthreads();
-    const int nbg = nbig;
-    for (int i = 0; i < nbg; ++i) {
-        const int h = (int)(base + big[i]);
-        for (long long r = big_r0[i] + threadIdx.x; r < big_r1[i]; r += 256) run_cube[r] = h;
+    const long long kmask = (1LL << k_shift) - 1;
+    const long long r0 = (es[0] + kmask) >> k_shift;
+    long long r1 = (es[nc] + kmask) >> k_shift;
+    if (r1 > run_cap) r1 = run_cap;
+    // The thread's runs increase, so its cube index only grows: a run still
+    // inside the previous run's cube costs one LDS read; otherwise binary
+    // search for the largest i in (lo, nc) with es[i] <= s.  es[lo] <= s holds
+    // throughout (es[0] <= first sample of run r0).
+    int lo = 0;
+    for (long long r = r0 + threadIdx.x; r < r1; r += 256) {
+        const long long s = r << k_shift;
+        if (lo + 1 < nc && es[lo + 1] <= s) {
+            int a = lo + 1, b = nc;
+            while (b - a > 1) {
+                const int mid = (a + b) >> 1;
+                if (es[mid] <= s) a = mid; else b = mid;
+            }
+            lo = a;
+        }
+        run_cube[r] = (int)(base + lo);
     }
 }
 
