@@ -250,10 +250,20 @@ struct cvg_state {
         return CVG_OK;
     }
 
-    int launch_results() {
+    // The results launch also refines the map (d extra blocks) when the
+    // refinement's LDS arrays are small; measured on the 8-D double Gaussian,
+    // the refinement (30 us on its own, latency-bound) then runs in the shadow
+    // of the per-cube results.
+    bool refine_in_results() const { return vg_results_refine_lds(p.n_intervals) <= 32 * 1024; }
+
+    int launch_results(bool with_refine = false) {
         VG_HIP(vg_launch_bigfold(stream, fold_list, fold_count, fold_cap, nh_sum, run_cube, s_total,
                                  win_first, win_last, 6 + k_shift, JS));
-        VG_HIP(vg_launch_results(stream, (int)C, V, p.beta, nh, JS, dh, part));
+        if (with_refine)
+            VG_HIP(vg_launch_results_refine(stream, (int)C, V, p.beta, nh, JS, dh, part, p.n_dim, p.n_intervals,
+                                            p.alpha, x_edges, weights, counts, refine_scratch, map64, map32));
+        else
+            VG_HIP(vg_launch_results(stream, (int)C, V, p.beta, nh, JS, dh, part));
         VG_HIP(vg_launch_finalize(stream, nb, part, sync, s_total, it_p, p.max_it, hist, dsum, err));
         return CVG_OK;
     }
@@ -277,8 +287,12 @@ struct cvg_state {
                                       x_edges, nh, snap_x, snap_nh));
         if ((rc = launch_fill())) return rc;
         if (e_fill) VG_HIP(hipEventRecord(e_fill, stream));
-        if ((rc = launch_refine())) return rc;
-        if ((rc = launch_results())) return rc;
+        if (refine_in_results()) {
+            if ((rc = launch_results(true))) return rc;
+        } else {
+            if ((rc = launch_refine())) return rc;
+            if ((rc = launch_results())) return rc;
+        }
         if ((rc = launch_realloc(1, 1))) return rc;
         if (e_end) VG_HIP(hipEventRecord(e_end, stream));
         return CVG_OK;

@@ -9,6 +9,13 @@
 // folded the window partials of every cube that crosses a sampling window.
 hipError_t vg_launch_results(hipStream_t st, int C, double V, double beta, const int* nh,
                              const double2* JS, double* dh, double* part);
+// Results plus the map refinement of the d dimensions in d extra blocks of the
+// same launch (needs vg_results_refine_lds(n) bytes of dynamic LDS per block).
+hipError_t vg_launch_results_refine(hipStream_t st, int C, double V, double beta, const int* nh,
+                                    const double2* JS, double* dh, double* part, int d, int n, double alpha,
+                                    double* x_edges, double* weights, unsigned* counts, double* scratch,
+                                    double2* map64, float2* map32);
+long long vg_results_refine_lds(int n);
 // part holds 3 * (nparts + ceil(nparts / 1024)) doubles; sync is one zeroed
 // counter that the launch leaves zeroed.
 hipError_t vg_launch_finalize(hipStream_t st, long long nparts, double* part, unsigned* sync,

@@ -171,11 +171,12 @@ __device__ inline void vg_cube_result_sums(long long h, int C, double V, double 
     dh[h] = dd;
 }
 
-__global__ void __launch_bounds__(256)
-vg_results_kernel(int C, double V, double beta, const int* __restrict__ nh,
-                  const double2* __restrict__ JS, double* __restrict__ dh, double* __restrict__ part) {
-    __shared__ double red[256];
-    const long long base = (long long)blockIdx.x * VG_TREE + threadIdx.x;
+// Results of the 1024 cubes of block b: d_h, and the block partials of
+// (sum I_h, sum sigma_h^2 / n_h, sum d_h) in part[3b + q].  256 threads.
+__device__ inline void vg_results_block(long long b, int C, double V, double beta,
+                                        const int* __restrict__ nh, const double2* __restrict__ JS,
+                                        double* __restrict__ dh, double* __restrict__ part, double* red) {
+    const long long base = b * VG_TREE + threadIdx.x;
     double r[4][3];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
@@ -183,8 +184,15 @@ vg_results_kernel(int C, double V, double beta, const int* __restrict__ nh,
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const double tot = vg_tree1024_256(r[0][q], r[1][q], r[2][q], r[3][q], red);
-        if (threadIdx.x == 0) part[3 * blockIdx.x + q] = tot;
+        if (threadIdx.x == 0) part[3 * b + q] = tot;
     }
+}
+
+__global__ void __launch_bounds__(256)
+vg_results_kernel(int C, double V, double beta, const int* __restrict__ nh,
+                  const double2* __restrict__ JS, double* __restrict__ dh, double* __restrict__ part) {
+    __shared__ double red[256];
+    vg_results_block(blockIdx.x, C, V, beta, nh, JS, dh, part, red);
 }
 
 // Cubes whose samples span more than VG_FOLD_MAX windows: the window partials
@@ -570,8 +578,11 @@ vg_init_map_kernel(int n, double* __restrict__ x_edges, double2* __restrict__ ma
     vg_write_tables(n, x, map64 + (long long)dim * n, map32 + (long long)dim * n);
 }
 
-// Strided-pairwise tree sum of a[0..m) (global), result broadcast; the same
-// order as the per-cube reductions.  part: scratch of (m+1023)/1024 entries.
+// Strided-pairwise tree sum of a[0..m), result broadcast; the same order as
+// the per-cube reductions.  part: scratch of (m+1023)/1024 entries.  NT = 1024
+// threads (t: 1024 doubles of LDS) or 256 threads holding 4 elements each
+// (t: 256 doubles; vg_tree1024_256, bit-identical).
+template <int NT>
 __device__ double vg_block_tree_sum(const double* __restrict__ a, int m, double* __restrict__ part,
                                     double* t) {
     int cur = m;
@@ -579,11 +590,22 @@ __device__ double vg_block_tree_sum(const double* __restrict__ a, int m, double*
     while (true) {
         const int nch = (cur + VG_TREE - 1) / VG_TREE;
         for (int c = 0; c < nch; ++c) {
-            const int j = c * VG_TREE + threadIdx.x;
-            __syncthreads();
-            t[threadIdx.x] = j < cur ? src[j] : 0.0;
-            vg_tree1024(t);
-            if (threadIdx.x == 0) part[c] = t[0];
+            if (NT == VG_TREE) {
+                const int j = c * VG_TREE + threadIdx.x;
+                __syncthreads();
+                t[threadIdx.x] = j < cur ? src[j] : 0.0;
+                vg_tree1024(t);
+                if (threadIdx.x == 0) part[c] = t[0];
+            } else {
+                double v[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int j = c * VG_TREE + threadIdx.x + 256 * q;
+                    v[q] = j < cur ? src[j] : 0.0;
+                }
+                const double tot = vg_tree1024_256(v[0], v[1], v[2], v[3], t);
+                if (threadIdx.x == 0) part[c] = tot;
+            }
         }
         __syncthreads();
         if (nch == 1) break;
@@ -596,19 +618,39 @@ __device__ double vg_block_tree_sum(const double* __restrict__ a, int m, double*
     return r;
 }
 
+// Working arrays of one dimension's refinement.
+struct vg_refine_ws {
+    double* w;     // n   normalised bin weights
+    double* d;     // n   smoothed, damped weights
+    double* cum;   // n   their running sum
+    double* xo;    // n+1 edges before the update
+    double* part;  // n/1024 + 2 tree partials
+};
+
+// The arrays laid out consecutively from base (LDS or global).
+__device__ inline vg_refine_ws vg_refine_ws_at(double* base, int n) {
+    vg_refine_ws ws;
+    ws.w = base;
+    ws.d = ws.w + n;
+    ws.cum = ws.d + n;
+    ws.xo = ws.cum + n;
+    ws.part = ws.xo + n + 1;
+    return ws;
+}
+
 // Refinement of one dimension: normalise the bin sums, smooth, damp, cumulate,
 // invert (reference normalizeWeights / smoothWeights / setMap / updateXEdges /
-// updateDxEdges, cuda_wrapper.cu:51-90,134-191).  base holds the working
-// arrays w[n], d[n], cum[n], xo[n+1], part[n/1024 + 2] (LDS or global).
+// updateDxEdges, cuda_wrapper.cu:51-90,134-191), with a block of NT threads.
+template <int NT>
 __device__ void vg_refine_dim(int n, double alpha, double* __restrict__ x, double* __restrict__ wb,
-                              unsigned* __restrict__ cb, double* base, double2* __restrict__ map64,
+                              unsigned* __restrict__ cb, vg_refine_ws ws, double2* __restrict__ map64,
                               float2* __restrict__ map32, double* t) {
     const int E = n + 1;
-    double* __restrict__ w = base;
-    double* __restrict__ d = w + n;
-    double* __restrict__ cum = d + n;
-    double* __restrict__ xo = cum + n;
-    double* __restrict__ part = xo + E;
+    double* __restrict__ w = ws.w;
+    double* __restrict__ d = ws.d;
+    double* __restrict__ cum = ws.cum;
+    double* __restrict__ xo = ws.xo;
+    double* __restrict__ part = ws.part;
 
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         const unsigned c = cb[k];
@@ -618,7 +660,7 @@ __device__ void vg_refine_dim(int n, double alpha, double* __restrict__ x, doubl
     }
     for (int j = threadIdx.x; j < E; j += blockDim.x) xo[j] = x[j];
     __syncthreads();
-    const double dsum = vg_block_tree_sum(w, n, part, t);
+    const double dsum = vg_block_tree_sum<NT>(w, n, part, t);
     if (dsum > 0.0) {
         for (int k = threadIdx.x; k < n; k += blockDim.x) {
             double v;
@@ -629,7 +671,7 @@ __device__ void vg_refine_dim(int n, double alpha, double* __restrict__ x, doubl
             d[k] = v;
         }
         __syncthreads();
-        const double S = vg_block_tree_sum(d, n, part, t);
+        const double S = vg_block_tree_sum<NT>(d, n, part, t);
         if (threadIdx.x == 0) {  // sequential inclusive sum, loads batched ahead of the adds
             double acc = 0.0;
             int k = 0;
@@ -671,10 +713,40 @@ vg_refine_kernel(int n, double alpha, int use_lds, double* __restrict__ x_edges,
     extern __shared__ double dyn[];
     const int dim = blockIdx.x;
     double* base = use_lds ? dyn : scratch + (long long)dim * (4LL * n + 1 + n / VG_TREE + 2);
-    vg_refine_dim(n, alpha, x_edges + (long long)dim * (n + 1), weights + (long long)dim * n,
-                  counts + (long long)dim * n, base, map64 + (long long)dim * n,
-                  map32 + (long long)dim * n, t);
+    vg_refine_dim<VG_TREE>(n, alpha, x_edges + (long long)dim * (n + 1), weights + (long long)dim * n,
+                           counts + (long long)dim * n, vg_refine_ws_at(base, n), map64 + (long long)dim * n,
+                           map32 + (long long)dim * n, t);
 }
+
+// Per-cube results in blocks rd.. and, beside them, the map refinement of
+// dimension b in block b < rd (the two read and write disjoint buffers, and
+// the refinement of a few dimensions is latency-bound, so it runs in the
+// shadow of the results).  The refinement keeps d, cum and the tree partials
+// in dynamic LDS (2n + n/1024 + 2 doubles) and w, xo in the global scratch.
+__global__ void __launch_bounds__(256)
+vg_results_refine_kernel(int C, double V, double beta, const int* __restrict__ nh,
+                         const double2* __restrict__ JS, double* __restrict__ dh, double* __restrict__ part,
+                         int rd, int n, double alpha, double* __restrict__ x_edges,
+                         double* __restrict__ weights, unsigned* __restrict__ counts,
+                         double* __restrict__ scratch, double2* __restrict__ map64,
+                         float2* __restrict__ map32) {
+    __shared__ double red[256];
+    extern __shared__ double dyn[];
+    if ((int)blockIdx.x < rd) {
+        const int dim = blockIdx.x;
+        vg_refine_ws ws = vg_refine_ws_at(scratch + (long long)dim * (4LL * n + 1 + n / VG_TREE + 2), n);
+        ws.d = dyn;
+        ws.cum = dyn + n;
+        ws.part = dyn + 2 * n;
+        vg_refine_dim<256>(n, alpha, x_edges + (long long)dim * (n + 1), weights + (long long)dim * n,
+                           counts + (long long)dim * n, ws, map64 + (long long)dim * n,
+                           map32 + (long long)dim * n, red);
+        return;
+    }
+    vg_results_block(blockIdx.x - rd, C, V, beta, nh, JS, dh, part, red);
+}
+
+long long vg_results_refine_lds(int n) { return 8LL * (2LL * n + n / VG_TREE + 2); }
 
 long long vg_refine_scratch_doubles(int d, int n) {
     return (long long)d * (4LL * n + 1 + n / VG_TREE + 2);
@@ -688,6 +760,17 @@ hipError_t vg_launch_results(hipStream_t st, int C, double V, double beta, const
                              const double2* JS, double* dh, double* part) {
     const unsigned nb = (unsigned)((C + VG_TREE - 1) / VG_TREE);
     hipLaunchKernelGGL(vg_results_kernel, dim3(nb), dim3(256), 0, st, C, V, beta, nh, JS, dh, part);
+    return hipGetLastError();
+}
+
+hipError_t vg_launch_results_refine(hipStream_t st, int C, double V, double beta, const int* nh,
+                                    const double2* JS, double* dh, double* part, int d, int n, double alpha,
+                                    double* x_edges, double* weights, unsigned* counts, double* scratch,
+                                    double2* map64, float2* map32) {
+    const unsigned nb = (unsigned)((C + VG_TREE - 1) / VG_TREE);
+    hipLaunchKernelGGL(vg_results_refine_kernel, dim3(nb + d), dim3(256), (size_t)vg_results_refine_lds(n), st,
+                       C, V, beta, nh, JS, dh, part, d, n, alpha, x_edges, weights, counts, scratch, map64,
+                       map32);
     return hipGetLastError();
 }
 
@@ -1075,7 +1158,8 @@ vg_brefine_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ rd
         vg_write_tables(n, x, map64 + mo, map32 + mo);
         return;
     }
-    vg_refine_dim(n, alpha, x, weights + mo, counts + mo, dyn, map64 + mo, map32 + mo, t);
+    vg_refine_dim<VG_TREE>(n, alpha, x, weights + mo, counts + mo, vg_refine_ws_at(dyn, n), map64 + mo,
+                           map32 + mo, t);
 }
 
 hipError_t vg_launch_batch_iteration_steps(hipStream_t st, const vg_bitem* items, int n_items,
