@@ -105,7 +105,13 @@ int cvg_device_info(int* n_cu, int* clock_khz, char* arch, int arch_len);
 
 /* Compile an integrand source (HIP C++) for the given shape with hipRTC.
  * shape supplies n_dim, n_intervals, n_strat, precision, run_len, has_domain.
- * options: extra hipRTC options separated by spaces, or NULL.
+ * options: space-separated launch knobs and extra hipRTC options, or NULL.
+ * Knobs (consumed here): block=64..1024 (multiple of 64), pipe=0|1 (one
+ * sample per lane step; 1: the next sample's map entries are requested before
+ * the current one is evaluated, the default for n_dim <= 8), pair=0|1 (two
+ * samples per step), lds_bins=0|1, lds_map=0|1 (FP32 map table in LDS),
+ * bin32=0|1, grad=0|1 (compile the backward replay kernel).  Every other
+ * token is passed to hipRTC.
  * On CVG_ERR_COMPILE, *out is still set so cvg_program_log can be read. */
 int cvg_program_create(const char* integrand_src, const cvg_params* shape, const char* options,
                        cvg_program** out);
