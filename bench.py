@@ -100,7 +100,7 @@ def cpu_baseline(itg, family, nbins, n_strat, n_eval_it, seed):
     this process may use (OpenMP over sample windows), and on one core."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    fid = {"double_gaussian": orc.F_DGAUSS, "oscillatory": orc.F_OSC}[family]
+    fid = {"double_gaussian": orc.F_DGAUSS, "oscillatory": orc.F_OSC, "gaussian_peak": orc.F_GAUSS}[family]
     cores = min(len(os.sched_getaffinity(0)), 16)
     out = {}
     for threads in (cores, 1):
@@ -274,15 +274,14 @@ def copy_probe():
 
 
 def _traffic_from_profile(workload):
-    """HBM bytes per vg_fill launch (FP32 form) from the committed PMC summary."""
-    path = os.path.join(ROOT, "profiles", f"vg_fill_{workload}_hbm_traffic.md")
-    try:
+    """HBM bytes per vg_fill launch (FP32 form) from the committed PMC summary of
+    the workload (profiles/vg_fill_<workload>_*.md, line traffic_bytes_per_launch)."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"vg_fill_{workload}_*.md"))):
         with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
-    except OSError:
-        pass
     return None
 
 
@@ -299,8 +298,10 @@ def main():
                     help="batch512: 'block,run_len,task_windows' instead of autotuning")
     ap.add_argument("--skip-sequential", action="store_true",
                     help="batch512: time the batched run only (for profiling)")
-    ap.add_argument("--workload", choices=["dg8", "osc16", "mix6", "batch512", "copy"], default="dg8",
+    ap.add_argument("--workload", choices=["dg8", "osc16", "gauss4", "mix6", "batch512", "copy"], default="dg8",
                     help="dg8: the 8-D double Gaussian headline; osc16: 16-D oscillatory; "
+                         "gauss4: the 4-D Gaussian of the CPU configuration at 1e8 evals/iteration "
+                         "(a cheap integrand: bin atomics dominate); "
                          "mix6: 6-D mixture forward + gradient; batch512: 512-integral batch; "
                          "copy: 1 GiB device copies for counter calibration")
     args = ap.parse_args()
@@ -331,6 +332,16 @@ def main():
         n_eval_it = int(args.neval or 1e8)
         itg = I.double_gaussian(d, 0.33, 0.67, 0.05)
         desc = (f"8-D double Gaussian peak sigma=0.05, nbins=1024, n_strat=8 (16,777,216 cubes), "
+                f"{n_eval_it:.0e} evals/iteration")
+        cpu_neval = n_eval_it
+    elif args.workload == "gauss4":
+        # the CPU configuration's integrand (4-D normalised Gaussian, sigma = 0.1, nbins 1024)
+        # at 1e8 evaluations per iteration; n_strat by the reference's formula
+        d, nbins, family = 4, 1024, "gaussian_peak"
+        n_eval_it = int(args.neval or 1e8)
+        n_strat = max(1, int(math.pow(n_eval_it / 4.0, 1.0 / d)))
+        itg = I.gaussian_peak(d, 0.5, 0.1)
+        desc = (f"4-D Gaussian peak sigma=0.1, nbins=1024, n_strat={n_strat} ({n_strat ** d:,} cubes), "
                 f"{n_eval_it:.0e} evals/iteration")
         cpu_neval = n_eval_it
     else:  # osc16: parameters drawn once (numpy default_rng(1)), committed as a fixture
