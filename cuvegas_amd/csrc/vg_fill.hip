@@ -681,7 +681,6 @@ __device__ inline unsigned vg_udiv(unsigned x, unsigned m, unsigned sh1, unsigne
 }
 
 constexpr int MD = VG_MAXD;
-constexpr int MG = (MD + 3) / 4;
 constexpr int NB = VG_NBINS;
 constexpr int K = VG_K;
 constexpr int WPB = VG_BLOCK / 64;
@@ -693,45 +692,44 @@ typedef float2 vg_map_t;
 typedef double2 vg_map_t;
 #endif
 
-__device__ inline void vg_beval(double& jf, unsigned* ids, long long s, int h, int d, unsigned it,
+// J f of sample s (cube h) of an item of dimension DD; bin ids to ids[0..DD).
+template <int DD>
+__device__ inline void vg_beval(double& jf, unsigned* ids, long long s, int h, unsigned it,
                                 unsigned seed, unsigned stream, const vg_item& I,
                                 const vg_map_t* __restrict__ map, const vg_real* __restrict__ theta) {
-    unsigned words[4 * MG];
+    constexpr int GG = (DD + 3) / 4;
+    unsigned words[4 * GG];
 #pragma unroll
-    for (int g = 0; g < MG; ++g) {
-        if (4 * g < d) {
-            const vg_u4 v = vg_sample_words((unsigned long long)s, it, (unsigned)g, seed, stream);
-            words[4 * g + 0] = v.x; words[4 * g + 1] = v.y;
-            words[4 * g + 2] = v.z; words[4 * g + 3] = v.w;
-        }
+    for (int g = 0; g < GG; ++g) {
+        const vg_u4 v = vg_sample_words((unsigned long long)s, it, (unsigned)g, seed, stream);
+        words[4 * g + 0] = v.x; words[4 * g + 1] = v.y;
+        words[4 * g + 2] = v.z; words[4 * g + 3] = v.w;
     }
     unsigned tmp = (unsigned)h;
-    vg_real x[MD], jac = (vg_real)1;
+    vg_real x[DD], jac = (vg_real)1;
 #pragma unroll
-    for (int i = 0; i < MD; ++i) {
-        if (i < d) {
-            const unsigned q = vg_udiv(tmp, I.ns_m, I.ns_sh1, I.ns_sh2);
-            const unsigned k = tmp - q * I.ns;
-            tmp = q;
-            const unsigned long long L = ((unsigned long long)words[i] * 2ull + 1ull) * (unsigned long long)NB;
-            const unsigned Q = k * (unsigned)NB + (unsigned)(L >> 33);
-            const unsigned id = vg_udiv(Q, I.ns_m, I.ns_sh1, I.ns_sh2);
-            const unsigned rem = Q - id * I.ns;
-            const unsigned long long low = L & ((1ull << 33) - 1ull);
+    for (int i = 0; i < DD; ++i) {
+        const unsigned q = vg_udiv(tmp, I.ns_m, I.ns_sh1, I.ns_sh2);
+        const unsigned k = tmp - q * I.ns;
+        tmp = q;
+        const unsigned long long L = ((unsigned long long)words[i] * 2ull + 1ull) * (unsigned long long)NB;
+        const unsigned Q = k * (unsigned)NB + (unsigned)(L >> 33);
+        const unsigned id = vg_udiv(Q, I.ns_m, I.ns_sh1, I.ns_sh2);
+        const unsigned rem = Q - id * I.ns;
+        const unsigned long long low = L & ((1ull << 33) - 1ull);
 #if VG_FP32
-            const float frac = ((float)rem + (float)(unsigned)(low >> 9) * 0x1p-24f) * I.inv_ns_f;
-            const float2 m = map[i * NB + id];
-            x[i] = fmaf(m.y, frac, m.x);
+        const float frac = ((float)rem + (float)(unsigned)(low >> 9) * 0x1p-24f) * I.inv_ns_f;
+        const float2 m = map[i * NB + id];
+        x[i] = fmaf(m.y, frac, m.x);
 #else
-            const double frac = ((double)rem + (double)low * 0x1p-33) * I.inv_ns;
-            const double2 m = map[i * NB + id];
-            x[i] = fma(m.y, frac, m.x);
+        const double frac = ((double)rem + (double)low * 0x1p-33) * I.inv_ns;
+        const double2 m = map[i * NB + id];
+        x[i] = fma(m.y, frac, m.x);
 #endif
-            jac = jac * ((vg_real)NB * m.y);
-            ids[i] = id;
-        }
+        jac = jac * ((vg_real)NB * m.y);
+        ids[i] = id;
     }
-    const vg_real f = integrand(x, theta, d);
+    const vg_real f = integrand(x, theta, DD);
     jf = (double)(f * jac);
 }
 
@@ -740,6 +738,96 @@ __device__ inline void vg_beval(double& jf, unsigned* ids, long long s, int h, i
 #define VG_TASKW 8
 #endif
 
+// The windows of one task of an item of dimension DD (the block's LDS bins
+// hold that item's partials).
+template <int DD>
+__device__ inline void vg_btask(const vg_item& I, int item, long long w0, long long S, long long nwin,
+                                unsigned it, const vg_map_t* __restrict__ map,
+                                const long long* __restrict__ nhs, const int* __restrict__ run_cube,
+                                double* __restrict__ jf1, double* __restrict__ jf2,
+                                double2* __restrict__ win_first, double2* __restrict__ win_last,
+                                const vg_real* __restrict__ theta, unsigned seed, double* lw,
+                                unsigned* lc) {
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    for (int tw = 0; tw < VG_TASKW; ++tw) {
+        const long long w = w0 + wid + (long long)tw * WPB;
+        if (w >= nwin) continue;
+        const long long r = w * 64 + lane;
+        const long long rs = r * K;
+        const bool active = rs < S;
+        const long long re = active ? (rs + K < S ? rs + K : S) : rs;
+        int h = 0;
+        long long cstart = 0, cend = 0;
+        if (active) {
+            h = run_cube[I.win_off * 64 + r];
+            cstart = nhs[h];
+            cend = nhs[h + 1];
+        }
+        const long long startF = cstart;
+        const bool sb = active && cstart < rs;
+        bool hasF = false;
+        int hF = 0, npieces = 0;
+        double f1 = 0.0, f2 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int j = 0; j < K; ++j) {
+            const long long s = rs + j;
+            if (s < re) {
+                if (s == cend) {
+                    if (npieces == 0 && sb) { hasF = true; hF = h; f1 = a1; f2 = a2; }
+                    else { jf1[h] = a1; jf2[h] = a2; }
+                    ++npieces;
+                    ++h;
+                    cstart = cend;
+                    cend = nhs[h + 1];
+                    a1 = 0.0; a2 = 0.0;
+                }
+                double jf;
+                unsigned ids[DD];
+                vg_beval<DD>(jf, ids, s, h, it, seed, (unsigned)item, I, map, theta);
+                const double jq = jf * jf;
+#pragma unroll
+                for (int i = 0; i < DD; ++i) {
+                    atomicAdd(&lw[i * NB + ids[i]], jq);
+                    atomicAdd(&lc[i * NB + ids[i]], 1u);
+                }
+                a1 = a1 + jf;
+                a2 = a2 + jq;
+            }
+        }
+        const bool cont = cend > re;
+        int key = active ? h : -1;
+        double s1 = active ? a1 : 0.0, s2 = active ? a2 : 0.0;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int kp = __shfl_up(key, off, 64);
+            const double p1 = __shfl_up(s1, off, 64);
+            const double p2 = __shfl_up(s2, off, 64);
+            if (lane >= off && kp == key) { s1 = p1 + s1; s2 = p2 + s2; }
+        }
+        const double q1 = __shfl_up(s1, 1, 64), q2 = __shfl_up(s2, 1, 64);
+        const long long wstart = w * WS;
+        const long long gw = I.win_off + w;
+        if (active) {
+            if (hasF) {
+                double t1 = f1, t2 = f2;
+                if (lane > 0) { t1 = q1 + t1; t2 = q2 + t2; }
+                if (startF < wstart) win_first[gw] = make_double2(t1, t2);
+                else { jf1[hF] = t1; jf2[hF] = t2; }
+            }
+            if (!cont) {
+                if (cstart < wstart) win_first[gw] = make_double2(s1, s2);
+                else { jf1[h] = s1; jf2[h] = s2; }
+            } else if (lane == 63) {
+                win_last[gw] = make_double2(s1, s2);
+                if (cstart < wstart) win_first[gw] = make_double2(s1, s2);
+            }
+        }
+    }
+}
+
+// Work is a static list of tasks (item, first window); the task body is
+// specialised for the item's dimension (a switch that is uniform over the
+// block), so every loop over dimensions unrolls.
 extern "C" __global__ void __launch_bounds__(VG_BLOCK)
 vg_bfill(const vg_item* __restrict__ items, const int* __restrict__ task_item,
          const int* __restrict__ task_w0, int ntasks, const vg_map_t* __restrict__ gmap,
@@ -752,8 +840,6 @@ vg_bfill(const vg_item* __restrict__ items, const int* __restrict__ task_item,
     __shared__ double lw[MD * NB];
     __shared__ unsigned lc[MD * NB];
     const unsigned it = (unsigned)*it_p;
-    const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
     for (int task = blockIdx.x; task < ntasks; task += gridDim.x) {
         const int item = task_item[task];
         const long long w0 = task_w0[task];
@@ -769,82 +855,20 @@ vg_bfill(const vg_item* __restrict__ items, const int* __restrict__ task_item,
         double* jf2 = JF2 + I.cube_off;
         for (int i = threadIdx.x; i < d * NB; i += VG_BLOCK) { lw[i] = 0.0; lc[i] = 0u; }
         __syncthreads();
-        for (int tw = 0; tw < VG_TASKW; ++tw) {
-        const long long w = w0 + wid + (long long)tw * WPB;
-        if (w < nwin) {
-            const long long r = w * 64 + lane;
-            const long long rs = r * K;
-            const bool active = rs < S;
-            const long long re = active ? (rs + K < S ? rs + K : S) : rs;
-            int h = 0;
-            long long cstart = 0, cend = 0;
-            if (active) {
-                h = run_cube[I.win_off * 64 + r];
-                cstart = nhs[h];
-                cend = nhs[h + 1];
-            }
-            const long long startF = cstart;
-            const bool sb = active && cstart < rs;
-            bool hasF = false;
-            int hF = 0, npieces = 0;
-            double f1 = 0.0, f2 = 0.0, a1 = 0.0, a2 = 0.0;
-            for (int j = 0; j < K; ++j) {
-                const long long s = rs + j;
-                if (s < re) {
-                    if (s == cend) {
-                        if (npieces == 0 && sb) { hasF = true; hF = h; f1 = a1; f2 = a2; }
-                        else { jf1[h] = a1; jf2[h] = a2; }
-                        ++npieces;
-                        ++h;
-                        cstart = cend;
-                        cend = nhs[h + 1];
-                        a1 = 0.0; a2 = 0.0;
-                    }
-                    double jf;
-                    unsigned ids[MD];
-                    vg_beval(jf, ids, s, h, d, it, seed, (unsigned)item, I, map, theta);
-                    const double jq = jf * jf;
-#pragma unroll
-                    for (int i = 0; i < MD; ++i) {
-                        if (i < d) {
-                            atomicAdd(&lw[i * NB + ids[i]], jq);
-                            atomicAdd(&lc[i * NB + ids[i]], 1u);
-                        }
-                    }
-                    a1 = a1 + jf;
-                    a2 = a2 + jq;
-                }
-            }
-            const bool cont = cend > re;
-            int key = active ? h : -1;
-            double s1 = active ? a1 : 0.0, s2 = active ? a2 : 0.0;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int kp = __shfl_up(key, off, 64);
-                const double p1 = __shfl_up(s1, off, 64);
-                const double p2 = __shfl_up(s2, off, 64);
-                if (lane >= off && kp == key) { s1 = p1 + s1; s2 = p2 + s2; }
-            }
-            const double q1 = __shfl_up(s1, 1, 64), q2 = __shfl_up(s2, 1, 64);
-            const long long wstart = w * WS;
-            const long long gw = I.win_off + w;
-            if (active) {
-                if (hasF) {
-                    double t1 = f1, t2 = f2;
-                    if (lane > 0) { t1 = q1 + t1; t2 = q2 + t2; }
-                    if (startF < wstart) win_first[gw] = make_double2(t1, t2);
-                    else { jf1[hF] = t1; jf2[hF] = t2; }
-                }
-                if (!cont) {
-                    if (cstart < wstart) win_first[gw] = make_double2(s1, s2);
-                    else { jf1[h] = s1; jf2[h] = s2; }
-                } else if (lane == 63) {
-                    win_last[gw] = make_double2(s1, s2);
-                    if (cstart < wstart) win_first[gw] = make_double2(s1, s2);
-                }
-            }
+        switch (d) {
+#define VG_BTASK_CASE(n)                                                                         \
+    case n:                                                                                      \
+        if constexpr (n <= MD)                                                                   \
+            vg_btask<n>(I, item, w0, S, nwin, it, map, nhs, run_cube, jf1, jf2, win_first,       \
+                        win_last, theta, seed, lw, lc);                                          \
+        break;
+            VG_BTASK_CASE(1) VG_BTASK_CASE(2) VG_BTASK_CASE(3) VG_BTASK_CASE(4)
+            VG_BTASK_CASE(5) VG_BTASK_CASE(6) VG_BTASK_CASE(7) VG_BTASK_CASE(8)
+            VG_BTASK_CASE(9) VG_BTASK_CASE(10) VG_BTASK_CASE(11) VG_BTASK_CASE(12)
+            VG_BTASK_CASE(13) VG_BTASK_CASE(14) VG_BTASK_CASE(15) VG_BTASK_CASE(16)
+#undef VG_BTASK_CASE
+            default: break;
         }
-        }  // windows of this task
         __syncthreads();
         for (int i = threadIdx.x; i < d * NB; i += VG_BLOCK) {
             const unsigned c = lc[i];

@@ -445,7 +445,7 @@ vg_scan_cubes_kernel(int C, const int* __restrict__ nh, const long long* __restr
                      const long long* __restrict__ goff, const long long* __restrict__ s_total, int k_shift,
                      long long run_cap, long long* __restrict__ nh_sum, int* __restrict__ run_cube,
                      int* __restrict__ fold_list, int* __restrict__ counts, int fold_cap,
-                     long long* __restrict__ big_list, int big_cap, int* __restrict__ err) {
+                     long long* __restrict__ big_list, int big_cap) {
     __shared__ long long wt[4];
     const long long base = (long long)blockIdx.x * VG_TREE;
     const long long h0 = base + 4 * threadIdx.x;
@@ -494,7 +494,6 @@ vg_scan_cubes_kernel(int C, const int* __restrict__ nh, const long long* __restr
         if (threadIdx.x == 0) slot0 = atomicAdd(counts + 1, (int)nblock);
         __syncthreads();
         slot += slot0;
-        if (threadIdx.x == 0 && slot0 + nblock > big_cap) atomicOr(err, VG_ERR_CAPACITY);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -505,10 +504,14 @@ vg_scan_cubes_kernel(int C, const int* __restrict__ nh, const long long* __restr
             const long long n = nr[j];
             if (n > VG_OWN_RUNS) {
                 for (long long c = 0; c * VG_BIG_CHUNK < n; ++c, ++slot) {
-                    if (slot >= big_cap) continue;
                     const long long m = n - c * VG_BIG_CHUNK < VG_BIG_CHUNK ? n - c * VG_BIG_CHUNK : VG_BIG_CHUNK;
-                    big_list[2 * slot] = r0[j] + c * VG_BIG_CHUNK;
-                    big_list[2 * slot + 1] = (m << 32) | h;
+                    const long long rc = r0[j] + c * VG_BIG_CHUNK;
+                    if (slot < big_cap) {
+                        big_list[2 * slot] = rc;
+                        big_list[2 * slot + 1] = (m << 32) | h;
+                    } else {  // never with big_cap = vg_big_list_cap(run_cap); kept correct anyway
+                        for (long long r = rc; r < rc + m; ++r) run_cube[r] = (int)h;
+                    }
                 }
             } else {
                 for (long long r = r0[j]; r < r0[j] + n; ++r) run_cube[r] = (int)h;
@@ -804,7 +807,7 @@ hipError_t vg_launch_scan(hipStream_t st, int C, const int* nh, long long* bsum,
     if (!run_cube) run_cap = 0;
     hipLaunchKernelGGL(vg_scan_cubes_kernel, dim3((unsigned)nb), dim3(256), 0, st, C, nh, boff, boff + nb,
                        s_total, k_shift, run_cap, nh_sum, run_cube, fold_list, counts, fold_cap, big_list,
-                       big_cap, err);
+                       big_cap);
     e = hipGetLastError();
     if (e != hipSuccess || run_cap <= 0) return e;
     hipLaunchKernelGGL(vg_big_runs_kernel, dim3(2048), dim3(256), 0, st, big_list, counts, big_cap, run_cube);
