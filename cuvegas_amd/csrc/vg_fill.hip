@@ -692,9 +692,36 @@ typedef float2 vg_map_t;
 typedef double2 vg_map_t;
 #endif
 
-// J f of sample s (cube h) of an item of dimension DD; bin ids to ids[0..DD).
+// Strata digits of cube h as kn[i] = k_i * NB (k_i the i-th base-n_strat digit).
 template <int DD>
-__device__ inline void vg_beval(double& jf, unsigned* ids, long long s, int h, unsigned it,
+__device__ inline void vg_bdigits(unsigned* kn, int h, const vg_item& I) {
+    unsigned tmp = (unsigned)h;
+#pragma unroll
+    for (int i = 0; i < DD; ++i) {
+        const unsigned q = vg_udiv(tmp, I.ns_m, I.ns_sh1, I.ns_sh2);
+        kn[i] = (tmp - q * I.ns) * (unsigned)NB;
+        tmp = q;
+    }
+}
+
+// kn of cube h + 1 from kn of cube h: the digits counted up like an odometer
+// (branch-free carry), the same values vg_bdigits gives for h + 1.
+template <int DD>
+__device__ inline void vg_bdigits_next(unsigned* kn, const vg_item& I) {
+    const unsigned top = I.ns * (unsigned)NB;
+    unsigned carry = NB;
+#pragma unroll
+    for (int i = 0; i < DD; ++i) {
+        const unsigned v = kn[i] + carry;
+        carry = v == top ? (unsigned)NB : 0u;
+        kn[i] = v == top ? 0u : v;
+    }
+}
+
+// J f of sample s of the cube with strata digits kn (vg_bdigits) of an item of
+// dimension DD; bin ids to ids[0..DD).
+template <int DD>
+__device__ inline void vg_beval(double& jf, unsigned* ids, long long s, const unsigned* kn, unsigned it,
                                 unsigned seed, unsigned stream, const vg_item& I,
                                 const vg_map_t* __restrict__ map, const vg_real* __restrict__ theta) {
     constexpr int GG = (DD + 3) / 4;
@@ -705,15 +732,11 @@ __device__ inline void vg_beval(double& jf, unsigned* ids, long long s, int h, u
         words[4 * g + 0] = v.x; words[4 * g + 1] = v.y;
         words[4 * g + 2] = v.z; words[4 * g + 3] = v.w;
     }
-    unsigned tmp = (unsigned)h;
     vg_real x[DD], jac = (vg_real)1;
 #pragma unroll
     for (int i = 0; i < DD; ++i) {
-        const unsigned q = vg_udiv(tmp, I.ns_m, I.ns_sh1, I.ns_sh2);
-        const unsigned k = tmp - q * I.ns;
-        tmp = q;
         const unsigned long long L = ((unsigned long long)words[i] * 2ull + 1ull) * (unsigned long long)NB;
-        const unsigned Q = k * (unsigned)NB + (unsigned)(L >> 33);
+        const unsigned Q = kn[i] + (unsigned)(L >> 33);
         const unsigned id = vg_udiv(Q, I.ns_m, I.ns_sh1, I.ns_sh2);
         const unsigned rem = Q - id * I.ns;
         const unsigned long long low = L & ((1ull << 33) - 1ull);
@@ -764,6 +787,8 @@ __device__ inline void vg_btask(const vg_item& I, int item, long long w0, long l
             cstart = nhs[h];
             cend = nhs[h + 1];
         }
+        unsigned kn[DD];
+        vg_bdigits<DD>(kn, h, I);
         const long long startF = cstart;
         const bool sb = active && cstart < rs;
         bool hasF = false;
@@ -777,13 +802,14 @@ __device__ inline void vg_btask(const vg_item& I, int item, long long w0, long l
                     else { jf1[h] = a1; jf2[h] = a2; }
                     ++npieces;
                     ++h;
+                    vg_bdigits_next<DD>(kn, I);
                     cstart = cend;
                     cend = nhs[h + 1];
                     a1 = 0.0; a2 = 0.0;
                 }
                 double jf;
                 unsigned ids[DD];
-                vg_beval<DD>(jf, ids, s, h, it, seed, (unsigned)item, I, map, theta);
+                vg_beval<DD>(jf, ids, s, kn, it, seed, (unsigned)item, I, map, theta);
                 const double jq = jf * jf;
 #pragma unroll
                 for (int i = 0; i < DD; ++i) {

@@ -1176,8 +1176,10 @@ struct cvg_batch {
     unsigned* counts = nullptr;
     void* theta = nullptr;
     long long* fold_list = nullptr;
-    int* fold_count = nullptr;
+    int* fold_count = nullptr;  // [0] fold list length, [1] big-run list length
     int fold_cap = 0;
+    long long* big_list = nullptr;  // run-table chunks of cubes holding many runs
+    int big_cap = 0;
     int iterations_done = 0;
     double loop_ms = 0.0, setup_ms = 0.0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1191,21 +1193,24 @@ struct cvg_batch {
         if (ev1) (void)hipEventDestroy(ev1);
         void* bufs[] = {fitems, bitems, task_item, task_w0, blk_item, rd_item, rd_dim, nh, nh_sum, JF, JF2,
                         dh, run_cube, win_first, win_last, part, dsum, bsum, boff, s_item, it_p, err,
-                        hist, x_all, map64, map32, weights, counts, theta, fold_list, fold_count};
+                        hist, x_all, map64, map32, weights, counts, theta, fold_list, fold_count, big_list};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 
     // phase 0: initial map and allocation; 1: refinement, results, reallocation.
+    // The refinement of phase 1 runs as extra blocks of the results launch.
     int steps(int phase) {
-        VG_HIP(vg_launch_batch_refine(stream, bitems, n_refine, rd_item, rd_dim, p.n_intervals, p.alpha,
-                                      phase == 0 ? 1 : 0, x_all, weights, counts, map64, map32));
+        if (phase == 0)
+            VG_HIP(vg_launch_batch_refine(stream, bitems, n_refine, rd_item, rd_dim, p.n_intervals, p.alpha, 1,
+                                          x_all, weights, counts, map64, map32));
         VG_HIP(vg_launch_batch_iteration_steps(
             stream, bitems, n_items, tot_blk, blk_item, n_refine, rd_item, rd_dim, p.n_intervals, p.alpha,
             p.beta, k_shift, p.max_it, nh, nh_sum, JF, JF2, win_first, win_last, dh, part, dsum, bsum, boff,
             s_item, run_cube, it_p, hist, x_all, weights, counts, map64, map32, err, phase, fold_list,
-            fold_count, fold_cap));
+            fold_count, fold_cap, task_item, task_w0, n_tasks, (prog->block / 64) * prog->task_w, big_list,
+            big_cap));
         return CVG_OK;
     }
 
@@ -1408,8 +1413,10 @@ int cvg_batch_create(cvg_program* prog, const cvg_params* common, const cvg_batc
         for (long long k = 0; k < nblk; ++k) blk_item.push_back(i);
         for (int dd = 0; dd < it.n_dim; ++dd) { rd_item.push_back(i); rd_dim.push_back(dd); }
         b->dims.push_back(it.n_dim);
-        b->tot_cubes += C;
-        b->tot_nhs += C + 1;
+        // item ranges padded so every item's n_h starts on a 16-byte and its prefix
+        // sums on a 16-byte boundary (vector accesses in the scan kernel)
+        b->tot_cubes += (C + 3) / 4 * 4;
+        b->tot_nhs += (C + 2) / 2 * 2;
         b->tot_win += wcap;
         b->tot_blk += nblk;
         b->tot_xe += (long long)it.n_dim * (n + 1);
@@ -1453,7 +1460,9 @@ int cvg_batch_create(cvg_program* prog, const cvg_params* common, const cvg_batc
     if ((rc = dalloc(&b->counts, b->tot_map))) return rc;
     b->fold_cap = (int)(b->tot_win / 64 + 2 * n_items + 2);
     if ((rc = dalloc(&b->fold_list, b->fold_cap))) return rc;
-    if ((rc = dalloc(&b->fold_count, 1))) return rc;
+    if ((rc = dalloc(&b->fold_count, 2))) return rc;
+    b->big_cap = (int)vg_big_list_cap(b->tot_win * 64);
+    if ((rc = dalloc(&b->big_list, 2LL * b->big_cap))) return rc;
     const int nth = n_theta > 0 ? n_theta : 1;
     if ((rc = dalloc((char**)&b->theta, 8LL * nth))) return rc;
     if (n_theta > 0) {

@@ -78,7 +78,9 @@ hipError_t vg_launch_batch_iteration_steps(hipStream_t st, const vg_bitem* items
                                            int* it_p, double* hist, double* x_all,
                                            double* weights, unsigned* counts, double2* map64,
                                            float2* map32, int* err, int phase,
-                                           long long* fold_list, int* fold_count, int fold_cap);
+                                           long long* fold_list, int* fold_count, int fold_cap,
+                                           const int* task_item, const int* task_w0, int n_tasks,
+                                           int task_windows, long long* big_list, int big_cap);
 hipError_t vg_launch_batch_refine(hipStream_t st, const vg_bitem* items, int n_refine, const int* rd_item,
                                   const int* rd_dim, int n, double alpha, int init, double* x_all,
                                   double* weights, unsigned* counts, double2* map64, float2* map32);

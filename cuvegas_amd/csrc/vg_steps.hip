@@ -99,58 +99,9 @@ __device__ inline bool vg_last_block(unsigned* cnt, int* flag) {
 }
 
 // ---------------------------------------------------------------------------
-// Per-cube results (reference `results`, cuda_wrapper.cu:26-49).  Also folds
-// the window partials of cubes that crossed sampling windows and writes the
-// complete per-cube sums back to JF / JF2.  part[3b+q]: block partial of
-// q = 0: sum I_h, 1: sum sigma_h^2 / n_h, 2: sum d_h.
+// Per-cube results (reference `results`, cuda_wrapper.cu:26-49).  part[3b+q]:
+// block partial of q = 0: sum I_h, 1: sum sigma_h^2 / n_h, 2: sum d_h.
 // ---------------------------------------------------------------------------
-__device__ inline void vg_cube_result(long long h, int C, double V, double beta,
-                                      const int* __restrict__ nh, const long long* __restrict__ nh_sum,
-                                      double* __restrict__ JF, double* __restrict__ JF2,
-                                      const double2* __restrict__ win_first,
-                                      const double2* __restrict__ win_last, int ws_shift,
-                                      double* __restrict__ dh, double& sr, double& ss, double& dd,
-                                      long long win_off = 0) {
-    sr = 0.0; ss = 0.0; dd = 0.0;
-    if (h >= C) return;
-    const int neval = nh[h];
-    const long long st = nh_sum[h], en = nh_sum[h + 1];
-    const long long wa = (st >> ws_shift) + win_off, wb = ((en - 1) >> ws_shift) + win_off;
-    double j1, j2;
-    if (wa == wb || wb - wa + 1 > VG_FOLD_MAX) {  // inside one window, or folded by vg_bigfold
-        j1 = JF[h];
-        j2 = JF2[h];
-    } else {  // left fold of the window partials; loads batched ahead of the adds
-        double2 v = win_last[wa];
-        j1 = v.x;
-        j2 = v.y;
-        long long w = wa + 1;
-        for (; w + 8 <= wb + 1; w += 8) {
-            double2 t[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) t[j] = win_first[w + j];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) { j1 = j1 + t[j].x; j2 = j2 + t[j].y; }
-        }
-        for (; w <= wb; ++w) {
-            v = win_first[w];
-            j1 = j1 + v.x;
-            j2 = j2 + v.y;
-        }
-        JF[h] = j1;
-        JF2[h] = j2;
-    }
-    const double Ih = j1 / neval * V;
-    const double Sig2 = (j2 / neval * V * V) - (Ih * Ih);
-    sr = Ih;
-    ss = Sig2 / neval;
-    const double u = V * j1 / neval;
-    double t = V * V * j2 / neval - u * u;
-    if (t < 0) t = 0;
-    dd = vg_pow(t, beta);
-    dh[h] = dd;
-}
-
 // Results of cube h from complete per-cube sums (the window folds ran before).
 __device__ inline void vg_cube_result_sums(long long h, int C, double V, double beta,
                                            const int* __restrict__ nh, const double2* __restrict__ JS,
@@ -911,26 +862,56 @@ hipError_t vg_launch_snapshot(hipStream_t st, const int* it_p, int skip, int cap
 // an item reproduces a single-integral run bit for bit.
 // ===========================================================================
 
+// Per-cube results of every item (blocks n_refine..), from complete per-cube
+// sums (vg_bbigfold_kernel folded the window partials before), and beside
+// them the map refinement of (item, dimension) rd_item[b], rd_dim[b] in block
+// b < n_refine, 256 threads, working arrays in dynamic LDS
+// (4n + 1 + n/1024 + 2 doubles).
 __global__ void __launch_bounds__(256)
 vg_bresults_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ blk_item,
-                   const int* __restrict__ nh, const long long* __restrict__ nh_sum,
-                   double* __restrict__ JF, double* __restrict__ JF2,
-                   const double2* __restrict__ win_first, const double2* __restrict__ win_last,
-                   int ws_shift, double beta, double* __restrict__ dh, double* __restrict__ part) {
+                   const int* __restrict__ nh, const double* __restrict__ JF,
+                   const double* __restrict__ JF2, double beta, double* __restrict__ dh,
+                   double* __restrict__ part, int n_refine, const int* __restrict__ rd_item,
+                   const int* __restrict__ rd_dim, int n, double alpha, double* __restrict__ x_all,
+                   double* __restrict__ weights, unsigned* __restrict__ counts,
+                   double2* __restrict__ map64, float2* __restrict__ map32) {
     __shared__ double red[256];
-    const vg_bitem I = items[blk_item[blockIdx.x]];
-    const long long lb = blockIdx.x - I.blk_off;
-    const long long base = lb * VG_TREE + threadIdx.x;
+    extern __shared__ double dyn[];
+    if ((int)blockIdx.x < n_refine) {
+        const vg_bitem R = items[rd_item[blockIdx.x]];
+        const int dim = rd_dim[blockIdx.x];
+        const long long mo = R.map_off + (long long)dim * n;
+        vg_refine_dim<256>(n, alpha, x_all + R.xe_off + (long long)dim * (n + 1), weights + mo, counts + mo,
+                           vg_refine_ws_at(dyn, n), map64 + mo, map32 + mo, red);
+        return;
+    }
+    const long long bb = blockIdx.x - n_refine;
+    const vg_bitem I = items[blk_item[bb]];
+    const long long base = (bb - I.blk_off) * VG_TREE + threadIdx.x;
     double r[4][3];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-        vg_cube_result(base + 256 * j, I.C, I.V, beta, nh + I.cube_off, nh_sum + I.nhs_off,
-                       JF + I.cube_off, JF2 + I.cube_off, win_first, win_last, ws_shift,
-                       dh + I.cube_off, r[j][0], r[j][1], r[j][2], I.win_off);
+    for (int j = 0; j < 4; ++j) {
+        const long long h = base + 256 * j;
+        double sr = 0.0, ss = 0.0, dd = 0.0;
+        if (h < I.C) {
+            const int neval = nh[I.cube_off + h];
+            const double j1 = JF[I.cube_off + h], j2 = JF2[I.cube_off + h];
+            const double Ih = j1 / neval * I.V;
+            const double Sig2 = (j2 / neval * I.V * I.V) - (Ih * Ih);
+            sr = Ih;
+            ss = Sig2 / neval;
+            const double u = I.V * j1 / neval;
+            double t = I.V * I.V * j2 / neval - u * u;
+            if (t < 0) t = 0;
+            dd = vg_pow(t, beta);
+            dh[I.cube_off + h] = dd;
+        }
+        r[j][0] = sr; r[j][1] = ss; r[j][2] = dd;
+    }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const double tot = vg_tree1024_256(r[0][q], r[1][q], r[2][q], r[3][q], red);
-        if (threadIdx.x == 0) part[3 * blockIdx.x + q] = tot;
+        if (threadIdx.x == 0) part[3 * bb + q] = tot;
     }
 }
 
@@ -1020,7 +1001,7 @@ __global__ void __launch_bounds__(VG_TREE)
 vg_bscan_blocks_kernel(const vg_bitem* __restrict__ items, const long long* __restrict__ bsum,
                        long long* __restrict__ boff, long long* __restrict__ s_item,
                        int* __restrict__ it_p, int advance, int* __restrict__ err,
-                       int* __restrict__ fold_count) {
+                       int* __restrict__ counts) {
     __shared__ long long wt[VG_TREE / 64];
     const int item = blockIdx.x;
     const vg_bitem I = items[item];
@@ -1039,20 +1020,60 @@ vg_bscan_blocks_kernel(const vg_bitem* __restrict__ items, const long long* __re
     __syncthreads();
     if (item == 0 && threadIdx.x == 0) {
         if (advance) *it_p = *it_p + 1;
-        *fold_count = 0;
+        counts[0] = 0;
+        counts[1] = 0;
     }
 }
 
-// Batched counterpart of vg_bigfold_kernel: list entries are (item << 32) | cube.
+// Batched counterpart of vg_bigfold_kernel.  Blocks [0, VG_BIGFOLD_BLOCKS):
+// cubes spanning more than VG_FOLD_MAX windows, list entries (item << 32) |
+// cube, tree fold.  The other blocks: cubes spanning 2..VG_FOLD_MAX windows,
+// found at window starts over all items' windows (walking the sampling
+// kernel's task list, (item, first window) per task_windows windows): the cube
+// holding the first sample of window w crosses from window w - 1 when it
+// starts there; its partials are left-folded in window order into JF / JF2.
 __global__ void __launch_bounds__(256)
-vg_bbigfold_kernel(const vg_bitem* __restrict__ items, const long long* __restrict__ list,
-                   const int* __restrict__ count, int cap, const long long* __restrict__ nh_sum,
-                   const double2* __restrict__ win_first, const double2* __restrict__ win_last,
-                   int ws_shift, double* __restrict__ JF, double* __restrict__ JF2) {
+vg_bbigfold_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ task_item,
+                   const int* __restrict__ task_w0, int n_tasks, int task_windows,
+                   const long long* __restrict__ list, const int* __restrict__ count, int cap,
+                   const long long* __restrict__ nh_sum, const int* __restrict__ run_cube_all,
+                   const long long* __restrict__ s_item, const double2* __restrict__ win_first,
+                   const double2* __restrict__ win_last, int ws_shift, double* __restrict__ JF,
+                   double* __restrict__ JF2) {
     __shared__ double red[256];
     __shared__ double part[2 * 2048];
+    if (blockIdx.x >= VG_BIGFOLD_BLOCKS) {
+        const long long stride = (long long)(gridDim.x - VG_BIGFOLD_BLOCKS) * blockDim.x;
+        const long long nwt = (long long)n_tasks * task_windows;
+        for (long long g = (long long)(blockIdx.x - VG_BIGFOLD_BLOCKS) * blockDim.x + threadIdx.x; g < nwt;
+             g += stride) {
+            const long long task = g / task_windows;
+            const int item = task_item[task];
+            const long long w = task_w0[task] + g % task_windows;
+            const long long nwin = (s_item[item] + (1LL << ws_shift) - 1) >> ws_shift;
+            if (w < 1 || w >= nwin) continue;
+            const vg_bitem I = items[item];
+            const long long gw = I.win_off + w;
+            const long long* nhs = nh_sum + I.nhs_off;
+            const int h = run_cube_all[gw << 6];
+            const long long st = nhs[h];
+            if ((st >> ws_shift) != w - 1) continue;  // starts at window w, or folded from an earlier one
+            const long long wb = (nhs[h + 1] - 1) >> ws_shift;
+            if (wb - (w - 1) + 1 > VG_FOLD_MAX) continue;  // tree fold
+            double2 v = win_last[gw - 1];
+            double j1 = v.x, j2 = v.y;
+            for (long long u = w; u <= wb; ++u) {
+                v = win_first[I.win_off + u];
+                j1 = j1 + v.x;
+                j2 = j2 + v.y;
+            }
+            JF[I.cube_off + h] = j1;
+            JF2[I.cube_off + h] = j2;
+        }
+        return;
+    }
     const int n = *count < cap ? *count : cap;
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    for (int i = blockIdx.x; i < n; i += VG_BIGFOLD_BLOCKS) {
         const int item = (int)(list[i] >> 32);
         const int h = (int)(list[i] & 0xffffffffLL);
         const vg_bitem I = items[item];
@@ -1065,15 +1086,23 @@ vg_bbigfold_kernel(const vg_bitem* __restrict__ items, const long long* __restri
 }
 
 // Batched counterpart of vg_scan_cubes_kernel: block b scans 1024 cubes of
-// item blk_item[b]; the item's run table is filled run-centrically as there.
+// item blk_item[b] (the items' cube ranges are not aligned for vector
+// accesses).  The block's capped cube starts go to LDS and its runs are spread
+// over the threads, each finding its cube by a binary search that starts from
+// its previous run's cube, so the run table is written in consecutive order.
+// Cubes holding more than VG_OWN_RUNS runs go to big_list (absolute run
+// indices, written by vg_big_runs_kernel) and the threads jump past their
+// runs.  counts[0]: fold list length, counts[1]: big list length.
 __global__ void __launch_bounds__(256)
 vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ blk_item,
                       const int* __restrict__ nh_all, const long long* __restrict__ boff,
                       const long long* __restrict__ s_item, int k_shift,
                       long long* __restrict__ nh_sum_all, int* __restrict__ run_cube_all,
-                      long long* __restrict__ fold_list, int* __restrict__ fold_count, int fold_cap) {
+                      long long* __restrict__ fold_list, int* __restrict__ counts, int fold_cap,
+                      long long* __restrict__ big_list, int big_cap) {
     __shared__ long long wt[4];
     __shared__ long long es[VG_TREE + 1];  // capped start of the block's cubes; es[nc] = end
+    __shared__ long long slot0;
     const int item = blk_item[blockIdx.x];
     const vg_bitem I = items[item];
     const int* nh = nh_all + I.cube_off;
@@ -1084,13 +1113,18 @@ vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict_
     const long long base = (blockIdx.x - I.blk_off) * VG_TREE;
     const int nc = C - base < VG_TREE ? (int)(C - base) : VG_TREE;
     const long long h0 = base + 4 * threadIdx.x;
+    const bool full = h0 + 4 <= C;  // item ranges are padded: 16-byte load, 2 x 16-byte stores
     long long v[4];
     long long own = 0;
+    if (full) {
+        const int4 q = *reinterpret_cast<const int4*>(nh + h0);
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        v[j] = h0 + j < C ? nh[h0 + j] : 0;
-        own += v[j];
+        for (int j = 0; j < 4; ++j) v[j] = h0 + j < C ? nh[h0 + j] : 0;
     }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) own += v[j];
     long long total;
     const long long S = s_item[item];
     const long long ex = vg_block_scan_excl(own, wt, &total) + boff[blockIdx.x];
@@ -1100,43 +1134,82 @@ vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict_
     for (int j = 0; j < 4; ++j) e[j + 1] = e[j] + v[j];
 #pragma unroll
     for (int j = 0; j < 5; ++j) e[j] = e[j] < S ? e[j] : S;
+    const long long kmask = (1LL << k_shift) - 1;
+    long long r0[4], nr[4];
+    long long nchunk = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        r0[j] = (e[j] + kmask) >> k_shift;
+        long long r1 = (e[j + 1] + kmask) >> k_shift;
+        if (r1 > run_cap) r1 = run_cap;
+        nr[j] = h0 + j < C ? r1 - r0[j] : 0;
+        if (nr[j] > VG_OWN_RUNS) nchunk += (nr[j] + VG_BIG_CHUNK - 1) / VG_BIG_CHUNK;
+    }
+    long long nblock;
+    long long slot = vg_block_scan_excl(nchunk, wt, &nblock);
+    if (nblock > 0) {
+        if (threadIdx.x == 0) slot0 = atomicAdd(counts + 1, (int)nblock);
+        __syncthreads();
+        slot += slot0;
+    }
     const int i0 = 4 * threadIdx.x;
+    if (full) {
+        longlong2* o = reinterpret_cast<longlong2*>(nh_sum + h0);
+        o[0] = make_longlong2(e[0], e[1]);
+        o[1] = make_longlong2(e[2], e[3]);
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const long long h = h0 + j;
         if (h < C) {
-            nh_sum[h] = e[j];
+            if (!full) nh_sum[h] = e[j];
             if (h == C - 1) nh_sum[C] = S;
             es[i0 + j] = e[j];
             if (i0 + j == nc - 1) es[nc] = e[j + 1];
+            const long long n = nr[j];
+            if (n > VG_OWN_RUNS) {
+                for (long long c = 0; c * VG_BIG_CHUNK < n; ++c, ++slot) {
+                    const long long m = n - c * VG_BIG_CHUNK < VG_BIG_CHUNK ? n - c * VG_BIG_CHUNK : VG_BIG_CHUNK;
+                    const long long rc = r0[j] + c * VG_BIG_CHUNK;
+                    if (slot < big_cap) {
+                        big_list[2 * slot] = I.win_off * 64 + rc;
+                        big_list[2 * slot + 1] = (m << 32) | h;
+                    } else {  // never with big_cap = vg_big_list_cap(total runs); kept correct anyway
+                        for (long long r = rc; r < rc + m; ++r) run_cube[r] = (int)h;
+                    }
+                }
+            }
             const long long st = e[j], en = e[j + 1];
             if (en > st && ((en - 1) >> (k_shift + 6)) - (st >> (k_shift + 6)) + 1 > VG_FOLD_MAX) {
-                const int slot = atomicAdd(fold_count, 1);
-                if (slot < fold_cap) fold_list[slot] = ((long long)item << 32) | (long long)h;
+                const int fs = atomicAdd(counts, 1);
+                if (fs < fold_cap) fold_list[fs] = ((long long)item << 32) | (long long)h;
             }
         }
     }
     __syncthreads();
-    const long long kmask = (1LL << k_shift) - 1;
-    const long long r0 = (es[0] + kmask) >> k_shift;
-    long long r1 = (es[nc] + kmask) >> k_shift;
-    if (r1 > run_cap) r1 = run_cap;
-    // The thread's runs increase, so its cube index only grows: a run still
-    // inside the previous run's cube costs one LDS read; otherwise binary
-    // search for the largest i in (lo, nc) with es[i] <= s.  es[lo] <= s holds
-    // throughout (es[0] <= first sample of run r0).
-    int lo = 0;
-    for (long long r = r0 + threadIdx.x; r < r1; r += 256) {
+    const long long rb0 = (es[0] + kmask) >> k_shift;
+    long long rb1 = (es[nc] + kmask) >> k_shift;
+    if (rb1 > run_cap) rb1 = run_cap;
+    int lo = 0;  // the thread's runs increase, so its cube index only grows
+    for (long long r = rb0 + threadIdx.x; r < rb1;) {
         const long long s = r << k_shift;
         if (lo + 1 < nc && es[lo + 1] <= s) {
-            int a = lo + 1, b = nc;
+            int a = lo + 1, b = nc;  // largest i in (lo, nc) with es[i] <= s
             while (b - a > 1) {
                 const int mid = (a + b) >> 1;
                 if (es[mid] <= s) a = mid; else b = mid;
             }
             lo = a;
         }
+        const long long cr0 = (es[lo] + kmask) >> k_shift;
+        long long cr1 = (es[lo + 1] + kmask) >> k_shift;
+        if (cr1 > run_cap) cr1 = run_cap;
+        if (cr1 - cr0 > VG_OWN_RUNS) {  // listed: jump to this thread's first run past the cube
+            r += ((cr1 - r + 255) >> 8) << 8;
+            continue;
+        }
         run_cube[r] = (int)(base + lo);
+        r += 256;
     }
 }
 
@@ -1176,15 +1249,22 @@ hipError_t vg_launch_batch_iteration_steps(hipStream_t st, const vg_bitem* items
                                            int* it_p, double* hist, double* x_all,
                                            double* weights, unsigned* counts, double2* map64,
                                            float2* map32, int* err, int phase,
-                                           long long* fold_list, int* fold_count, int fold_cap) {
+                                           long long* fold_list, int* fold_count, int fold_cap,
+                                           const int* task_item, const int* task_w0, int n_tasks,
+                                           int task_windows, long long* big_list, int big_cap) {
     if (phase == 0) {  // initial allocation (the map is initialised by vg_launch_batch_refine)
         hipLaunchKernelGGL(vg_balloc_kernel, dim3((unsigned)nblk), dim3(256), 0, st, items, blk_item, 0,
                            dh, dsum, nh, bsum);
     } else {
-        hipLaunchKernelGGL(vg_bbigfold_kernel, dim3(128), dim3(256), 0, st, items, fold_list, fold_count,
-                           fold_cap, nh_sum, win_first, win_last, 6 + k_shift, JF, JF2);
-        hipLaunchKernelGGL(vg_bresults_kernel, dim3((unsigned)nblk), dim3(256), 0, st, items, blk_item,
-                           nh, nh_sum, JF, JF2, win_first, win_last, 6 + k_shift, beta, dh, part);
+        const long long wblocks = ((long long)n_tasks * task_windows + 255) / 256;
+        hipLaunchKernelGGL(vg_bbigfold_kernel, dim3((unsigned)(VG_BIGFOLD_BLOCKS + wblocks)), dim3(256), 0, st,
+                           items, task_item,
+                           task_w0, n_tasks, task_windows, fold_list, fold_count, fold_cap, nh_sum, run_cube,
+                           s_item, win_first, win_last, 6 + k_shift, JF, JF2);
+        const size_t lds = sizeof(double) * (size_t)(4LL * n + 1 + n / VG_TREE + 2);
+        hipLaunchKernelGGL(vg_bresults_kernel, dim3((unsigned)(nblk + n_refine)), dim3(256), lds, st, items,
+                           blk_item, nh, JF, JF2, beta, dh, part, n_refine, rd_item, rd_dim, n, alpha, x_all,
+                           weights, counts, map64, map32);
         hipLaunchKernelGGL(vg_bfinalize_kernel, dim3(n_items), dim3(256), 0, st, items, part, s_item,
                            it_p, hist_cap, hist, dsum, err);
         hipLaunchKernelGGL(vg_balloc_kernel, dim3((unsigned)nblk), dim3(256), 0, st, items, blk_item, 1,
@@ -1195,7 +1275,10 @@ hipError_t vg_launch_batch_iteration_steps(hipStream_t st, const vg_bitem* items
     hipLaunchKernelGGL(vg_bscan_blocks_kernel, dim3(n_items), dim3(VG_TREE), 0, st, items, bsum, boff,
                        s_item, it_p, phase == 0 ? 0 : 1, err, fold_count);
     hipLaunchKernelGGL(vg_bscan_cubes_kernel, dim3((unsigned)nblk), dim3(256), 0, st, items, blk_item, nh,
-                       boff, s_item, k_shift, nh_sum, run_cube, fold_list, fold_count, fold_cap);
+                       boff, s_item, k_shift, nh_sum, run_cube, fold_list, fold_count, fold_cap, big_list,
+                       big_cap);
+    hipLaunchKernelGGL(vg_big_runs_kernel, dim3(2048), dim3(256), 0, st, big_list, fold_count, big_cap,
+                       run_cube);
     return hipGetLastError();
 }
 
