@@ -19,7 +19,7 @@ from typing import Dict, Sequence, Tuple
 from .batch import Batch, BatchProgram
 
 DEFAULT_CANDIDATES: Tuple[Tuple[int, int, int], ...] = (
-    (256, 16, 8), (512, 16, 8), (1024, 16, 4), (512, 16, 2), (512, 32, 4), (512, 8, 8))
+    (256, 16, 4), (256, 16, 8), (512, 16, 2), (512, 32, 2), (256, 32, 4), (1024, 16, 2))
 
 _PROGRAMS: Dict[tuple, BatchProgram] = {}
 

@@ -656,10 +656,13 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
 // where theta points at the item's parameters.  Item i draws from Philox
 // stream i, so it reproduces a single-integral run with stream_id = i.
 //
-// Work is a static list of tasks (item, chunk of VG_BLOCK/64 windows); a block
-// takes whole tasks, so its LDS bins belong to one item at a time and are
-// merged into the item's global bins at the end of the task.  Windows past the
-// item's sample count this iteration are skipped.
+// Work is a static list of tasks (item, chunk of VG_TASKW * VG_BLOCK/64
+// windows); a block takes whole tasks, so its LDS bins belong to one item at a
+// time and are merged into the item's global bins at the end of the task.  The
+// host launches one block per task by default (the hardware then hands tasks
+// to CUs as they free up, which balances tasks of unequal cost) or a resident
+// grid that strides over the list.  Windows past the item's sample count this
+// iteration are skipped.
 // ---------------------------------------------------------------------------
 // Item descriptor; the host writes the same 80-byte layout (vg_host.cpp).
 struct vg_item {

@@ -88,7 +88,7 @@ struct cvg_program {
     int d = 0, nbins = 0, nstrat = 0, precision = 0, K = kDefaultRun, has_domain = 0;
     int block = kFillBlock, lds_bins = 0, lds_map = 0;
     int has_grad = 0, n_theta = 0, bin32 = 0, pair = 1, pipe = 0;
-    int batch = 0, max_dim = 0, task_w = 8;
+    int batch = 0, max_dim = 0, task_w = 8, fill_grid = 1;
     std::string log;
     std::vector<char> code;
     double compile_ms = 0.0;
@@ -1217,8 +1217,11 @@ struct cvg_batch {
     int fill() {
         int ncu = 0;
         VG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+        // fill_grid 1: one block per task, so the hardware hands tasks (whose
+        // cost varies with the item's dimension and integrand) to CUs as they
+        // free up; 0: a resident grid striding over the task list
         unsigned grid = (unsigned)(ncu * prog->fill_blocks_per_cu);
-        if ((long long)grid > n_tasks) grid = (unsigned)n_tasks;
+        if (prog->fill_grid || (long long)grid > n_tasks) grid = (unsigned)n_tasks;
         const void* map = p.precision == CVG_FP32C ? (const void*)map32 : (const void*)map64;
         unsigned seed = p.seed;
         int nt = n_tasks;
@@ -1277,6 +1280,9 @@ int cvg_batch_program_create(const char* integrand_src, int max_dim, const cvg_p
             } else if (key == "task_windows") {
                 if (val < 1 || val > 64) return fail(CVG_ERR_INVALID, "task_windows must be in 1..64");
                 prog->task_w = val;
+            } else if (key == "fill_grid") {
+                if (val < 0 || val > 1) return fail(CVG_ERR_INVALID, "fill_grid must be 0 or 1");
+                prog->fill_grid = val;
             } else {
                 rtc_extra += tok + " ";
             }
