@@ -451,14 +451,23 @@ vg_dump(const vg_map_t* __restrict__ gmap, const long long* __restrict__ nh_sum,
 // differences.  part[block][2 + 2P]: fixed-order block partials.
 // ---------------------------------------------------------------------------
 constexpr int NT = VG_NTHETA;
-// Cubes with more samples than this are replayed by the whole wave.
-// Measured on the 6-D mixture (1e6 cubes, n_h from 2 to ~3,000): 16 -> 8.8 ms,
-// 32 -> 7.4, 128 -> 6.0, 256 -> 5.8, 512 -> 6.9, no wave path -> 18 ms (FP32
-// backward); each wave-replayed cube pays 66 wave sums.
-constexpr int VG_GRAD_LANE_MAX = 128;
-// FP32 tables of up to 64 KB are staged in LDS (two 256-thread blocks per CU
-// still fit); FP64 tables are read through L1/L2.
-constexpr bool kGradLdsMap = VG_FP32 && (long long)D * NB * 8 <= 65536;
+// Cubes with more samples than this are replayed by the whole wave (samples
+// strided over the lanes); smaller ones by one lane each.
+#ifndef VG_GRAD_LMAX
+#define VG_GRAD_LMAX 16
+#endif
+constexpr int VG_GRAD_LANE_MAX = VG_GRAD_LMAX;
+// The map is read through L1/L2 by default: the LDS holds the waves' transpose
+// buffers (VG_GRAD_LDS_MAP=1 stages FP32 tables of up to 64 KB in LDS too).
+#ifndef VG_GRAD_LDS_MAP
+#define VG_GRAD_LDS_MAP 0
+#endif
+constexpr bool kGradLdsMap = VG_GRAD_LDS_MAP && VG_FP32 && (long long)D * NB * 8 <= 65536;
+// Transposed wave sums of NT + 1 values (one value term, NT gradient terms)
+// go through a per-wave LDS buffer of 64 rows of VG_TW doubles, in VG_NCH
+// column chunks.
+constexpr int VG_TW = NT + 1 < 33 ? NT + 1 : 33;
+constexpr int VG_NCH = (NT + 1 + VG_TW - 1) / VG_TW;
 
 // J f and J df/dtheta (FP64) of sample s of cube h under the frozen map.
 __device__ inline void vg_grad_sample(long long h, long long s, unsigned it, const vg_map_t* __restrict__ gmap,
@@ -533,38 +542,58 @@ __device__ inline void vg_grad_cube(int n, double V, double& sf, double& sf2, do
     }
 }
 
-// Wave 64 sum, the same value in every lane: fixed butterfly order, so the
-// result does not depend on timing.
-__device__ inline double vg_wave_sum(double v) {
+// Sums over the wave's 64 lanes of the NT + 1 values (a0, g[0..NT)), value
+// k = 0 being a0 and k = 1 + p being g[p], through the per-wave LDS buffer
+// tb[64][VG_TW]: for each chunk c of VG_TW values, every lane writes its row,
+// then lane q < VG_TW sums column q (value k = c * VG_TW + q) over the lanes in
+// a fixed order (four interleaved partial sums), so the result does not
+// depend on timing.  r[c] receives that sum (0 where k > NT or q >= VG_TW).
+// Costs VG_TW row writes and 64 column reads per lane and chunk, where a
+// butterfly costs 6 shuffle steps per value.
+__device__ inline void vg_wave_tsum(double* tb, int lane, double a0, const double* g, double* r) {
+    double* row = tb + lane * VG_TW;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Adds (wave-uniform) values to the wave's accumulator in LDS (lane 0 writes).
-__device__ inline void vg_wacc_add(double* wacc, int lane, double sf, double sf2, const double* g1,
-                                   const double* g2) {
-    if (lane == 0) {
-        wacc[0] = wacc[0] + sf;
-        wacc[1] = wacc[1] + sf2;
+    for (int c = 0; c < VG_NCH; ++c) {
 #pragma unroll
-        for (int p = 0; p < NT; ++p) {
-            wacc[2 + p] = wacc[2 + p] + g1[p];
-            wacc[2 + NT + p] = wacc[2 + NT + p] + g2[p];
+        for (int q = 0; q < VG_TW; ++q) {
+            const int k = c * VG_TW + q;
+            if (k <= NT) row[q] = k == 0 ? a0 : g[k - 1];
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double t = 0.0;
+        if (lane < VG_TW && c * VG_TW + lane <= NT) {
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+#pragma unroll 4
+            for (int l = 0; l < 64; l += 4) {
+                t0 = t0 + tb[(l + 0) * VG_TW + lane];
+                t1 = t1 + tb[(l + 1) * VG_TW + lane];
+                t2 = t2 + tb[(l + 2) * VG_TW + lane];
+                t3 = t3 + tb[(l + 3) * VG_TW + lane];
+            }
+            t = (t0 + t1) + (t2 + t3);
+        }
+        r[c] = t;
+        // the rows are rewritten only after every lane has read its column
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
 // Each wave takes groups of 64 consecutive cubes (grid-stride), lane l owning
 // cube group + l.  Cubes of more than VG_GRAD_LANE_MAX samples are replayed
-// first, by the whole wave, one after another (samples strided over the lanes,
-// fixed butterfly sums); then every other cube by its own lane, and the 64
-// lanes' contributions are summed across the wave.  Both go into the wave's
-// LDS accumulator in that fixed order, so only one set of per-cube sums is
-// live at a time (register pressure sets the occupancy of this kernel).  The
-// allocation puts most cubes at n_h = 2 and a few at thousands of samples,
-// which one lane per cube would leave mostly idle.  Two waves per SIMD (256
-// VGPRs, a few dozen spilled) measured 12% faster than one wave without spills.
+// first, by the whole wave, one after another (samples strided over the
+// lanes); the lanes' partial sums are combined by two transposed sums, after
+// which the lane holding value k applies the cube expressions to it.
+// Then every other cube is replayed by its own lane, and the 64 lanes'
+// contributions are summed across the wave the same way.  Both go into the
+// wave's LDS accumulator (entry 0, 1: value terms; 2 + p, 2 + NT + p: gradient
+// terms, each written by one lane) in that fixed order.  Small cubes dominate
+// the allocation (most n_h = 2, a few thousand samples), and one lane per cube
+// leaves a lane idle while the wave's largest own cube finishes, so the lane
+// cap trades idle lanes against the per-cube cost of the wave path.
 extern "C" __global__ void __launch_bounds__(256, 2)
 vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
         const long long* __restrict__ nh_sum, int C, unsigned it, double V,
@@ -572,6 +601,7 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
         unsigned seed, unsigned stream, double* __restrict__ part) {
     constexpr int NV = 2 + 2 * NT;
     __shared__ double wacc_all[4][NV];
+    __shared__ double tbuf[4][64 * VG_TW];
     __shared__ vg_map_t lmap[kGradLdsMap ? D * NB : 1];
     const vg_map_t* map = gmap;
     if (kGradLdsMap) {
@@ -582,6 +612,7 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     double* wacc = wacc_all[wid];
+    double* tb = tbuf[wid];
     for (int q = lane; q < NV; q += 64) wacc[q] = 0.0;
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
@@ -607,12 +638,28 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
 #pragma unroll
                 for (int p = 0; p < NT; ++p) { g1[p] = g1[p] + jg[p]; g2[p] = fma(jg[p], jg[p], g2[p]); }
             }
-            sf = vg_wave_sum(sf);
-            sf2 = vg_wave_sum(sf2);
+            // the cube's sums of J f and (J f)^2 (value k = 0), of J df_p and
+            // (J df_p)^2 (k = 1 + p); the lane holding k applies the cube
+            // expressions to it
+            double c1[VG_NCH], c2[VG_NCH];
+            vg_wave_tsum(tb, lane, sf, g1, c1);
+            vg_wave_tsum(tb, lane, sf2, g2, c2);
 #pragma unroll
-            for (int p = 0; p < NT; ++p) { g1[p] = vg_wave_sum(g1[p]); g2[p] = vg_wave_sum(g2[p]); }
-            vg_grad_cube(nb, V, sf, sf2, g1, g2);
-            vg_wacc_add(wacc, lane, sf, sf2, g1, g2);
+            for (int c = 0; c < VG_NCH; ++c) {
+                const int k = c * VG_TW + lane;
+                if (lane >= VG_TW || k > NT) continue;
+                if (k == 0) {
+                    const double Ih = c1[c] / nb * V;
+                    const double Sig2 = (c2[c] / nb * V * V) - (Ih * Ih);
+                    wacc[0] = wacc[0] + Ih;
+                    wacc[1] = wacc[1] + Sig2 / nb;
+                } else {
+                    const double Gh = c1[c] * (V / nb);
+                    const double Gv = c2[c] * (V * V / nb) - Gh * Gh;
+                    wacc[1 + k] = wacc[1 + k] + Gh;
+                    wacc[1 + NT + k] = wacc[1 + NT + k] + Gv * (1.0 / (nb - 1));
+                }
+            }
         }
         double sf = 0.0, sf2 = 0.0, g1[NT], g2[NT];
 #pragma unroll
@@ -630,11 +677,22 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
             vg_grad_cube(n, V, sf, sf2, g1, g2);
         }
         if (__ballot(own)) {
-            sf = vg_wave_sum(sf);
-            sf2 = vg_wave_sum(sf2);
+            // sums of I_h and sigma_h^2 / n_h (k = 0), of G_h,p and Var_h,p (k = 1 + p)
+            double c1[VG_NCH], c2[VG_NCH];
+            vg_wave_tsum(tb, lane, sf, g1, c1);
+            vg_wave_tsum(tb, lane, sf2, g2, c2);
 #pragma unroll
-            for (int p = 0; p < NT; ++p) { g1[p] = vg_wave_sum(g1[p]); g2[p] = vg_wave_sum(g2[p]); }
-            vg_wacc_add(wacc, lane, sf, sf2, g1, g2);
+            for (int c = 0; c < VG_NCH; ++c) {
+                const int k = c * VG_TW + lane;
+                if (lane >= VG_TW || k > NT) continue;
+                if (k == 0) {
+                    wacc[0] = wacc[0] + c1[c];
+                    wacc[1] = wacc[1] + c2[c];
+                } else {
+                    wacc[1 + k] = wacc[1 + k] + c1[c];
+                    wacc[1 + NT + k] = wacc[1 + NT + k] + c2[c];
+                }
+            }
         }
     }
     // block partials: the waves' accumulators in wave order

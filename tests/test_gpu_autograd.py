@@ -55,6 +55,34 @@ def test_replay_matches_oracle(orc, prec):
     assert abs(v - I_k[2]) <= 1e-12 * abs(I_k[2]) + (1e-6 * abs(I_k[2]) if prec == "fp32" else 0)
 
 
+@pytest.mark.parametrize("d", [2, 9])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_replay_matches_oracle_other_dims(orc, prec, d):
+    """The replay for other parameter counts (2-D: 16 parameters; 9-D: 44,
+    more than one chunk of the kernel's transposed wave sums) against the
+    oracle's replay of the same frozen iteration."""
+    from cuvegas_amd import Integrator
+    from cuvegas_amd import integrands as I
+    rng = np.random.default_rng(10 + d)
+    th = np.concatenate([rng.uniform(0.5, 1.5, 4), rng.uniform(0.05, 0.15, 4), rng.uniform(0.2, 0.8, 4 * d)])
+    _say("grad-oracle-dims", prec, d)
+    ns, nb, N = (40, 64, 40_000) if d == 2 else (2, 32, 60_000)
+    st = Integrator(I.gaussian_mixture_param(d, th).source, d, N, n_intervals=nb, n_strat=ns, max_it=3,
+                    skip=2, seed=5, precision=prec, theta=th, options="grad")
+    st.set_record(True)
+    st.run(3)
+    v, vv, g, gv = st.backward()
+    x, nh = st.snapshot(2)
+    p, keep = orc.make_params(d, nb, ns, N, 3, orc.F_MIXTURE, list(th), skip=2, seed=5,
+                              precision=0 if prec == "fp64" else 1)
+    oI, oS2, og, ogv = orc.grad_replay(p, keep, x, nh, 2, th)
+    tol = 1e-11 if prec == "fp64" else 2e-5
+    assert abs(v - oI) <= tol * abs(oI)
+    assert abs(vv - oS2) <= 1e-6 * abs(oS2) + 1e-300
+    assert np.max(np.abs(g - og) / (np.abs(og) + 1e-12 * np.abs(og).max())) < tol * 10
+    assert np.max(np.abs(gv - ogv) / (np.abs(ogv) + 1e-30)) < 1e-4
+
+
 def test_gradient_vs_finite_differences():
     """d value / d theta of the frozen estimator vs central differences (h = 1e-4)."""
     th = _theta()
