@@ -454,7 +454,7 @@ constexpr int NT = VG_NTHETA;
 // Cubes with more samples than this are replayed by the whole wave (samples
 // strided over the lanes); smaller ones by one lane each.
 #ifndef VG_GRAD_LMAX
-#define VG_GRAD_LMAX 16
+#define VG_GRAD_LMAX 8
 #endif
 constexpr int VG_GRAD_LANE_MAX = VG_GRAD_LMAX;
 // The map is read through L1/L2 by default: the LDS holds the waves' transpose
@@ -582,26 +582,38 @@ __device__ inline void vg_wave_tsum(double* tb, int lane, double a0, const doubl
     }
 }
 
-// Each wave takes groups of 64 consecutive cubes (grid-stride), lane l owning
-// cube group + l.  Cubes of more than VG_GRAD_LANE_MAX samples are replayed
-// first, by the whole wave, one after another (samples strided over the
-// lanes); the lanes' partial sums are combined by two transposed sums, after
-// which the lane holding value k applies the cube expressions to it.
-// Then every other cube is replayed by its own lane, and the 64 lanes'
-// contributions are summed across the wave the same way.  Both go into the
-// wave's LDS accumulator (entry 0, 1: value terms; 2 + p, 2 + NT + p: gradient
-// terms, each written by one lane) in that fixed order.  Small cubes dominate
-// the allocation (most n_h = 2, a few thousand samples), and one lane per cube
-// leaves a lane idle while the wave's largest own cube finishes, so the lane
-// cap trades idle lanes against the per-cube cost of the wave path.
-extern "C" __global__ void __launch_bounds__(256, 2)
+// Work units of the backward launch: the cubes are cut into chunks of
+// rows_per_unit rows of 64, and each chunk into VG_GRAD_SPLIT units; unit
+// (chunk c, slot s) replays the chunk's wave-replayed cubes (more than
+// VG_GRAD_LANE_MAX samples) whose index among them is s mod VG_GRAD_SPLIT, and
+// slot 0 also the chunk's other cubes.  Large cubes cluster where the
+// integrand peaks, so spreading them over the slots keeps the costliest unit
+// near the cost of a few large cubes.
+#ifndef VG_GRAD_SPLIT
+#define VG_GRAD_SPLIT 4
+#endif
+
+// One wave (one block) per work unit; the hardware hands units to SIMDs as
+// they free up.  Within a row, each wave-replayed cube is replayed by the
+// whole wave (samples strided over the lanes) and its lanes' partial sums are
+// combined by two transposed sums, after which the lane holding value k
+// applies the cube expressions to it; then the row's other cubes are
+// replayed one per lane and their contributions summed across the wave the
+// same way.  Everything goes into the unit's LDS accumulator (entry 0, 1:
+// value terms; 2 + p, 2 + NT + p: gradient terms, each written by one lane)
+// in that fixed order, and the unit writes it to part[unit][2 + 2 NT];
+// vg_grad_reduce sums the units in index order.
+#ifndef VG_GRAD_WPE
+#define VG_GRAD_WPE 2
+#endif
+extern "C" __global__ void __launch_bounds__(64, VG_GRAD_WPE)
 vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
-        const long long* __restrict__ nh_sum, int C, unsigned it, double V,
+        const long long* __restrict__ nh_sum, int C, int rows_per_unit, unsigned it, double V,
         const double* __restrict__ domain, double domain_vol, const vg_real* __restrict__ theta,
         unsigned seed, unsigned stream, double* __restrict__ part) {
     constexpr int NV = 2 + 2 * NT;
-    __shared__ double wacc_all[4][NV];
-    __shared__ double tbuf[4][64 * VG_TW];
+    __shared__ double wacc[NV];
+    __shared__ double tb[64 * VG_TW];
     __shared__ vg_map_t lmap[kGradLdsMap ? D * NB : 1];
     const vg_map_t* map = gmap;
     if (kGradLdsMap) {
@@ -609,21 +621,22 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
         __syncthreads();
         map = lmap;
     }
-    const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
-    double* wacc = wacc_all[wid];
-    double* tb = tbuf[wid];
+    const int lane = threadIdx.x;
     for (int q = lane; q < NV; q += 64) wacc[q] = 0.0;
-    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
-    for (long long g0 = wave * 64; g0 < C; g0 += nwaves * 64) {
+    const int slot = (int)(blockIdx.x % VG_GRAD_SPLIT);
+    const long long row0 = (long long)(blockIdx.x / VG_GRAD_SPLIT) * rows_per_unit;
+    int nbig = 0;  // wave-replayed cubes of the chunk seen so far
+    for (int r = 0; r < rows_per_unit; ++r) {
+        const long long g0 = (row0 + r) * 64;
+        if (g0 >= C) break;
         const long long h = g0 + lane;
         const int n = h < C ? nh[h] : 0;
-        const bool own = h < C && n <= VG_GRAD_LANE_MAX;
-        unsigned long long big = __ballot(h < C && !own);
+        const bool own = slot == 0 && h < C && n <= VG_GRAD_LANE_MAX;
+        unsigned long long big = __ballot(h < C && n > VG_GRAD_LANE_MAX);
         while (big) {
             const int j = __ffsll((long long)big) - 1;
             big &= big - 1;
+            if ((nbig++) % VG_GRAD_SPLIT != slot) continue;
             const long long hb = g0 + j;
             const int nb = __shfl(n, j, 64);
             const long long s0 = nh_sum[hb];
@@ -661,22 +674,22 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
                 }
             }
         }
-        double sf = 0.0, sf2 = 0.0, g1[NT], g2[NT];
-#pragma unroll
-        for (int p = 0; p < NT; ++p) { g1[p] = 0.0; g2[p] = 0.0; }
-        if (own) {
-            const long long s0 = nh_sum[h];
-            for (int e = 0; e < n; ++e) {
-                double jf, jg[NT];
-                vg_grad_sample(h, s0 + e, it, map, domain, domain_vol, theta, seed, stream, jf, jg);
-                sf = sf + jf;
-                sf2 = fma(jf, jf, sf2);
-#pragma unroll
-                for (int p = 0; p < NT; ++p) { g1[p] = g1[p] + jg[p]; g2[p] = fma(jg[p], jg[p], g2[p]); }
-            }
-            vg_grad_cube(n, V, sf, sf2, g1, g2);
-        }
         if (__ballot(own)) {
+            double sf = 0.0, sf2 = 0.0, g1[NT], g2[NT];
+#pragma unroll
+            for (int p = 0; p < NT; ++p) { g1[p] = 0.0; g2[p] = 0.0; }
+            if (own) {
+                const long long s0 = nh_sum[h];
+                for (int e = 0; e < n; ++e) {
+                    double jf, jg[NT];
+                    vg_grad_sample(h, s0 + e, it, map, domain, domain_vol, theta, seed, stream, jf, jg);
+                    sf = sf + jf;
+                    sf2 = fma(jf, jf, sf2);
+#pragma unroll
+                    for (int p = 0; p < NT; ++p) { g1[p] = g1[p] + jg[p]; g2[p] = fma(jg[p], jg[p], g2[p]); }
+                }
+                vg_grad_cube(n, V, sf, sf2, g1, g2);
+            }
             // sums of I_h and sigma_h^2 / n_h (k = 0), of G_h,p and Var_h,p (k = 1 + p)
             double c1[VG_NCH], c2[VG_NCH];
             vg_wave_tsum(tb, lane, sf, g1, c1);
@@ -695,13 +708,27 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
             }
         }
     }
-    // block partials: the waves' accumulators in wave order
     __syncthreads();
-    for (int q = threadIdx.x; q < NV; q += blockDim.x) {
-        double t = wacc_all[0][q];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) t = t + wacc_all[w][q];
-        part[(long long)blockIdx.x * NV + q] = t;
+    for (int q = lane; q < NV; q += 64) part[(long long)blockIdx.x * NV + q] = wacc[q];
+}
+
+// out[q] = sum over units u < n_units of part[u][q], one block per q: thread t
+// adds units t, t + 256, ... in order, then a fixed pairwise tree over the 256
+// threads, so the sum does not depend on timing.
+extern "C" __global__ void __launch_bounds__(256)
+vg_grad_reduce(const double* __restrict__ part, int n_units, double* __restrict__ out) {
+    constexpr int NV = 2 + 2 * NT;
+    __shared__ double t[256];
+    const int q = blockIdx.x;
+    double a = 0.0;
+    for (int u = threadIdx.x; u < n_units; u += 256) a = a + part[(long long)u * NV + q];
+    t[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) t[threadIdx.x] = t[threadIdx.x] + t[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[q] = t[0];
 }
 #endif
 

@@ -47,6 +47,10 @@ double now_ms() {
 
 constexpr int kLdsBytes = 160 * 1024;  // gfx950 LDS per workgroup
 constexpr int kFillBlock = 1024;
+// Backward replay: work units per chunk of cube rows (vg_grad's VG_GRAD_SPLIT)
+// and the most units one launch writes partials for.
+constexpr int kGradSplit = 4;
+constexpr long long kGradMaxUnits = 65536;
 constexpr int kDefaultRun = 16;
 
 bool checked_cubes(int ns, int d, long long* C) {
@@ -95,7 +99,7 @@ struct cvg_program {
     // module loaded on first use, on the device current at that time
     int device = -1;
     hipModule_t mod = nullptr;
-    hipFunction_t fill = nullptr, dump = nullptr, grad = nullptr, bfill = nullptr;
+    hipFunction_t fill = nullptr, dump = nullptr, grad = nullptr, grad_reduce = nullptr, bfill = nullptr;
     int fill_blocks_per_cu = 0;
 
     int load() {
@@ -111,7 +115,10 @@ struct cvg_program {
         } else {
             VG_HIP(hipModuleGetFunction(&fill, mod, "vg_fill"));
             VG_HIP(hipModuleGetFunction(&dump, mod, "vg_dump"));
-            if (has_grad) VG_HIP(hipModuleGetFunction(&grad, mod, "vg_grad"));
+            if (has_grad) {
+                VG_HIP(hipModuleGetFunction(&grad, mod, "vg_grad"));
+                VG_HIP(hipModuleGetFunction(&grad_reduce, mod, "vg_grad_reduce"));
+            }
             VG_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&fill_blocks_per_cu, fill, block, 0));
         }
         if (fill_blocks_per_cu < 1) fill_blocks_per_cu = 1;
@@ -195,8 +202,8 @@ struct cvg_state {
     double* bw_x = nullptr;
     double2* bw_map64 = nullptr;
     float2* bw_map32 = nullptr;
-    double* bw_part = nullptr;
-    int bw_grid = 0;
+    double* bw_part = nullptr;  // [bw_units][2 + 2 n_theta] unit partials, then the totals
+    int bw_units = 0, bw_rows_per_unit = 0;
     std::vector<char> recorded;  // per iteration: snapshot taken
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // optional per-iteration timing (eager launches): fill and whole iteration
@@ -440,6 +447,7 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
         "-DVG_LDS_MAP=" + std::to_string(prog->lds_map),
         "-DVG_HAS_DOMAIN=" + std::to_string(prog->has_domain),
         "-DVG_GRAD=" + std::to_string(prog->has_grad),
+        "-DVG_GRAD_SPLIT=" + std::to_string(kGradSplit),
         "-DVG_BIN32=" + std::to_string(prog->bin32),
         "-DVG_PAIR=" + std::to_string(prog->pair),
         "-DVG_PIPE=" + std::to_string(prog->pipe),
@@ -973,10 +981,13 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         if (st->recorded[k]) its.push_back(k);
     if (its.empty()) return fail(CVG_ERR_INVALID, "no recorded iterations: enable recording before running");
     if (!st->bw_part) {
-        int ncu = 0;
-        VG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, st->device));
-        st->bw_grid = ncu * 4;
-        if ((rc = dalloc(&st->bw_part, (long long)st->bw_grid * NV))) return rc;
+        // rows of 64 cubes, cut into chunks so that at most kGradMaxUnits units
+        // (kGradSplit per chunk) hold partials
+        const long long rows = (st->C + 63) / 64;
+        const long long rpu = (rows * kGradSplit + kGradMaxUnits - 1) / kGradMaxUnits;
+        st->bw_rows_per_unit = (int)(rpu > 0 ? rpu : 1);
+        st->bw_units = (int)((rows + st->bw_rows_per_unit - 1) / st->bw_rows_per_unit) * kGradSplit;
+        if ((rc = dalloc(&st->bw_part, ((long long)st->bw_units + 1) * NV))) return rc;
         if ((rc = dalloc(&st->bw_nh_sum, st->C + 1))) return rc;
         if ((rc = dalloc(&st->bw_s_total, 1))) return rc;
         if ((rc = dalloc(&st->bw_x, (long long)d * (n + 1)))) return rc;
@@ -1002,7 +1013,7 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
     std::vector<double> hist(4 * (size_t)st->iterations_done);
     VG_HIP(hipMemcpyAsync(hist.data(), st->hist, sizeof(double) * hist.size(), hipMemcpyDeviceToHost, st->stream));
     const long long nx = (long long)d * (n + 1);
-    std::vector<double> part((size_t)st->bw_grid * NV);
+    std::vector<double> tot((size_t)NV);
     double W = 0.0, val = 0.0, vvar = 0.0;
     std::vector<double> g((size_t)P, 0.0), gv((size_t)P, 0.0);
     int C = (int)st->C;
@@ -1021,15 +1032,17 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         const void* map = st->p.precision == CVG_FP32C ? (const void*)st->bw_map32 : (const void*)st->bw_map64;
         unsigned itu = (unsigned)k;
         double V = st->V;
-        void* args[] = {(void*)&map, (void*)&snh, (void*)&st->bw_nh_sum, (void*)&C, (void*)&itu, (void*)&V,
-                        (void*)&st->domain, (void*)&vol, (void*)&th, (void*)&seed, (void*)&sid,
+        int rpu = st->bw_rows_per_unit, nu = st->bw_units;
+        void* args[] = {(void*)&map, (void*)&snh, (void*)&st->bw_nh_sum, (void*)&C, (void*)&rpu, (void*)&itu,
+                        (void*)&V, (void*)&st->domain, (void*)&vol, (void*)&th, (void*)&seed, (void*)&sid,
                         (void*)&st->bw_part};
-        VG_HIP(hipModuleLaunchKernel(prog->grad, (unsigned)st->bw_grid, 1, 1, 256, 1, 1, 0, st->stream, args, nullptr));
-        VG_HIP(hipMemcpyAsync(part.data(), st->bw_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st->stream));
+        VG_HIP(hipModuleLaunchKernel(prog->grad, (unsigned)nu, 1, 1, 64, 1, 1, 0, st->stream, args, nullptr));
+        double* out = st->bw_part + (long long)nu * NV;
+        void* rargs[] = {(void*)&st->bw_part, (void*)&nu, (void*)&out};
+        VG_HIP(hipModuleLaunchKernel(prog->grad_reduce, (unsigned)NV, 1, 1, 256, 1, 1, 0, st->stream, rargs,
+                                     nullptr));
+        VG_HIP(hipMemcpyAsync(tot.data(), out, sizeof(double) * NV, hipMemcpyDeviceToHost, st->stream));
         VG_HIP(hipStreamSynchronize(st->stream));
-        std::vector<double> tot((size_t)NV, 0.0);
-        for (int b = 0; b < st->bw_grid; ++b)
-            for (int q = 0; q < NV; ++q) tot[q] += part[(size_t)b * NV + q];
         const double w = 1.0 / hist[4 * (size_t)k + 1];  // forward weight, held fixed
         W += w;
         val += w * tot[0];
