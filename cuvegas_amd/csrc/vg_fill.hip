@@ -601,8 +601,8 @@ __device__ inline void vg_wave_tsum(double* tb, int lane, double a0, const doubl
 // replayed one per lane and their contributions summed across the wave the
 // same way.  Everything goes into the unit's LDS accumulator (entry 0, 1:
 // value terms; 2 + p, 2 + NT + p: gradient terms, each written by one lane)
-// in that fixed order, and the unit writes it to part[unit][2 + 2 NT];
-// vg_grad_reduce sums the units in index order.
+// in that fixed order, and the unit writes it to part[q][unit] (value q of
+// 2 + 2 NT, unit-contiguous); vg_grad_reduce sums the units in index order.
 #ifndef VG_GRAD_WPE
 #define VG_GRAD_WPE 2
 #endif
@@ -709,22 +709,30 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
         }
     }
     __syncthreads();
-    for (int q = lane; q < NV; q += 64) part[(long long)blockIdx.x * NV + q] = wacc[q];
+    for (int q = lane; q < NV; q += 64) part[(long long)q * gridDim.x + blockIdx.x] = wacc[q];
 }
 
-// out[q] = sum over units u < n_units of part[u][q], one block per q: thread t
-// adds units t, t + 256, ... in order, then a fixed pairwise tree over the 256
+// out[q] = sum over units u < n_units of part[q][u], one block of 1024 threads
+// per q: thread t adds units t, t + 1024, ... into four interleaved partial
+// sums (independent loads in flight), then a fixed pairwise tree over the
 // threads, so the sum does not depend on timing.
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(1024)
 vg_grad_reduce(const double* __restrict__ part, int n_units, double* __restrict__ out) {
-    constexpr int NV = 2 + 2 * NT;
-    __shared__ double t[256];
+    __shared__ double t[1024];
     const int q = blockIdx.x;
-    double a = 0.0;
-    for (int u = threadIdx.x; u < n_units; u += 256) a = a + part[(long long)u * NV + q];
-    t[threadIdx.x] = a;
+    const double* col = part + (long long)q * n_units;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int u = threadIdx.x;
+    for (; u + 3 * 1024 < n_units; u += 4 * 1024) {
+        a0 = a0 + col[u];
+        a1 = a1 + col[u + 1024];
+        a2 = a2 + col[u + 2 * 1024];
+        a3 = a3 + col[u + 3 * 1024];
+    }
+    for (; u < n_units; u += 1024) a0 = a0 + col[u];
+    t[threadIdx.x] = (a0 + a1) + (a2 + a3);
     __syncthreads();
-    for (int w = 128; w >= 1; w >>= 1) {
+    for (int w = 512; w >= 1; w >>= 1) {
         if ((int)threadIdx.x < w) t[threadIdx.x] = t[threadIdx.x] + t[threadIdx.x + w];
         __syncthreads();
     }

@@ -202,7 +202,7 @@ struct cvg_state {
     double* bw_x = nullptr;
     double2* bw_map64 = nullptr;
     float2* bw_map32 = nullptr;
-    double* bw_part = nullptr;  // [bw_units][2 + 2 n_theta] unit partials, then the totals
+    double* bw_part = nullptr;  // [2 + 2 n_theta][bw_units] unit partials, then the totals
     int bw_units = 0, bw_rows_per_unit = 0;
     std::vector<char> recorded;  // per iteration: snapshot taken
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1039,7 +1039,7 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         VG_HIP(hipModuleLaunchKernel(prog->grad, (unsigned)nu, 1, 1, 64, 1, 1, 0, st->stream, args, nullptr));
         double* out = st->bw_part + (long long)nu * NV;
         void* rargs[] = {(void*)&st->bw_part, (void*)&nu, (void*)&out};
-        VG_HIP(hipModuleLaunchKernel(prog->grad_reduce, (unsigned)NV, 1, 1, 256, 1, 1, 0, st->stream, rargs,
+        VG_HIP(hipModuleLaunchKernel(prog->grad_reduce, (unsigned)NV, 1, 1, 1024, 1, 1, 0, st->stream, rargs,
                                      nullptr));
         VG_HIP(hipMemcpyAsync(tot.data(), out, sizeof(double) * NV, hipMemcpyDeviceToHost, st->stream));
         VG_HIP(hipStreamSynchronize(st->stream));
