@@ -107,6 +107,40 @@ def test_gradient_vs_finite_differences():
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_gradient_over_domain_vs_finite_differences(prec):
+    """Replay over a sub-box (x = start + range * y, values and gradients
+    scaled by the box volume): the value at the forward's theta equals the
+    forward's weighted mean of the recorded iterations, and the gradient
+    matches central differences of the frozen estimator."""
+    from cuvegas_amd import Integrator
+    from cuvegas_amd import integrands as I
+    th = _theta()
+    _say("grad-domain", prec)
+    dom = [[0.1, 0.15, 0.1, 0.2, 0.1, 0.05], [0.8, 0.7, 0.85, 0.7, 0.8, 0.9]]
+    st = Integrator(I.gaussian_mixture_param(6, th).source, 6, 200_000, n_intervals=256, max_it=6, skip=3,
+                    seed=7, precision=prec, theta=th, options="grad", domain=dom)
+    st.set_record(True)
+    st.run(6)
+    v0, _, g, _ = st.backward()
+    I_k, S2, _ = st.history()
+    w = 1.0 / np.asarray(S2[3:6])
+    mean = float((w * np.asarray(I_k[3:6])).sum() / w.sum())
+    assert abs(v0 - mean) <= (1e-12 if prec == "fp64" else 1e-6) * abs(mean)
+    scale = np.abs(g).max()
+    # FP32: the frozen estimator (|v| ~ 3.4) carries ~1e-7 relative rounding
+    # noise, ~1e-4 after dividing by 2h = 4e-3, hence the absolute floor
+    h = 1e-4 if prec == "fp64" else 2e-3
+    tol = 2e-4 if prec == "fp64" else 2e-2
+    floor = 2e-7 * scale if prec == "fp64" else 1e-4 * scale
+    for p in (0, 3, 5, 8, 13, 27):
+        tp, tm = th.copy(), th.copy()
+        tp[p] += h
+        tm[p] -= h
+        fd = (st.backward(tp)[0] - st.backward(tm)[0]) / (2 * h)
+        assert abs(fd - g[p]) <= tol * abs(g[p]) + floor, (p, fd, g[p])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_gradient_vs_closed_form(prec):
     """Gradient within 4 sigma of the closed form per component, and the
     squared pulls average below 2 (32 parameters)."""
