@@ -161,7 +161,10 @@ __device__ inline void vg_locate(vg_pre<MapT>& p, const unsigned (&words)[4 * NG
         p.frac[i] = ((double)b.rem + (double)b.low * 0x1p-33) * inv_ns;
 #endif
 #endif
-        p.m[i] = map[i * NB + b.id];
+        // row pointer of dimension i (uniform: the load takes a scalar base
+        // and a 32-bit lane offset instead of a 64-bit address per lane)
+        const MapT* __restrict__ row = map + i * NB;
+        p.m[i] = row[b.id];
         p.ids[i] = b.id;
     }
 }
@@ -492,12 +495,12 @@ __device__ inline void vg_grad_sample(long long h, long long s, unsigned it, con
 #if VG_FP32
         constexpr float inv_ns = 1.0f / (float)NS;
         const float frac = ((float)b.rem + (float)(unsigned)(b.low >> 9) * 0x1p-24f) * inv_ns;
-        const float2 m = gmap[i * NB + b.id];
+        const float2 m = (gmap + i * NB)[b.id];
         x[i] = fmaf(m.y, frac, m.x);
 #else
         constexpr double inv_ns = 1.0 / (double)NS;
         const double frac = ((double)b.rem + (double)b.low * 0x1p-33) * inv_ns;
-        const double2 m = gmap[i * NB + b.id];
+        const double2 m = (gmap + i * NB)[b.id];
         x[i] = fma(m.y, frac, m.x);
 #endif
         dxv[i] = m.y;
@@ -838,11 +841,11 @@ __device__ inline void vg_beval(double& jf, unsigned* ids, long long s, const un
         const unsigned long long low = L & ((1ull << 33) - 1ull);
 #if VG_FP32
         const float frac = ((float)rem + (float)(unsigned)(low >> 9) * 0x1p-24f) * I.inv_ns_f;
-        const float2 m = map[i * NB + id];
+        const float2 m = (map + i * NB)[id];
         x[i] = fmaf(m.y, frac, m.x);
 #else
         const double frac = ((double)rem + (double)low * 0x1p-33) * I.inv_ns;
-        const double2 m = map[i * NB + id];
+        const double2 m = (map + i * NB)[id];
         x[i] = fma(m.y, frac, m.x);
 #endif
         jac = jac * ((vg_real)NB * m.y);

@@ -472,3 +472,34 @@ def test_large_shape_properties(cv):
     JF, JF2 = st.get_cube_sums()
     assert np.all(JF2 >= 0)
     assert np.isfinite(r["result"])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_double_gaussian_full_size(cv, prec):
+    """The 8-D double Gaussian at its full size (n = 1024, n_strat = 8: 16.8 M
+    cubes, 1e8 samples per iteration, 20 iterations, 5 skipped): the averaged
+    result is within 4 sigma and 1e-4 relative of the closed form with
+    chi2/dof < 2, and the invariants of every size hold (n_h >= 2, sum of n_h
+    <= N + 2C, total evaluations = sum of the iterations', map edges strictly
+    increasing from 0 to 1, per-cube sums of squares >= 0)."""
+    from cuvegas_amd import integrands as I
+    d, n, ns, N, iters = 8, 1024, 8, 100_000_000, 20
+    itg = I.double_gaussian(d)
+    _say("double-gaussian-full-size", prec)
+    st = _integrator(cv, itg, d, n, ns, N, iters, prec, skip=5)
+    st.run(iters)
+    r = st.result()
+    nh = st.get_nh()
+    _, _, ne = st.history()
+    x = st.get_map()
+    JF, JF2 = st.get_cube_sums()
+    C = st.n_cubes
+    st.close()
+    assert C == 8 ** 8
+    assert abs(r["result"] - itg.exact) < 4 * r["error"], (r["result"], itg.exact, r["error"])
+    assert abs(r["result"] - itg.exact) <= 1e-4 * abs(itg.exact)
+    assert r["chi2_dof"] < 2.0
+    assert nh.min() >= 2 and nh.sum() <= N + 2 * C
+    assert r["tot_nevals"] == ne.sum() and len(ne) == iters
+    assert np.all(np.diff(x, axis=1) > 0) and np.all(x[:, 0] == 0) and np.all(x[:, -1] == 1)
+    assert np.all(JF2 >= 0)
