@@ -460,12 +460,8 @@ constexpr int NT = VG_NTHETA;
 #define VG_GRAD_LMAX 8
 #endif
 constexpr int VG_GRAD_LANE_MAX = VG_GRAD_LMAX;
-// The map is read through L1/L2 by default: the LDS holds the waves' transpose
-// buffers (VG_GRAD_LDS_MAP=1 stages FP32 tables of up to 64 KB in LDS too).
-#ifndef VG_GRAD_LDS_MAP
-#define VG_GRAD_LDS_MAP 0
-#endif
-constexpr bool kGradLdsMap = VG_GRAD_LDS_MAP && VG_FP32 && (long long)D * NB * 8 <= 65536;
+// The map is read through L1/L2: the LDS holds the transpose buffers (a
+// 48 KB FP32 map staged in LDS as well measured slower).
 // Transposed wave sums of NT + 1 values (one value term, NT gradient terms)
 // go through a per-wave LDS buffer of 64 rows of VG_TW doubles, in VG_NCH
 // column chunks.
@@ -606,10 +602,9 @@ __device__ inline void vg_wave_tsum(double* tb, int lane, double a0, const doubl
 // value terms; 2 + p, 2 + NT + p: gradient terms, each written by one lane)
 // in that fixed order, and the unit writes it to part[q][unit] (value q of
 // 2 + 2 NT, unit-contiguous); vg_grad_reduce sums the units in index order.
-#ifndef VG_GRAD_WPE
-#define VG_GRAD_WPE 2
-#endif
-extern "C" __global__ void __launch_bounds__(64, VG_GRAD_WPE)
+// Two waves per SIMD (at most 256 VGPRs; one wave per SIMD without spills
+// measured slower in both forms).
+extern "C" __global__ void __launch_bounds__(64, 2)
 vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
         const long long* __restrict__ nh_sum, int C, int rows_per_unit, unsigned it, double V,
         const double* __restrict__ domain, double domain_vol, const vg_real* __restrict__ theta,
@@ -617,13 +612,7 @@ vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
     constexpr int NV = 2 + 2 * NT;
     __shared__ double wacc[NV];
     __shared__ double tb[64 * VG_TW];
-    __shared__ vg_map_t lmap[kGradLdsMap ? D * NB : 1];
     const vg_map_t* map = gmap;
-    if (kGradLdsMap) {
-        for (int i = threadIdx.x; i < D * NB; i += blockDim.x) lmap[i] = gmap[i];
-        __syncthreads();
-        map = lmap;
-    }
     const int lane = threadIdx.x;
     for (int q = lane; q < NV; q += 64) wacc[q] = 0.0;
     const int slot = (int)(blockIdx.x % VG_GRAD_SPLIT);
