@@ -91,3 +91,31 @@ def test_batch_graph_matches_eager():
     (ra, h0a), (rb, h0b) = out
     assert h0a == h0b  # first iteration: identical inputs, deterministic sums
     assert np.all(np.abs(ra["result"] - rb["result"]) <= 4 * np.hypot(ra["error"], rb["error"]))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_launch_shapes_agree(prec):
+    """One block per task (the default) and a resident grid striding over the
+    tasks, at two block sizes and task lengths: identical per-cube sums,
+    allocation and first-iteration estimates, maps within 1e-12 (the bin
+    atomics' order differs)."""
+    from cuvegas_amd.batch import FAMILY_SOURCE, Batch, BatchProgram, serving_workload
+    _say("batch-launch-shapes", prec)
+    items, theta, _, _ = serving_workload(24, seed=9)
+    items = [(d, min(N, 200_000), off) for d, N, off in items]
+    ref = None
+    for opts in ("block=256 task_windows=4", "block=256 task_windows=4 fill_grid=0",
+                 "block=512 task_windows=2 fill_grid=0", "block=1024 task_windows=1"):
+        prog = BatchProgram(FAMILY_SOURCE, 12, 128, prec, options=opts)
+        b = Batch(prog, items, theta, max_it=2, skip=0, seed=4)
+        b.run(1, use_graph=False)
+        got = [(b.get_cube_sums(i), b.get_nh(i), b.get_map(i), b.history(i)) for i in range(len(items))]
+        b.close()
+        if ref is None:
+            ref = got
+            continue
+        for i, ((js, nh, x, h), (rjs, rnh, rx, rh)) in enumerate(zip(got, ref)):
+            assert np.array_equal(js[0], rjs[0]) and np.array_equal(js[1], rjs[1]), (opts, i)
+            assert np.array_equal(nh, rnh), (opts, i)
+            assert np.max(np.abs(x - rx)) < 1e-12, (opts, i)
+            assert h[0][0] == rh[0][0] and h[1][0] == rh[1][0] and h[2][0] == rh[2][0], (opts, i)
