@@ -204,6 +204,7 @@ struct cvg_state {
     float2* bw_map32 = nullptr;
     double* bw_part = nullptr;  // [2 + 2 n_theta][bw_units] unit partials, then the totals
     int bw_units = 0, bw_rows_per_unit = 0;
+    void* bw_theta = nullptr;   // parameters of a replay at another theta (n_theta doubles)
     std::vector<char> recorded;  // per iteration: snapshot taken
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // optional per-iteration timing (eager launches): fill and whole iteration
@@ -223,7 +224,7 @@ struct cvg_state {
                         JS, run_cube, win_first, win_last, part, bsum, boff, sync, s_total,
                         it_p, dsum, err, hist, domain, theta, fold_list, fold_count, big_list, bw_fold_list,
                         bw_fold_count, snap_x, snap_nh, bw_nh_sum,
-                        bw_s_total, bw_x, bw_map64, bw_map32, bw_part};
+                        bw_s_total, bw_x, bw_map64, bw_map32, bw_part, bw_theta};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -988,6 +989,7 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         st->bw_rows_per_unit = (int)(rpu > 0 ? rpu : 1);
         st->bw_units = (int)((rows + st->bw_rows_per_unit - 1) / st->bw_rows_per_unit) * kGradSplit;
         if ((rc = dalloc(&st->bw_part, ((long long)st->bw_units + 1) * NV))) return rc;
+        if ((rc = dalloc((double**)&st->bw_theta, P))) return rc;
         if ((rc = dalloc(&st->bw_nh_sum, st->C + 1))) return rc;
         if ((rc = dalloc(&st->bw_s_total, 1))) return rc;
         if ((rc = dalloc(&st->bw_x, (long long)d * (n + 1)))) return rc;
@@ -998,17 +1000,17 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
     }
     // parameters for the replay: given theta, or the forward's
     void* th = st->theta;
-    void* tmp_theta = nullptr;
     if (theta) {
-        VG_HIP(hipMalloc(&tmp_theta, 8 * (size_t)P));
+        // the state's replay buffer: no allocation per call, nothing to free on
+        // an error return
         if (st->p.precision == CVG_FP32C) {
             std::vector<float> f(theta, theta + P);
-            VG_HIP(hipMemcpyAsync(tmp_theta, f.data(), 4 * (size_t)P, hipMemcpyHostToDevice, st->stream));
+            VG_HIP(hipMemcpyAsync(st->bw_theta, f.data(), 4 * (size_t)P, hipMemcpyHostToDevice, st->stream));
             VG_HIP(hipStreamSynchronize(st->stream));
         } else {
-            VG_HIP(hipMemcpyAsync(tmp_theta, theta, 8 * (size_t)P, hipMemcpyHostToDevice, st->stream));
+            VG_HIP(hipMemcpyAsync(st->bw_theta, theta, 8 * (size_t)P, hipMemcpyHostToDevice, st->stream));
         }
-        th = tmp_theta;
+        th = st->bw_theta;
     }
     std::vector<double> hist(4 * (size_t)st->iterations_done);
     VG_HIP(hipMemcpyAsync(hist.data(), st->hist, sizeof(double) * hist.size(), hipMemcpyDeviceToHost, st->stream));
@@ -1052,7 +1054,6 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
             gv[q] += w * w * tot[2 + P + q];
         }
     }
-    if (tmp_theta) (void)hipFree(tmp_theta);
     if (value) *value = val / W;
     if (value_var) *value_var = vvar / (W * W);
     for (int q = 0; q < P; ++q) {
