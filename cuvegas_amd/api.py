@@ -14,6 +14,7 @@ import ctypes as C
 import math
 import os
 import time
+from collections import OrderedDict
 from typing import Optional, Sequence
 
 import numpy as np
@@ -114,6 +115,28 @@ class Program:
             pass
 
 
+# Compiled programs kept for reuse within the process (least recently used
+# dropped beyond the bound); a program can back any number of states.
+_PROGRAMS: "OrderedDict[tuple, Program]" = OrderedDict()
+_PROGRAMS_MAX = 16
+
+
+def cached_program(source: str, params: cvg_params, options: Optional[str] = None) -> Program:
+    """The hipRTC program of (source, compile-time shape, options), compiled on
+    first use and reused afterwards (saves the compile, ~0.2 s, on repeated calls)."""
+    key = (source, params.n_dim, params.n_intervals, params.n_strat, params.precision, params.run_len,
+           params.has_domain, params.n_theta, options or "")
+    prog = _PROGRAMS.get(key)
+    if prog is None:
+        prog = Program(source, params, options)
+        _PROGRAMS[key] = prog
+        while len(_PROGRAMS) > _PROGRAMS_MAX:
+            _PROGRAMS.popitem(last=False)
+    else:
+        _PROGRAMS.move_to_end(key)
+    return prog
+
+
 class Integrator:
     """One integral's device state and iteration loop."""
 
@@ -122,16 +145,25 @@ class Integrator:
                  alpha: float = 0.5, beta: float = 0.75, seed: int = 0, precision="fp64",
                  stream_id: int = 0, run_len: int = 0, domain: Optional[Sequence] = None,
                  options: Optional[str] = None, theta: Optional[Sequence[float]] = None,
-                 program: Optional[Program] = None):
+                 program: Optional[Program] = None, cache_program: bool = False):
         if n_strat is None:
             n_strat = max(1, int(math.pow(n_eval_it / 4.0, 1.0 / n_dim)))
         n_theta = 0 if theta is None else len(theta)
         self.params, self._dom = make_params(n_dim, n_intervals, n_strat, max_it, skip, n_eval_it,
                                              alpha, beta, seed, precision, stream_id, run_len,
                                              domain, n_theta)
-        self.program = program or Program(source, self.params, options)
+        if program is None:
+            program = (cached_program(source, self.params, options) if cache_program
+                       else Program(source, self.params, options))
+        self.program = program
         h = C.c_void_p()
-        check(lib.cvg_state_create(self.program.handle, C.byref(self.params), C.byref(h)))
+        rc = lib.cvg_state_create(self.program.handle, C.byref(self.params), C.byref(h))
+        if rc < 0 and cache_program:
+            # a cached program is bound to the device it was first loaded on:
+            # compile a fresh one for this state
+            self.program = Program(source, self.params, options)
+            rc = lib.cvg_state_create(self.program.handle, C.byref(self.params), C.byref(h))
+        check(rc)
         self._h = h
         self.n_dim, self.n_intervals, self.n_strat = n_dim, n_intervals, n_strat
         self.n_cubes = int(lib.cvg_state_n_cubes(h))
@@ -398,7 +430,8 @@ def integrate(device_function, n_dim, domain=None, n_evals=1000000, max_it=20, s
     n_eval_it = n_evals // max_it
     integ = Integrator(source, n_dim, n_eval_it, n_intervals=n_intervals, n_strat=n_strat,
                        max_it=max_it, skip=skip, alpha=alpha, beta=beta, seed=seed,
-                       precision=precision, run_len=run_len, domain=dom, options=options)
+                       precision=precision, run_len=run_len, domain=dom, options=options,
+                       cache_program=True)
     try:
         integ.run(max_it, use_graph=use_graph)
         r = integ.result()

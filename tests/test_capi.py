@@ -137,3 +137,26 @@ def test_run_len_must_be_power_of_two():
         rc = lib.cvg_program_create(b"__device__ vg_real integrand(vg_real* x){return x[0];}",
                                     C.byref(_params(run_len=bad)), None, C.byref(h))
         assert rc == CVG_ERR_INVALID
+
+
+def test_program_cache_reuses_compiled_programs():
+    """integrate() reuses the hipRTC program of an identical (source, shape,
+    options) and compiles a new one when any of them changes; the cache is
+    bounded (least recently used dropped)."""
+    from cuvegas_amd import api
+    src = "__device__ vg_real integrand(vg_real* x) { return x[0] + x[1]; }"
+    api._PROGRAMS.clear()
+    p1, _ = api.make_params(3, 64, 4, 5, 1, 10_000, 0.5, 0.75, 0)
+    a = api.cached_program(src, p1)
+    assert api.cached_program(src, p1) is a
+    p2, _ = api.make_params(3, 64, 5, 5, 1, 10_000, 0.5, 0.75, 0)  # other n_strat
+    assert api.cached_program(src, p2) is not a
+    assert api.cached_program(src, p1, "pipe=0") is not a
+    p3, _ = api.make_params(3, 64, 4, 9, 2, 50_000, 0.3, 0.5, 7)  # loop parameters only: same program
+    assert api.cached_program(src, p3) is a
+    for k in range(api._PROGRAMS_MAX + 2):
+        pk, _ = api.make_params(2, 16 + k, 3, 2, 1, 1_000, 0.5, 0.75, 0)
+        api.cached_program(src, pk)
+    assert len(api._PROGRAMS) == api._PROGRAMS_MAX
+    assert all(v is not a for v in api._PROGRAMS.values())
+    api._PROGRAMS.clear()
