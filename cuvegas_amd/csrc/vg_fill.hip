@@ -451,7 +451,8 @@ vg_dump(const vg_map_t* __restrict__ gmap, const long long* __restrict__ nh_sum,
 // (unbiased per-cube sample variance; n_h >= 2 always)
 // Also replays I_k and sigma_k^2 with the forward's expressions (results
 // kernel) so value(theta) of the frozen estimator is available for finite
-// differences.  part[block][2 + 2P]: fixed-order block partials.
+// differences.  part[q][unit] (q < 2 + 2P): fixed-order partials of the
+// launch's work units, summed in unit order by vg_grad_reduce.
 // ---------------------------------------------------------------------------
 constexpr int NT = VG_NTHETA;
 // Cubes with more samples than this are replayed by the whole wave (samples
@@ -462,6 +463,7 @@ constexpr int NT = VG_NTHETA;
 constexpr int VG_GRAD_LANE_MAX = VG_GRAD_LMAX;
 // The map is read through L1/L2: the LDS holds the transpose buffers (a
 // 48 KB FP32 map staged in LDS as well measured slower).
+
 // Transposed wave sums of NT + 1 values (one value term, NT gradient terms)
 // go through a per-wave LDS buffer of 64 rows of VG_TW doubles, in VG_NCH
 // column chunks.
