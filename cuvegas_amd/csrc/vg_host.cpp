@@ -13,7 +13,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <list>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -1063,6 +1065,55 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
     return st->check_err();
 }
 
+}  // extern "C"
+
+namespace {
+// Programs compiled by cvg_integrate_cuda, kept for later calls with the same
+// integrand source, compile-time shape, options and device (most recently
+// used first, at most kOneShotPrograms).  The one-shot call is the only user;
+// a program in use stays alive through its shared_ptr when it is evicted.
+constexpr size_t kOneShotPrograms = 16;
+struct one_shot_cache {
+    std::mutex mu;
+    std::list<std::pair<std::string, std::shared_ptr<cvg_program>>> lru;
+};
+one_shot_cache& one_shot_programs() {
+    static one_shot_cache c;
+    return c;
+}
+
+int one_shot_program(const char* src, const cvg_params& p, const std::string& opts,
+                     std::shared_ptr<cvg_program>* out) {
+    int dev = 0;
+    VG_HIP(hipGetDevice(&dev));
+    const std::string key = std::string(src) + '\x01' + std::to_string(dev) + ',' + std::to_string(p.n_dim) + ',' +
+                            std::to_string(p.n_intervals) + ',' + std::to_string(p.n_strat) + ',' +
+                            std::to_string(p.precision) + ',' + opts;
+    one_shot_cache& c = one_shot_programs();
+    std::lock_guard<std::mutex> lock(c.mu);
+    for (auto it = c.lru.begin(); it != c.lru.end(); ++it) {
+        if (it->first == key) {
+            c.lru.splice(c.lru.begin(), c.lru, it);
+            *out = it->second;
+            return CVG_OK;
+        }
+    }
+    cvg_program* raw = nullptr;
+    int rc = cvg_program_create(src, &p, opts.c_str(), &raw);
+    if (!rc) rc = raw->load();  // loaded here, under the lock, on this device
+    if (rc) {
+        cvg_program_destroy(raw);
+        return rc;
+    }
+    out->reset(raw, cvg_program_destroy);
+    c.lru.emplace_front(key, *out);
+    if (c.lru.size() > kOneShotPrograms) c.lru.pop_back();
+    return CVG_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int cvg_integrate_cuda(long long tot_eval, int max_it, int skip, long long max_batch_size,
                        int n_intervals, int n_strat, int n_dim, double alpha, double beta,
                        unsigned int seed, int multi_gpu, int multi_cpu, int gpu_num, int cpu_num,
@@ -1095,19 +1146,16 @@ int cvg_integrate_cuda(long long tot_eval, int max_it, int skip, long long max_b
             else rtc_opts += tok + " ";
         }
     }
-    cvg_program* prog = nullptr;
-    int rc = cvg_program_create(integrand_src, &p, rtc_opts.c_str(), &prog);
-    if (rc) {
-        cvg_program_destroy(prog);
-        return rc;
-    }
+    if (!integrand_src) return fail(CVG_ERR_INVALID, "integrand_src is NULL");
+    std::shared_ptr<cvg_program> prog;
+    int rc = one_shot_program(integrand_src, p, rtc_opts, &prog);
+    if (rc) return rc;
     cvg_state* st = nullptr;
-    rc = cvg_state_create(prog, &p, &st);
+    rc = cvg_state_create(prog.get(), &p, &st);
     if (!rc) rc = cvg_state_run(st, max_it, 1);
     cvg_result r{};
     if (!rc) rc = cvg_state_result(st, &r);
     cvg_state_destroy(st);
-    cvg_program_destroy(prog);
     if (rc) return rc;
     out->res = r.result;
     out->err = r.error;

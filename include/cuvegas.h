@@ -235,7 +235,10 @@ int cvg_batch_get_cube_sums(cvg_batch* b, int item, double* jf, double* jf2);
  * and may hold "fp32" (CVG_FP32C) and extra hipRTC options.  batch_size,
  * multi_cpu, cpu_num and gpu_thr are accepted and reported back unchanged;
  * the device path needs no sample batching or host-side cube mapping.
- * multi_gpu runs on the current device only. */
+ * multi_gpu runs on the current device only.  The compiled program is kept
+ * for later calls with the same source, shape, options and device (up to 16,
+ * least recently used dropped; safe from several threads), so only the first
+ * such call pays the hipRTC compile. */
 int cvg_integrate_cuda(long long tot_eval, int max_it, int skip, long long max_batch_size,
                        int n_intervals, int n_strat, int n_dim, double alpha, double beta,
                        unsigned int seed, int multi_gpu, int multi_cpu, int gpu_num, int cpu_num,
