@@ -503,3 +503,35 @@ def test_double_gaussian_full_size(cv, prec):
     assert r["tot_nevals"] == ne.sum() and len(ne) == iters
     assert np.all(np.diff(x, axis=1) > 0) and np.all(x[:, 0] == 0) and np.all(x[:, -1] == 1)
     assert np.all(JF2 >= 0)
+
+
+def test_one_shot_calls_from_threads(cv):
+    """cvg_integrate_cuda from four threads at once, twice each, over two
+    integrands (the calls share the process's compiled-program cache): every
+    result is within 5 sigma of its closed form."""
+    import threading
+    from vegas_cuda_extension.cuda_extension import integrate_cuda
+    _say("one-shot-threads")
+    srcs = [("__device__ vg_real integrand(vg_real* x) { return x[0] * x[1] * x[2]; }", 0.125),
+            ("__device__ vg_real integrand(vg_real* x) { return x[0] + x[1] + x[2]; }", 1.5)]
+    out, errs = [], []
+
+    def work(t):
+        try:
+            for k in range(2):
+                src, exact = srcs[(t + k) % 2]
+                r = integrate_cuda(500_000, 8, 3, 1 << 20, 256, 25, 3, 0.5, 0.75, 100 * t + k, False, False,
+                                   -1, -1, 0.1, src, "fp32" if t % 2 else "")
+                out.append((r[0], r[1], exact))
+        except Exception as e:  # reported below, on the main thread
+            errs.append(e)
+
+    threads = [threading.Thread(target=work, args=(t,)) for t in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errs, errs
+    assert len(out) == 8
+    for res, err, exact in out:
+        assert abs(res - exact) < 5 * err, (res, err, exact)
