@@ -19,7 +19,8 @@
 //
 // Compile-time shape (hipRTC -D options): VG_NDIM, VG_NBINS, VG_NSTRAT, VG_K,
 // VG_BLOCK, VG_FP32 (0: FP64 sampling and evaluation; 1: FP32 sampling and
-// evaluation with FP64 accumulation), VG_LDS_BINS, VG_LDS_MAP, VG_HAS_DOMAIN,
+// evaluation with FP64 accumulation), VG_LDS_BINS, VG_BIN_DIMS (dimensions
+// whose bins one pass keeps in LDS), VG_LDS_MAP, VG_HAS_DOMAIN,
 // VG_PAIR (two samples per lane step), VG_PIPE (one sample per step, the next
 // sample's map entries requested before the current one is evaluated).
 #include "vg_philox.h"
@@ -234,11 +235,17 @@ vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
         double2* __restrict__ win_first, double2* __restrict__ win_last,
         double* __restrict__ weights, unsigned* __restrict__ counts,
         const double* __restrict__ domain, double domain_vol,
-        const vg_real* __restrict__ theta, unsigned seed, unsigned stream) {
+        const vg_real* __restrict__ theta, unsigned seed, unsigned stream,
+        int bin_lo, int write_sums) {
+    // When the bins of all D dimensions do not fit in LDS, the host runs one
+    // pass per group of BD dimensions [bin_lo, bin_lo + BD): every pass draws
+    // the same samples, adds the bins of its group, and only the pass with
+    // write_sums set stores the per-cube sums.
 #if VG_LDS_BINS
-    __shared__ vg_bin_t lw[D * NB];
-    __shared__ unsigned lc[D * NB];
-    for (int i = threadIdx.x; i < D * NB; i += VG_BLOCK) { lw[i] = (vg_bin_t)0; lc[i] = 0u; }
+    constexpr int BD = VG_BIN_DIMS;
+    __shared__ vg_bin_t lw[BD * NB];
+    __shared__ unsigned lc[BD * NB];
+    for (int i = threadIdx.x; i < BD * NB; i += VG_BLOCK) { lw[i] = (vg_bin_t)0; lc[i] = 0u; }
 #endif
 #if VG_LDS_MAP
     __shared__ vg_map_t lmap[D * NB];
@@ -279,7 +286,7 @@ vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
         auto close_cube = [&]() {
             if (npieces == 0 && sb) {
                 hasF = true; hF = h; f1 = a1; f2 = a2;
-            } else {
+            } else if (write_sums) {
                 JS[h] = make_double2(a1, a2);
             }
             ++npieces;
@@ -292,8 +299,11 @@ vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
 #pragma unroll
             for (int i = 0; i < D; ++i) {
 #if VG_LDS_BINS
-                atomicAdd(&lw[i * NB + o.ids[i]], (vg_bin_t)o.jf2);
-                atomicAdd(&lc[i * NB + o.ids[i]], 1u);
+                const int li = BD == D ? i : i - bin_lo;  // this pass's slot of dimension i
+                if (BD == D || (unsigned)li < (unsigned)BD) {
+                    atomicAdd(&lw[li * NB + o.ids[i]], (vg_bin_t)o.jf2);
+                    atomicAdd(&lc[li * NB + o.ids[i]], 1u);
+                }
 #else
                 unsafeAtomicAdd(&weights[i * NB + o.ids[i]], o.jf2);
                 atomicAdd(&counts[i * NB + o.ids[i]], 1u);
@@ -380,7 +390,7 @@ vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
         }
         const double q1 = __shfl_up(s1, 1, 64), q2 = __shfl_up(s2, 1, 64);
         const long long wstart = w * WS;
-        if (active) {
+        if (active && write_sums) {
             if (hasF) {
                 double t1 = f1, t2 = f2;
                 if (lane > 0) { t1 = q1 + t1; t2 = q2 + t2; }
@@ -399,11 +409,12 @@ vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
 
 #if VG_LDS_BINS
     __syncthreads();
-    for (int i = threadIdx.x; i < D * NB; i += VG_BLOCK) {
+    const int nb_pass = (D - bin_lo < BD ? D - bin_lo : BD) * NB;
+    for (int i = threadIdx.x; i < nb_pass; i += VG_BLOCK) {
         const unsigned c = lc[i];
         if (c) {
-            unsafeAtomicAdd(&weights[i], (double)lw[i]);
-            atomicAdd(&counts[i], c);
+            unsafeAtomicAdd(&weights[bin_lo * NB + i], (double)lw[i]);
+            atomicAdd(&counts[bin_lo * NB + i], c);
         }
     }
 #endif

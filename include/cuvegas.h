@@ -109,9 +109,11 @@ int cvg_device_info(int* n_cu, int* clock_khz, char* arch, int arch_len);
  * Knobs (consumed here): block=64..1024 (multiple of 64), pipe=0|1 (one
  * sample per lane step; 1: the next sample's map entries are requested before
  * the current one is evaluated, the default for n_dim <= 8), pair=0|1 (two
- * samples per step), lds_bins=0|1, lds_map=0|1 (FP32 map table in LDS),
- * bin32=0|1, grad=0|1 (compile the backward replay kernel).  Every other
- * token is passed to hipRTC.
+ * samples per step), lds_bins=0|1 (1, the default: per-block bins in LDS,
+ * in one sampling pass per group of dimensions when d x n_intervals bins do
+ * not fit; 0: every sample to the global bins), lds_map=0|1 (FP32 map table
+ * in LDS), bin32=0|1, grad=0|1 (compile the backward replay kernel).  Every
+ * other token is passed to hipRTC.
  * On CVG_ERR_COMPILE, *out is still set so cvg_program_log can be read. */
 int cvg_program_create(const char* integrand_src, const cvg_params* shape, const char* options,
                        cvg_program** out);

@@ -172,6 +172,42 @@ def test_sampling_loop_variants_bit_exact(cv, orc, prec, options, K):
     assert np.array_equal(JF2, oJF2)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("shape", [(14, 1024, 2, 60_000), (16, 1024, 2, 120_000)])
+def test_bins_in_dimension_groups(cv, orc, prec, shape):
+    """Bins larger than LDS (d x 1024 x 12 B > 160 KB): the fill runs once per
+    group of 13 dimensions, each pass drawing the same samples.  Bin counts
+    equal the oracle's, weights agree to 1e-12, and the per-cube sums (stored
+    by the first pass only) equal the oracle's."""
+    from cuvegas_amd import integrands as I
+    d, n, ns, N = shape
+    itg = I.poly_peak(d)
+    _say("bin-groups", prec, shape)
+    p, keep = _orc_params(orc, itg, d, n, ns, N, 1, 0 if prec == "fp64" else 1)
+    st = _integrator(cv, itg, d, n, ns, N, 2, prec)
+    nh0 = st.get_nh()
+    x0 = st.get_map()
+    S = int(nh0.sum())
+    _, ids, _, ojf = orc.sample_dump(p, keep, x0, nh0, 0, S)
+    oJF, oJF2 = orc.cube_sums(nh0, ojf, ojf * ojf, 16)
+    oc = np.zeros((d, n), dtype=np.uint32)
+    ow = np.zeros((d, n))
+    for i in range(d):
+        oc[i] = np.bincount(ids[:, i], minlength=n)
+        np.add.at(ow[i], ids[:, i], ojf * ojf)
+    st.fill()
+    w, c = st.get_bins()
+    assert np.array_equal(c, oc)
+    assert np.max(np.abs(w - ow) / np.maximum(ow, 1e-300)) < 1e-12
+    st2 = cv.Integrator(itg.source, d, N, n_intervals=n, n_strat=ns, max_it=2, skip=1, seed=0,
+                        precision=prec, program=st.program)
+    st2.run(1, use_graph=False)
+    JF, JF2 = st2.get_cube_sums()
+    assert np.array_equal(JF, oJF) and np.array_equal(JF2, oJF2)
+    st.close()
+    st2.close()
+
+
 def test_refine_pure_function_bit_exact(cv, orc):
     """Identical bin accumulators in, identical map out."""
     from cuvegas_amd import integrands as I
