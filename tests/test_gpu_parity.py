@@ -130,12 +130,14 @@ def test_first_iteration_bit_exact(cv, orc, prec, layout, K):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("options", ["pair=1", "pipe=0", "pipe=1", "pipe=0 lds_map=1", "pipe=1 lds_map=1"])
+@pytest.mark.parametrize("options", ["pair=1", "pipe=0", "pipe=1", "pipe=0 lds_map=1", "pipe=1 lds_map=1",
+                                     "pipe=1 lds_bins=0"])
 @pytest.mark.parametrize("K", [16, 2])
 def test_sampling_loop_variants_bit_exact(cv, orc, prec, options, K):
     """Every sampling-loop schedule (two samples per step; one per step; the
     next sample's map entries requested early; FP32 map read from LDS or
-    through the caches) gives the oracle's per-cube sums and bin counts bit for
+    through the caches; bins added to global memory per sample instead of
+    per block in LDS) gives the oracle's per-cube sums and bin counts bit for
     bit: cubes of 2 samples next to cubes spanning windows, runs
     of 2 (the peeled last sample is every other sample), a partial last run."""
     from cuvegas_amd import integrands as I
