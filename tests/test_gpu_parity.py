@@ -326,12 +326,14 @@ def test_fp32_form_double_gaussian(cv):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("shape", [(2, 100, 300, 200_000), (1, 1000, 1000, 50_000), (3, 48, 7, 30_000),
-                                   (14, 1024, 2, 60_000)])
+                                   (14, 1024, 2, 60_000), (37, 64, 1, 20_000), (64, 32, 1, 8_000)])
 def test_first_iteration_bit_exact_shapes(cv, orc, prec, shape):
     """Shapes off the common path: n_strat > 256 (the in-bin offset's general
-    form), bin counts that are not powers of two, one dimension, and bins
-    too large for LDS (14 x 1024 x 12 B > 160 KB: per-sample global bin
-    atomics).  Sample dump and first iteration (per-cube sums, d_h, n_h, I_0,
+    form), bin counts that are not powers of two, one dimension, bins too
+    large for LDS (14 x 1024 x 12 B > 160 KB: the fill runs in dimension
+    groups), and many dimensions (37: a last Philox block of one word; 64: the
+    largest single-integral dimension) with a single cube that spans every
+    window.  Sample dump and first iteration (per-cube sums, d_h, n_h, I_0,
     sigma_0^2) bit-exact."""
     from cuvegas_amd import integrands as I
     d, n, ns, N = shape
