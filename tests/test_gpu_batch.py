@@ -22,15 +22,22 @@ def _say(*a):
     print("[gpu-case]", *a, flush=True)
 
 
+BATCH_SHAPES = [
+    # max_dim, n_intervals, items (d, n_eval_it, theta offset)
+    (6, 64, [(2, 20_000, 0), (5, 60_000, 0), (3, 3_000, 0), (4, 250_000, 0), (1, 5_000, 0), (6, 40_000, 0)]),
+    (16, 256, [(16, 50_000, 0), (9, 30_000, 0), (12, 20_000, 0)]),
+]
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_batch_items_match_single_integrals(prec):
+@pytest.mark.parametrize("shape", BATCH_SHAPES, ids=["maxd6", "maxd16"])
+def test_batch_items_match_single_integrals(prec, shape):
     from cuvegas_amd import Integrator
     from cuvegas_amd.batch import Batch, BatchProgram
-    _say("batch-vs-single", prec)
-    n = 64
-    items = [(2, 20_000, 0), (5, 60_000, 0), (3, 3_000, 0), (4, 250_000, 0), (1, 5_000, 0)]
+    max_dim, n, items = shape
+    _say("batch-vs-single", prec, max_dim, n)
     bprog = BatchProgram(POLY + "__device__ vg_real integrand(vg_real* x, const vg_real* th, int d)"
-                         " { return vg_poly(x, d); }", 6, n, prec)
+                         " { return vg_poly(x, d); }", max_dim, n, prec)
     b = Batch(bprog, items, [0.0], max_it=4, skip=1, seed=7)
     b.run(1, use_graph=False)
     for i, (d, N, _) in enumerate(items):
