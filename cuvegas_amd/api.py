@@ -13,6 +13,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+import threading
 import time
 from collections import OrderedDict
 from typing import Optional, Sequence
@@ -116,9 +117,13 @@ class Program:
 
 
 # Compiled programs kept for reuse within the process (least recently used
-# dropped beyond the bound); a program can back any number of states.
+# dropped beyond the bound); a program can back any number of states.  The
+# lock covers the table only: a compile runs outside it (ctypes releases the
+# GIL), and two threads that miss on the same key both compile, the second
+# result replacing the first in the table.
 _PROGRAMS: "OrderedDict[tuple, Program]" = OrderedDict()
 _PROGRAMS_MAX = 16
+_PROGRAMS_LOCK = threading.Lock()
 
 
 def cached_program(source: str, params: cvg_params, options: Optional[str] = None) -> Program:
@@ -126,14 +131,16 @@ def cached_program(source: str, params: cvg_params, options: Optional[str] = Non
     first use and reused afterwards (saves the compile, ~0.2 s, on repeated calls)."""
     key = (source, params.n_dim, params.n_intervals, params.n_strat, params.precision, params.run_len,
            params.has_domain, params.n_theta, options or "")
-    prog = _PROGRAMS.get(key)
-    if prog is None:
-        prog = Program(source, params, options)
+    with _PROGRAMS_LOCK:
+        prog = _PROGRAMS.get(key)
+        if prog is not None:
+            _PROGRAMS.move_to_end(key)
+            return prog
+    prog = Program(source, params, options)
+    with _PROGRAMS_LOCK:
         _PROGRAMS[key] = prog
         while len(_PROGRAMS) > _PROGRAMS_MAX:
             _PROGRAMS.popitem(last=False)
-    else:
-        _PROGRAMS.move_to_end(key)
     return prog
 
 
