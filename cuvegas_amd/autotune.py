@@ -85,8 +85,10 @@ def autotune_batch(source: str, items: Sequence, theta: Sequence[float], *, max_
         try:
             os.makedirs(os.path.dirname(path), exist_ok=True)
             cache[sig] = out
-            with open(path, "w") as f:
+            tmp = f"{path}.{os.getpid()}.tmp"  # whole-file replace: readers never see a partial file
+            with open(tmp, "w") as f:
                 json.dump(cache, f)
+            os.replace(tmp, path)
         except OSError:
             pass
     return dict(out, cached=False)
