@@ -1453,7 +1453,7 @@ int cvg_batch_create(cvg_program* prog, const cvg_params* common, const cvg_batc
         const cvg_batch_item& it = items[i];
         if (it.n_dim < 1 || it.n_dim > prog->max_dim) return fail(CVG_ERR_INVALID, "item dimension out of range");
         if (it.n_eval_it < 1) return fail(CVG_ERR_INVALID, "item budget must be >= 1");
-        if (it.theta_offset < 0 || (n_theta > 0 && it.theta_offset >= n_theta))
+        if (it.theta_offset < 0 || (n_theta > 0 ? it.theta_offset >= n_theta : it.theta_offset != 0))
             return fail(CVG_ERR_INVALID, "item theta_offset out of range");
         int ns = it.n_strat;
         if (ns <= 0) ns = (int)std::pow(it.n_eval_it / 4.0, 1.0 / it.n_dim);  // python_wrapper.py:15-18
@@ -1561,6 +1561,8 @@ int cvg_batch_create(cvg_program* prog, const cvg_params* common, const cvg_batc
         } else {
             VG_HIP(hipMemcpyAsync(b->theta, theta, 8 * (size_t)n_theta, hipMemcpyHostToDevice, s));
         }
+    } else {
+        VG_HIP(hipMemsetAsync(b->theta, 0, 8, s));
     }
     VG_HIP(hipMemsetAsync(b->it_p, 0, sizeof(int), s));
     VG_HIP(hipMemsetAsync(b->err, 0, sizeof(int), s));

@@ -210,7 +210,10 @@ typedef struct cvg_batch_item {
     int n_dim;            /* 1..max_dim */
     int n_strat;          /* 0: int((n_eval_it / 4)^(1/n_dim)) */
     long long n_eval_it;  /* samples per iteration */
-    int theta_offset;     /* first parameter of this item */
+    int theta_offset;     /* first parameter of this item: in [0, n_theta), or 0 when
+                             n_theta is 0; the integrand reads only theta[offset ..
+                             n_theta) (the device copy holds n_theta doubles, at
+                             least one) */
 } cvg_batch_item;
 /* shape supplies n_intervals (<= 1024), precision and run_len; options may
  * hold block=N (threads per block), task_windows=N (windows per wave in one

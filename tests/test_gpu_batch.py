@@ -126,3 +126,23 @@ def test_batch_launch_shapes_agree(prec):
             assert np.array_equal(nh, rnh), (opts, i)
             assert np.max(np.abs(x - rx)) < 1e-12, (opts, i)
             assert h[0][0] == rh[0][0] and h[1][0] == rh[1][0] and h[2][0] == rh[2][0], (opts, i)
+
+
+def test_batch_rejects_bad_items():
+    """Items outside the program's limits or the parameter block are refused
+    with an error before anything is launched."""
+    from cuvegas_amd._lib import CuvegasError
+    from cuvegas_amd.batch import Batch, BatchProgram
+    _say("batch-bad-items")
+    prog = BatchProgram(POLY + "__device__ vg_real integrand(vg_real* x, const vg_real* th, int d)"
+                        " { return vg_poly(x, d); }", 4, 32, "fp64")
+    for items, theta in [([(5, 1000, 0)], [0.0]),       # d > max_dim
+                         ([(2, 1000, 1)], [0.0]),       # offset past the parameters
+                         ([(2, 1000, 1)], []),          # offset with no parameters
+                         ([(2, 0, 0)], [0.0])]:         # empty budget
+        with pytest.raises(CuvegasError):
+            Batch(prog, items, theta, max_it=2, skip=0)
+    b = Batch(prog, [(2, 1000, 0)], [], max_it=2, skip=0)
+    b.run(2)
+    assert np.isfinite(b.results()["result"][0])
+    b.close()
