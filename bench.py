@@ -387,8 +387,15 @@ def main():
     achieved = S * head["evals_per_fill_avg"] / (head["fill_ms_avg"] * 1e-3)
     roof = {"bound": "valu", "kernel": "vg_fill",
             "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tslot/s",
-            "frac": achieved / peak, "slots_per_eval": S, "traffic": _traffic_from_profile(args.workload),
+            "frac": achieved / peak, "slots_per_eval": S,
+            "traffic": _traffic_from_profile(args.workload) if head is forms.get("fp32") else None,
             "fill_ms_avg": head["fill_ms_avg"], "evals_per_fill": head["evals_per_fill_avg"]}
+    if roof["traffic"]:
+        # the same launch against the HBM roofline: measured bytes per launch / its duration
+        from cuvegas_amd.roofline import MI355X_HBM_BPS
+        hbm = roof["traffic"] / (head["fill_ms_avg"] * 1e-3)
+        roof["hbm"] = {"achieved": hbm / 1e9, "peak": MI355X_HBM_BPS / 1e9, "unit": "GB/s",
+                       "frac": hbm / MI355X_HBM_BPS}
 
     cpu = None
     if not args.no_cpu:
