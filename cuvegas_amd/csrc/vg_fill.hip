@@ -89,6 +89,14 @@ __device__ inline R call(R* x, const R*, long, long) {
 constexpr int D = VG_NDIM;
 constexpr int NB = VG_NBINS;
 constexpr unsigned NS = VG_NSTRAT;
+// The sampling tables hold (edge, NB * width) when NB is a power of two: the
+// Jacobian factor is read instead of formed, and the in-bin offset carries
+// the 1/NB.  Both scalings are exact, so the point edge + width * offset and
+// the Jacobian are bit-identical to the unscaled form.
+constexpr bool VG_WSCALED = (NB & (NB - 1)) == 0;
+constexpr float VG_OFF_SCALE_F = VG_WSCALED ? 1.0f / (float)NB : 1.0f;
+constexpr double VG_OFF_SCALE = VG_WSCALED ? 1.0 / (double)NB : 1.0;
+__device__ inline vg_real vg_jac_factor(vg_real w) { return VG_WSCALED ? w : (vg_real)NB * w; }
 constexpr int K = VG_K;
 constexpr int NG = (D + 3) / 4;
 constexpr int WPB = VG_BLOCK / 64;
@@ -147,9 +155,9 @@ __device__ inline void vg_locate(vg_pre<MapT>& p, const unsigned (&words)[4 * NG
 #if VG_NSTRAT <= 256
         // rem * 2^24 + (low >> 9) < 2^32: one conversion rounds the same real
         // number as (float)rem + (float)(low >> 9) * 2^-24 does, times 2^24.
-        p.frac[i] = (float)((b.rem << 24) | (unsigned)(b.low >> 9)) * (0x1p-24f * inv_ns);
+        p.frac[i] = (float)((b.rem << 24) | (unsigned)(b.low >> 9)) * (0x1p-24f * inv_ns * VG_OFF_SCALE_F);
 #else
-        p.frac[i] = ((float)b.rem + (float)(unsigned)(b.low >> 9) * 0x1p-24f) * inv_ns;
+        p.frac[i] = ((float)b.rem + (float)(unsigned)(b.low >> 9) * 0x1p-24f) * (inv_ns * VG_OFF_SCALE_F);
 #endif
 #else
         constexpr double inv_ns = 1.0 / (double)NS;
@@ -157,9 +165,9 @@ __device__ inline void vg_locate(vg_pre<MapT>& p, const unsigned (&words)[4 * NG
         // rem * 2^33 + low < 2^41 is exact in FP64 (as is rem + low * 2^-33),
         // so scaling it by 2^-33 / n_strat rounds exactly as the sum does.
         const unsigned hi = (b.rem << 1) | (unsigned)(b.low >> 32);
-        p.frac[i] = fma((double)hi, 0x1p32, (double)(unsigned)b.low) * (0x1p-33 * inv_ns);
+        p.frac[i] = fma((double)hi, 0x1p32, (double)(unsigned)b.low) * (0x1p-33 * inv_ns * VG_OFF_SCALE);
 #else
-        p.frac[i] = ((double)b.rem + (double)b.low * 0x1p-33) * inv_ns;
+        p.frac[i] = ((double)b.rem + (double)b.low * 0x1p-33) * (inv_ns * VG_OFF_SCALE);
 #endif
 #endif
         // row pointer of dimension i (uniform: the load takes a scalar base
@@ -186,7 +194,7 @@ __device__ inline void vg_finish(vg_sample& o, const vg_pre<MapT>& p,
     }
     vg_real jac = (vg_real)1;
 #pragma unroll
-    for (int i = 0; i < D; ++i) jac = jac * ((vg_real)NB * p.m[i].y);
+    for (int i = 0; i < D; ++i) jac = jac * vg_jac_factor(p.m[i].y);
     vg_real xe[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -503,12 +511,12 @@ __device__ inline void vg_grad_sample(long long h, long long s, unsigned it, con
         const vg_binpos b = vg_bin_of(words[i], k * (unsigned)NB, (unsigned)NB, NS);
 #if VG_FP32
         constexpr float inv_ns = 1.0f / (float)NS;
-        const float frac = ((float)b.rem + (float)(unsigned)(b.low >> 9) * 0x1p-24f) * inv_ns;
+        const float frac = ((float)b.rem + (float)(unsigned)(b.low >> 9) * 0x1p-24f) * (inv_ns * VG_OFF_SCALE_F);
         const float2 m = (gmap + i * NB)[b.id];
         x[i] = fmaf(m.y, frac, m.x);
 #else
         constexpr double inv_ns = 1.0 / (double)NS;
-        const double frac = ((double)b.rem + (double)b.low * 0x1p-33) * inv_ns;
+        const double frac = ((double)b.rem + (double)b.low * 0x1p-33) * (inv_ns * VG_OFF_SCALE);
         const double2 m = (gmap + i * NB)[b.id];
         x[i] = fma(m.y, frac, m.x);
 #endif
@@ -516,7 +524,7 @@ __device__ inline void vg_grad_sample(long long h, long long s, unsigned it, con
     }
     vg_real jac = (vg_real)1;
 #pragma unroll
-    for (int i = 0; i < D; ++i) jac = jac * ((vg_real)NB * dxv[i]);
+    for (int i = 0; i < D; ++i) jac = jac * vg_jac_factor(dxv[i]);
 #if VG_HAS_DOMAIN
 #pragma unroll
     for (int i = 0; i < D; ++i) x[i] = x[i] * (vg_real)domain[D + i] + (vg_real)domain[i];
@@ -783,6 +791,12 @@ __device__ inline unsigned vg_udiv(unsigned x, unsigned m, unsigned sh1, unsigne
 
 constexpr int MD = VG_MAXD;
 constexpr int NB = VG_NBINS;
+// table convention as in the single-integral kernels: (edge, NB * width)
+// when NB is a power of two, the 1/NB carried by the in-bin offset
+constexpr bool VG_WSCALED = (NB & (NB - 1)) == 0;
+constexpr float VG_OFF_SCALE_F = VG_WSCALED ? 1.0f / (float)NB : 1.0f;
+constexpr double VG_OFF_SCALE = VG_WSCALED ? 1.0 / (double)NB : 1.0;
+__device__ inline vg_real vg_jac_factor(vg_real w) { return VG_WSCALED ? w : (vg_real)NB * w; }
 constexpr int K = VG_K;
 constexpr int WPB = VG_BLOCK / 64;
 constexpr long long WS = 64LL * K;
@@ -842,15 +856,15 @@ __device__ inline void vg_beval(double& jf, unsigned* ids, long long s, const un
         const unsigned rem = Q - id * I.ns;
         const unsigned long long low = L & ((1ull << 33) - 1ull);
 #if VG_FP32
-        const float frac = ((float)rem + (float)(unsigned)(low >> 9) * 0x1p-24f) * I.inv_ns_f;
+        const float frac = ((float)rem + (float)(unsigned)(low >> 9) * 0x1p-24f) * (I.inv_ns_f * VG_OFF_SCALE_F);
         const float2 m = (map + i * NB)[id];
         x[i] = fmaf(m.y, frac, m.x);
 #else
-        const double frac = ((double)rem + (double)low * 0x1p-33) * I.inv_ns;
+        const double frac = ((double)rem + (double)low * 0x1p-33) * (I.inv_ns * VG_OFF_SCALE);
         const double2 m = (map + i * NB)[id];
         x[i] = fma(m.y, frac, m.x);
 #endif
-        jac = jac * ((vg_real)NB * m.y);
+        jac = jac * vg_jac_factor(m.y);
         ids[i] = id;
     }
     const vg_real f = integrand(x, theta, DD);

@@ -508,11 +508,15 @@ long long vg_big_list_cap(long long run_cap) {
 
 __device__ void vg_write_tables(int n, const double* __restrict__ x, double2* __restrict__ map64,
                                 float2* __restrict__ map32) {
+    // (edge, n * width) when n is a power of two (an exact scaling: the
+    // sampling kernels read the Jacobian factor n * width directly and scale
+    // the in-bin offset by 1/n instead), else (edge, width)
+    const double ws = (n & (n - 1)) == 0 ? (double)n : 1.0;
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         const double a = x[k], b = x[k + 1];
-        map64[k] = make_double2(a, b - a);
+        map64[k] = make_double2(a, (b - a) * ws);
         const float fa = (float)a, fb = (float)b;
-        map32[k] = make_float2(fa, fb - fa);
+        map32[k] = make_float2(fa, (fb - fa) * (float)ws);
     }
 }
 
