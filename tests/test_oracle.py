@@ -106,6 +106,41 @@ def test_offset_single_conversion_rewrite_is_exact():
         assert np.array_equal(ref64.view(np.uint64), one64.view(np.uint64)), ns
 
 
+def test_scaled_table_form_is_exact():
+    """For a power-of-two bin count n the sampling tables hold (edge, n * width)
+    and the in-bin offset carries the 1/n: the kernel's offset constant
+    (2^-24 / ns) * (1/n) and the table width n * width are exact rescalings, so
+    (n * width) * (offset / n) is the same real number as width * offset (the
+    point's fma rounds it once, identically) and the Jacobian factor n * width
+    is the value the unscaled form multiplies out.  Checked over FP32 and FP64
+    offsets, widths down to the smallest subnormals, and n = 2 .. 2^14 (other
+    bin counts keep the unscaled form)."""
+    rng = np.random.default_rng(11)
+    f32, f64 = np.float32, np.float64
+    ns = 8
+    for n in [2 ** k for k in range(1, 15)]:
+        # FP32: offset = bits * c with c = 2^-24 / ns, scaled by 1/n in the constant
+        bits = rng.integers(0, 2**32, 4000, dtype=np.uint64).astype(np.uint32)
+        c = f32(2.0**-24) * (f32(1) / f32(ns))
+        off = bits.astype(f32) * c
+        off_s = bits.astype(f32) * (c * (f32(1) / f32(n)))
+        assert np.array_equal((off_s * f32(n)).view(np.uint32), off.view(np.uint32)), n
+        w = np.concatenate([rng.random(2000).astype(f32) * f32(2.0**-20),
+                            np.array([np.finfo(f32).smallest_subnormal, f32(1e-40), f32(1.0)], dtype=f32)])
+        w_s = w * f32(n)
+        assert np.array_equal((w_s / f32(n)).view(np.uint32), w.view(np.uint32)), n
+        # products of the two forms agree exactly (both exact in FP64)
+        m = min(len(w), len(off))
+        assert np.array_equal(w_s[:m].astype(f64) * off_s[:m].astype(f64),
+                              w[:m].astype(f64) * off[:m].astype(f64)), n
+        # FP64: offset constant 2^-33 / ns scaled by 1/n; widths scaled by n
+        v = rng.integers(0, 2**41, 4000).astype(f64)
+        c64 = 2.0**-33 * (1.0 / ns)
+        assert np.array_equal((v * (c64 * (1.0 / n))) * n, v * c64), n
+        w64 = np.concatenate([rng.random(2000) * 2.0**-40, [5e-324, 1e-310, 1.0]])
+        assert np.array_equal((w64 * n) / n, w64), n
+
+
 def test_tree_sum_exact_and_close(orc):
     rng = np.random.default_rng(3)
     for m in [1, 5, 1023, 1024, 1025, 4096 + 17, 1024 * 1024 + 3]:
