@@ -11,7 +11,10 @@ out of the average (skip), --steps iterations are timed.  Defaults 5 + 15 = the
 Prints one JSON line: value = evaluations/s of the FP32-compensated form over
 the timed iterations, with the plain FP64 form, time to 1e-4 relative error,
 errors against the closed form, the roofline of the sampling kernel and the
-CPU oracle timed on a bounded sample.
+CPU oracle timed on a bounded sample.  Under "configs" the same line carries
+the other configurations of the path, each at its own size: the 16-D
+oscillatory integrand (osc16), the 6-D mixture forward and gradient (mix6)
+and the 512-integral batch against sequential launches (batch512).
 """
 from __future__ import annotations
 
@@ -101,7 +104,12 @@ def cpu_baseline(itg, family, nbins, n_strat, n_eval_it, seed):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     fid = {"double_gaussian": orc.F_DGAUSS, "oscillatory": orc.F_OSC, "gaussian_peak": orc.F_GAUSS}[family]
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    # every core of the affinity mask, up to the process's OpenMP thread budget
+    # (OMP_NUM_THREADS: the host share a one-GPU job is given; the machine's
+    # own count is recorded beside it)
+    affinity = len(os.sched_getaffinity(0))
+    budget = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    cores = min(affinity, budget) if budget > 0 else affinity
     out = {}
     for threads in (cores, 1):
         p, keep = orc.make_params(itg.n_dim, nbins, n_strat, n_eval_it, 1, fid,
@@ -114,19 +122,26 @@ def cpu_baseline(itg, family, nbins, n_strat, n_eval_it, seed):
     return {"value": v, "unit": "evals/s", "cores": cores, "kind": "port",
             "sample": f"first iteration of the same workload ({ne} evals, FP64), {dt:.1f} s on "
                       f"{cores} threads; 1 thread: {out[1][0]:.3e} evals/s ({out[1][1]:.1f} s)",
-            "value_1core": out[1][0]}
+            "value_1core": out[1][0], "nproc": os.cpu_count(), "affinity_cores": affinity,
+            "omp_num_threads": budget or None}
 
 
-def batch_bench(args):
-    """512 independent integrals (d ~ U{2..12}, Gaussian / oscillatory / corner
-    peak, budgets log-uniform in [1e4, 1e7] per iteration), 10 iterations,
-    FP32-compensated: one batched sampling launch per iteration in a hipGraph
-    replay, against the same kernels launched one integral at a time."""
+def _workloads():
+    with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
+        return json.load(f)
+
+
+def batch_bench(args, sequential=True):
+    """512 independent integrals from the committed fixture (d ~ U{2..12},
+    Gaussian / oscillatory / corner peak, budgets log-uniform in [1e4, 1e7] per
+    iteration), 10 iterations, FP32-compensated: one batched sampling launch
+    per iteration in a hipGraph replay, against the same kernels launched one
+    integral at a time.  Returns the result dict."""
     import numpy as np
     from cuvegas_amd.autotune import autotune_batch, program
-    from cuvegas_amd.batch import FAMILY_SOURCE, Batch, serving_workload
+    from cuvegas_amd.batch import FAMILY_SOURCE, Batch, workload_from_items
     iters, skip = 10, 5
-    items, theta, exact, fams = serving_workload(512, seed=3)
+    items, theta, exact, fams = workload_from_items(_workloads()["batch_512"])
     if args.batch_config:
         blk, rl, tw = (int(v) for v in args.batch_config.split(","))
         tune = {"block": blk, "run_len": rl, "task_windows": tw, "table": []}
@@ -148,15 +163,37 @@ def batch_bench(args):
     # points inside it: N * (sigma sqrt(2 pi))^d >= 20.
     ok = np.array([fam != "gaussian" or N * (theta[off + 1] * math.sqrt(2 * math.pi)) ** d >= 20
                    for (d, N, off), fam in zip(items, fams)])
-    if args.skip_sequential:
-        print(json.dumps({"batched_wall_ms": batch_wall * 1e3, "device_loop_ms": r["loop_ms"],
-                          "evals": evals, "config": tune}), flush=True)
-        return
+    out = {
+        "metric": "batch of 512 integrals: total evals/s and p50/p99 per-integral completion (ms)",
+        "value": evals / batch_wall, "unit": "evals/s", "steps": iters, "dtype": "f32",
+        "data": "synthetic (seed 3 parameter draws, committed fixture tests/golden/workloads.json)",
+        "config": {"workload": "512 integrals, d~U{2..12}, Gaussian/oscillatory/corner, neval "
+                               "log-U[1e4,1e7]/iteration, nbins=256, 10 iterations, FP32-compensated",
+                   "block": tune["block"], "run_len": tune["run_len"],
+                   "task_windows": tune["task_windows"]},
+        # the items advance in lockstep, one launch per iteration for all of
+        # them: every item's last iteration ends in the same launch, so every
+        # per-integral completion time is the batch's wall time
+        "batched": {"wall_ms": batch_wall * 1e3, "device_loop_ms": r["loop_ms"], "evals": evals,
+                    "completion_p50_ms": batch_wall * 1e3, "completion_p99_ms": batch_wall * 1e3,
+                    "completion": "lockstep: all 512 complete with the last iteration's launch"},
+        "accuracy": {"pull_rms_resolvable": float(np.sqrt(np.mean(pull[ok] ** 2))),
+                     "n_resolvable": int(ok.sum()),
+                     "frac_within_3sigma_resolvable": float(np.mean(np.abs(pull[ok]) < 3)),
+                     "frac_within_3sigma": float(np.mean(np.abs(pull) < 3)),
+                     "median_rel_err": float(np.median(np.abs(r["result"] - exact) /
+                                                       np.maximum(np.abs(exact), 1e-300)))},
+        "autotune": tune["table"],
+    }
+    if not sequential:
+        return out
     # Sequential baseline: the same integrals one after another through the same
     # kernels.  Each integral runs once, so its iterations are launched eagerly
     # (a graph per integral costs its capture; measured slower), and the
     # baseline takes the launch shape that suits it best among the autotuner's
     # candidates (small integrals want small tasks), not the batch's choice.
+    # Completion of integral k = wall time from the start of the sequence to
+    # the end of its last iteration (integrals in fixture order).
     from cuvegas_amd.autotune import DEFAULT_CANDIDATES
     _log("sequential runs")
     seq = None
@@ -174,50 +211,29 @@ def batch_bench(args):
         _log("sequential", blk, rl, tw, f"{done[-1] * 1e3:.1f} ms")
         if seq is None or done[-1] < seq["wall"]:
             seq = {"wall": done[-1], "done": np.array(done) * 1e3, "evals": seq_evals, "config": (blk, rl, tw)}
-    seq_wall, done, seq_evals = seq["wall"], seq["done"], seq["evals"]
-    out = {
-        "metric": "batch of 512 integrals: total evals/s and p50/p99 per-integral completion (ms)",
-        "value": evals / batch_wall, "unit": "evals/s", "n_gpus": 1, "steps": iters, "warmup": 0,
-        "ms_per_step": batch_wall * 1e3 / iters, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seed 3 parameter draws)",
-        "config": {"workload": "512 integrals, d~U{2..12}, Gaussian/oscillatory/corner, neval "
-                               "log-U[1e4,1e7]/iteration, nbins=256, 10 iterations, FP32-compensated",
-                   "block": tune["block"], "run_len": tune["run_len"],
-                   "task_windows": tune["task_windows"]},
-        "batched": {"wall_ms": batch_wall * 1e3, "device_loop_ms": r["loop_ms"], "evals": evals,
-                    "completion_p50_ms": batch_wall * 1e3, "completion_p99_ms": batch_wall * 1e3},
-        "sequential": {"wall_ms": seq_wall * 1e3, "evals": seq_evals,
-                       "evals_per_s": seq_evals / seq_wall, "launch": "eager",
-                       "block": seq["config"][0], "run_len": seq["config"][1],
-                       "task_windows": seq["config"][2],
-                       "completion_p50_ms": float(np.percentile(done, 50)),
-                       "completion_p99_ms": float(np.percentile(done, 99))},
-        "speedup_vs_sequential": seq_wall / batch_wall,
-        "accuracy": {"pull_rms_resolvable": float(np.sqrt(np.mean(pull[ok] ** 2))),
-                     "n_resolvable": int(ok.sum()),
-                     "frac_within_3sigma_resolvable": float(np.mean(np.abs(pull[ok]) < 3)),
-                     "frac_within_3sigma": float(np.mean(np.abs(pull) < 3)),
-                     "median_rel_err": float(np.median(np.abs(r["result"] - exact) /
-                                                       np.maximum(np.abs(exact), 1e-300)))},
-        "autotune": tune["table"],
-    }
-    print(json.dumps(out), flush=True)
+    out["sequential"] = {"wall_ms": seq["wall"] * 1e3, "evals": seq["evals"],
+                         "evals_per_s": seq["evals"] / seq["wall"], "launch": "eager",
+                         "block": seq["config"][0], "run_len": seq["config"][1],
+                         "task_windows": seq["config"][2],
+                         "completion_p50_ms": float(np.percentile(seq["done"], 50)),
+                         "completion_p99_ms": float(np.percentile(seq["done"], 99))}
+    out["speedup_vs_sequential"] = seq["wall"] / batch_wall
+    return out
 
 
-def mix6_bench(args):
+def mix6_bench(args, neval=None):
     """6-D four-component Gaussian mixture (32 parameters from the committed
     fixture, numpy default_rng(2)): 5e7 evaluations over 10 iterations (5e6 per
     iteration, 5 averaged), forward with recording, then the backward replay of
     the 5 recorded iterations, d value / d theta.  Reports forward evals/s per
-    form, backward time, and the gradient's pulls against the closed-form erf
-    gradient."""
+    form, backward time (HIP events around the replay), and the gradient's pulls
+    against the closed-form erf gradient.  Returns the result dict."""
     import numpy as np
     from cuvegas_amd import Integrator
     from cuvegas_amd import integrands as I
-    with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
-        theta = np.array(json.load(f)["mixture_6d"]["theta"])
+    theta = np.array(_workloads()["mixture_6d"]["theta"])
     d, nbins, iters, skip = 6, 1024, 10, 5
-    n_eval_it = int(args.neval or 5e6)
+    n_eval_it = int(neval or 5e6)
     itg = I.gaussian_mixture_param(d, theta)
     exact_grad = np.array(itg.exact_grad(theta))
     forms = {}
@@ -237,6 +253,7 @@ def mix6_bench(args):
         for _ in range(reps):
             v, vv, g, gv = st.backward()
         bw_ms = (time.perf_counter() - t0) * 1e3 / reps
+        bw_dev = st.backward_timing()
         replayed = int(ne[skip:].sum())
         pull = (g - exact_grad) / np.sqrt(gv)
         forms[prec] = {
@@ -245,24 +262,25 @@ def mix6_bench(args):
             "forward_ms_per_iteration": float(iter_ms.mean()), "fill_ms_avg": float(fill_ms.mean()),
             "result": r["result"], "error": r["error"], "chi2_dof": r["chi2_dof"],
             "pull_value": (r["result"] - itg.exact) / r["error"],
-            "backward_ms": bw_ms, "backward_replayed_evals": replayed,
+            "backward_ms": bw_ms, "backward_replay_kernel_ms": bw_dev,
+            "backward_replayed_evals": replayed,
             "backward_evals_per_s": replayed / (bw_ms * 1e-3),
             "grad_pull_rms": float(np.sqrt(np.mean(pull ** 2))),
             "grad_pull_max": float(np.max(np.abs(pull))), "n_theta": int(theta.size),
+            "grad_within_3sigma": bool(np.all(np.abs(pull) < 3)),
         }
         _log(prec, json.dumps(forms[prec]))
         st.close()
     head = forms.get("fp32") or forms["fp64"]
-    print(json.dumps({
+    return {
         "metric": "6-D Gaussian mixture: forward evals/s, backward (32-parameter gradient) time",
-        "value": head["forward_evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": iters,
-        "warmup": 0, "ms_per_step": head["forward_ms_per_iteration"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if head is forms.get("fp32") else "f64",
+        "value": head["forward_evals_per_s"], "unit": "evals/s", "steps": iters,
+        "dtype": "f32" if head is forms.get("fp32") else "f64",
         "data": "synthetic (mixture parameters from numpy default_rng(2), committed fixture)",
         "config": {"workload": f"6-D 4-component Gaussian mixture, nbins={nbins}, {n_eval_it:.0e} "
                                f"evals/iteration, {iters} iterations ({iters - skip} averaged and "
                                "replayed by the backward)", "exact": itg.exact},
-        "forms": forms}), flush=True)
+        "forms": forms}
 
 
 def copy_probe():
@@ -298,6 +316,97 @@ def _traffic_from_profile(workload):
     return None
 
 
+def single_bench(args, workload, warmup, steps, with_cpu=True, neval=None):
+    """One single-integral workload in both forms (dg8, osc16 or gauss4):
+    evals/s over the timed iterations, time to 1e-4, errors against the closed
+    form, the sampling kernel's roofline and the CPU oracle on a bounded
+    sample.  Returns the result dict (the bench line's fields)."""
+    from cuvegas_amd import integrands as I
+    from cuvegas_amd import lib
+    from cuvegas_amd.roofline import slots_per_eval, valu_peak
+    import ctypes as C
+
+    seed = 0
+    if workload == "dg8":
+        d, nbins, n_strat, family = 8, 1024, 8, "double_gaussian"
+        n_eval_it = int(neval or 1e8)
+        itg = I.double_gaussian(d, 0.33, 0.67, 0.05)
+        desc = (f"8-D double Gaussian peak sigma=0.05, nbins=1024, n_strat=8 (16,777,216 cubes), "
+                f"{n_eval_it:.0e} evals/iteration")
+        cpu_neval = n_eval_it
+    elif workload == "gauss4":
+        # the CPU configuration's integrand (4-D normalised Gaussian, sigma = 0.1, nbins 1024)
+        # at 1e8 evaluations per iteration; n_strat by the reference's formula
+        d, nbins, family = 4, 1024, "gaussian_peak"
+        n_eval_it = int(neval or 1e8)
+        n_strat = max(1, int(math.pow(n_eval_it / 4.0, 1.0 / d)))
+        itg = I.gaussian_peak(d, 0.5, 0.1)
+        desc = (f"4-D Gaussian peak sigma=0.1, nbins=1024, n_strat={n_strat} ({n_strat ** d:,} cubes), "
+                f"{n_eval_it:.0e} evals/iteration")
+        cpu_neval = n_eval_it
+    else:  # osc16: parameters drawn once (numpy default_rng(1)), committed as a fixture
+        osc = _workloads()["oscillatory_16d"]
+        d, nbins, n_strat, family = 16, 512, 2, "oscillatory"
+        n_eval_it = int(neval or 2e8)
+        itg = I.oscillatory(d, osc["u"], osc["a"])
+        desc = (f"16-D oscillatory cos(2 pi u + sum a_i x_i), a_i ~ U(0,5), nbins=512, n_strat=2 "
+                f"(65,536 cubes), {n_eval_it:.0e} evals/iteration")
+        cpu_neval = n_eval_it // 10  # bounded CPU sample: ~1/10 of an iteration
+
+    n_cu, clk = C.c_int(0), C.c_int(0)
+    arch = C.create_string_buffer(64)
+    rc = lib.cvg_device_info(C.byref(n_cu), C.byref(clk), arch, 64)
+    if rc != 0:
+        sys.exit("no HIP device: " + lib.cvg_last_error().decode())
+    clock_hz = clk.value * 1e3
+    _log(workload, "device", arch.value.decode(), n_cu.value, "CUs", clock_hz / 1e9, "GHz")
+
+    forms = {}
+    for prec in (["fp64", "fp32"] if args.only is None else [args.only]):
+        _log(workload, "running", prec)
+        forms[prec] = run_form(prec, itg, n_eval_it, warmup, steps, n_strat, nbins, seed)
+        _log(workload, prec, json.dumps(forms[prec]))
+    head = forms.get("fp32") or forms["fp64"]
+
+    S = slots_per_eval(d, family)
+    peak = valu_peak(n_cu.value, clock_hz)
+    achieved = S * head["evals_per_fill_avg"] / (head["fill_ms_avg"] * 1e-3)
+    roof = {"bound": "valu", "kernel": "vg_fill",
+            "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tslot/s",
+            "frac": achieved / peak, "slots_per_eval": S,
+            "traffic": _traffic_from_profile(workload) if head is forms.get("fp32") else None,
+            "fill_ms_avg": head["fill_ms_avg"], "evals_per_fill": head["evals_per_fill_avg"]}
+    if roof["traffic"]:
+        # the same launch against the HBM roofline: measured bytes per launch / its duration
+        from cuvegas_amd.roofline import MI355X_HBM_BPS
+        hbm = roof["traffic"] / (head["fill_ms_avg"] * 1e-3)
+        roof["hbm"] = {"achieved": hbm / 1e9, "peak": MI355X_HBM_BPS / 1e9, "unit": "GB/s",
+                       "frac": hbm / MI355X_HBM_BPS}
+
+    cpu = None
+    if with_cpu:
+        _log(workload, "cpu baseline")
+        cpu = cpu_baseline(itg, family, nbins, n_strat, cpu_neval, seed)
+
+    return {
+        "metric": "integrand evals/s + time to 1e-4 rel. error, FP32-comp vs plain FP64, 1 GPU",
+        "value": head["evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if head is forms.get("fp32") else "f64",
+        "data": "synthetic (closed-form integrand, Philox4x32-10 seed 0)",
+        "config": {"workload": f"{desc}, alpha=0.5, beta=0.75, {warmup}+{steps} iterations",
+                   "n_eval_it": n_eval_it, "exact": itg.exact},
+        "forms": forms,
+        "fp32_over_fp64": (forms["fp32"]["evals_per_s"] / forms["fp64"]["evals_per_s"]
+                           if "fp32" in forms and "fp64" in forms else None),
+        "fp32_vs_fp64": _form_difference(forms),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "device": {"arch": arch.value.decode(), "n_cu": n_cu.value, "clock_hz": clock_hz},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -306,13 +415,16 @@ def main():
     ap.add_argument("--neval", type=float, default=None,
                     help="evaluations per iteration (default: the workload's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="dg8: leave out the osc16, mix6 and batch512 workloads that the default line carries")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
     ap.add_argument("--batch-config", default="",
                     help="batch512: 'block,run_len,task_windows' instead of autotuning")
     ap.add_argument("--skip-sequential", action="store_true",
                     help="batch512: time the batched run only (for profiling)")
     ap.add_argument("--workload", choices=["dg8", "osc16", "gauss4", "mix6", "batch512", "copy"], default="dg8",
-                    help="dg8: the 8-D double Gaussian headline; osc16: 16-D oscillatory; "
+                    help="dg8: the 8-D double Gaussian headline (with the other configurations under "
+                         "extra keys unless --no-extras); osc16: 16-D oscillatory; "
                          "gauss4: the 4-D Gaussian of the CPU configuration at 1e8 evals/iteration "
                          "(a cheap integrand: bin atomics dominate); "
                          "mix6: 6-D mixture forward + gradient; batch512: 512-integral batch; "
@@ -325,100 +437,31 @@ def main():
     if args.steps < 1 or args.warmup < 0:
         sys.exit("steps must be >= 1 and warmup >= 0")
     if args.workload == "batch512":
-        batch_bench(args)
+        out = batch_bench(args, sequential=not args.skip_sequential)
+        out.update(n_gpus=1, warmup=0, ms_per_step=out["batched"]["wall_ms"] / out["steps"],
+                   higher_is_better=True, scaling="weak", vs_baseline=None)
+        print(json.dumps(out), flush=True)
         return
     if args.workload == "copy":
         copy_probe()
         return
     if args.workload == "mix6":
-        mix6_bench(args)
+        out = mix6_bench(args, args.neval)
+        out.update(n_gpus=1, warmup=0, ms_per_step=out["forms"][next(iter(out["forms"]))]["forward_ms_per_iteration"],
+                   higher_is_better=True, scaling="weak", vs_baseline=None)
+        print(json.dumps(out), flush=True)
         return
-
-    from cuvegas_amd import integrands as I
-    from cuvegas_amd import lib
-    from cuvegas_amd.roofline import slots_per_eval, valu_peak
-    import ctypes as C
-
-    seed = 0
-    if args.workload == "dg8":
-        d, nbins, n_strat, family = 8, 1024, 8, "double_gaussian"
-        n_eval_it = int(args.neval or 1e8)
-        itg = I.double_gaussian(d, 0.33, 0.67, 0.05)
-        desc = (f"8-D double Gaussian peak sigma=0.05, nbins=1024, n_strat=8 (16,777,216 cubes), "
-                f"{n_eval_it:.0e} evals/iteration")
-        cpu_neval = n_eval_it
-    elif args.workload == "gauss4":
-        # the CPU configuration's integrand (4-D normalised Gaussian, sigma = 0.1, nbins 1024)
-        # at 1e8 evaluations per iteration; n_strat by the reference's formula
-        d, nbins, family = 4, 1024, "gaussian_peak"
-        n_eval_it = int(args.neval or 1e8)
-        n_strat = max(1, int(math.pow(n_eval_it / 4.0, 1.0 / d)))
-        itg = I.gaussian_peak(d, 0.5, 0.1)
-        desc = (f"4-D Gaussian peak sigma=0.1, nbins=1024, n_strat={n_strat} ({n_strat ** d:,} cubes), "
-                f"{n_eval_it:.0e} evals/iteration")
-        cpu_neval = n_eval_it
-    else:  # osc16: parameters drawn once (numpy default_rng(1)), committed as a fixture
-        with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
-            osc = json.load(f)["oscillatory_16d"]
-        d, nbins, n_strat, family = 16, 512, 2, "oscillatory"
-        n_eval_it = int(args.neval or 2e8)
-        itg = I.oscillatory(d, osc["u"], osc["a"])
-        desc = (f"16-D oscillatory cos(2 pi u + sum a_i x_i), a_i ~ U(0,5), nbins=512, n_strat=2 "
-                f"(65,536 cubes), {n_eval_it:.0e} evals/iteration")
-        cpu_neval = n_eval_it // 10  # bounded CPU sample: ~1/10 of an iteration
-
-    n_cu, clk = C.c_int(0), C.c_int(0)
-    arch = C.create_string_buffer(64)
-    rc = lib.cvg_device_info(C.byref(n_cu), C.byref(clk), arch, 64)
-    if rc != 0:
-        sys.exit("no HIP device: " + lib.cvg_last_error().decode())
-    clock_hz = clk.value * 1e3
-    _log("device", arch.value.decode(), n_cu.value, "CUs", clock_hz / 1e9, "GHz")
-
-    forms = {}
-    for prec in (["fp64", "fp32"] if args.only is None else [args.only]):
-        _log("running", prec)
-        forms[prec] = run_form(prec, itg, n_eval_it, args.warmup, args.steps, n_strat, nbins, seed)
-        _log(prec, json.dumps(forms[prec]))
-    head = forms.get("fp32") or forms["fp64"]
-
-    S = slots_per_eval(d, family)
-    peak = valu_peak(n_cu.value, clock_hz)
-    achieved = S * head["evals_per_fill_avg"] / (head["fill_ms_avg"] * 1e-3)
-    roof = {"bound": "valu", "kernel": "vg_fill",
-            "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tslot/s",
-            "frac": achieved / peak, "slots_per_eval": S,
-            "traffic": _traffic_from_profile(args.workload) if head is forms.get("fp32") else None,
-            "fill_ms_avg": head["fill_ms_avg"], "evals_per_fill": head["evals_per_fill_avg"]}
-    if roof["traffic"]:
-        # the same launch against the HBM roofline: measured bytes per launch / its duration
-        from cuvegas_amd.roofline import MI355X_HBM_BPS
-        hbm = roof["traffic"] / (head["fill_ms_avg"] * 1e-3)
-        roof["hbm"] = {"achieved": hbm / 1e9, "peak": MI355X_HBM_BPS / 1e9, "unit": "GB/s",
-                       "frac": hbm / MI355X_HBM_BPS}
-
-    cpu = None
-    if not args.no_cpu:
-        _log("cpu baseline")
-        cpu = cpu_baseline(itg, family, nbins, n_strat, cpu_neval, seed)
-
-    out = {
-        "metric": "integrand evals/s + time to 1e-4 rel. error, FP32-comp vs plain FP64, 1 GPU",
-        "value": head["evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if head is forms.get("fp32") else "f64",
-        "data": "synthetic (closed-form integrand, Philox4x32-10 seed 0)",
-        "config": {"workload": f"{desc}, alpha=0.5, beta=0.75, {args.warmup}+{args.steps} iterations",
-                   "n_eval_it": n_eval_it, "exact": itg.exact},
-        "forms": forms,
-        "fp32_over_fp64": (forms["fp32"]["evals_per_s"] / forms["fp64"]["evals_per_s"]
-                           if "fp32" in forms and "fp64" in forms else None),
-        "fp32_vs_fp64": _form_difference(forms),
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "device": {"arch": arch.value.decode(), "n_cu": n_cu.value, "clock_hz": clock_hz},
-    }
+    out = single_bench(args, args.workload, args.warmup, args.steps, with_cpu=not args.no_cpu, neval=args.neval)
+    if args.workload == "dg8" and not args.no_extras:
+        # the other configurations of the hot path, each at its own size and
+        # iteration count, in the same line (their own timed regions)
+        extras = {}
+        osc = single_bench(args, "osc16", 5, 10, with_cpu=False)
+        extras["osc16"] = {k: osc[k] for k in ("value", "unit", "ms_per_step", "dtype", "config", "forms",
+                                              "fp32_over_fp64", "fp32_vs_fp64", "roofline")}
+        extras["mix6"] = mix6_bench(args)
+        extras["batch512"] = batch_bench(args)
+        out["configs"] = extras
     print(json.dumps(out), flush=True)
 
 

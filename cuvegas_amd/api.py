@@ -175,6 +175,7 @@ class Integrator:
         self.n_dim, self.n_intervals, self.n_strat = n_dim, n_intervals, n_strat
         self.n_cubes = int(lib.cvg_state_n_cubes(h))
         self.max_it, self.skip = max_it, skip
+        self.theta = None
         if theta is not None:
             self.set_theta(theta)
 
@@ -264,11 +265,13 @@ class Integrator:
     # -- on-disk state -------------------------------------------------------
     # An .npz archive of plain arrays (np.load(path, allow_pickle=False) reads
     # it): the shape and run parameters, the map edges, the allocation n_h and
-    # the per-iteration (I_k, sigma_k^2, samples).  Restoring it reproduces the
-    # iteration the saved run would have run next: same map, same allocation
-    # and sample layout, same Philox counter block.  Not saved: the compiled
-    # integrand (the caller passes the source again), the bin accumulators
-    # (cleared at the end of every iteration) and the autograd record.
+    # the per-iteration (I_k, sigma_k^2, samples), and the parameters theta of
+    # a parametrised integrand.  Restoring it reproduces the iteration the
+    # saved run would have run next: same map, same allocation and sample
+    # layout, same Philox counter block, same parameters.  Not saved: the
+    # compiled integrand (the caller passes the source again), the bin
+    # accumulators (cleared at the end of every iteration) and the autograd
+    # record.
 
     def save(self, path) -> None:
         """Write the run's state (see above) to `path` (.npz)."""
@@ -279,7 +282,8 @@ class Integrator:
                  n_strat=p.n_strat, n_eval_it=p.n_eval_it, max_it=p.max_it, skip=p.skip,
                  alpha=p.alpha, beta=p.beta, seed=p.seed, stream_id=p.stream_id,
                  precision=p.precision, run_len=p.run_len, domain=dom, x_edges=self.get_map(),
-                 nh=self.get_nh(), I=I, sigma2=S2, nevals=ne)
+                 nh=self.get_nh(), I=I, sigma2=S2, nevals=ne, n_theta=p.n_theta,
+                 theta=self.theta if self.theta is not None else np.zeros(0))
 
     def restore(self, path) -> "Integrator":
         """Load a saved state into this integrator (same n_dim, n_intervals, n_strat)."""
@@ -292,9 +296,15 @@ class Integrator:
                                  f"{(self.n_dim, self.n_intervals, self.n_strat)}")
             if z["I"].size > self.max_it:
                 raise ValueError(f"saved run has {z['I'].size} iterations; max_it is {self.max_it}")
+            n_theta = int(z["n_theta"]) if "n_theta" in z else 0
+            if n_theta != self.params.n_theta:
+                raise ValueError(f"saved integrand has {n_theta} parameters; this integrator has "
+                                 f"{self.params.n_theta}")
             self.set_map(z["x_edges"])
             self.set_nh(z["nh"])
             self.set_history(z["I"], z["sigma2"], z["nevals"])
+            if n_theta > 0:
+                self.set_theta(z["theta"])
         return self
 
     @classmethod
@@ -312,12 +322,14 @@ class Integrator:
                       precision=int(z["precision"]), stream_id=int(z["stream_id"]),
                       run_len=int(z["run_len"]), domain=dom.reshape(2, -1) if dom.size else None)
             n_dim, n_eval_it = int(z["n_dim"]), int(z["n_eval_it"])
-        itg = cls(source, n_dim, n_eval_it, options=options, program=program, **kw)
+            theta = z["theta"].copy() if "n_theta" in z and int(z["n_theta"]) > 0 else None
+        itg = cls(source, n_dim, n_eval_it, options=options, program=program, theta=theta, **kw)
         return itg.restore(path)
 
     def set_theta(self, theta) -> None:
-        t = np.ascontiguousarray(theta, dtype=np.float64)
+        t = np.ascontiguousarray(theta, dtype=np.float64).ravel()
         check(lib.cvg_state_set_theta(self._h, _dptr(t), int(t.size)))
+        self.theta = t.copy()
 
     def set_stream(self, hip_stream: int) -> None:
         check(lib.cvg_state_set_stream(self._h, C.c_void_p(hip_stream)))
@@ -364,6 +376,12 @@ class Integrator:
         check(lib.cvg_state_backward(self._h, None if th is None else _dptr(th), _dptr(g), _dptr(gv),
                                      C.byref(v), C.byref(vv)))
         return v.value, vv.value, g, gv
+
+    def backward_timing(self) -> float:
+        """Device time (ms) of the last backward()'s replay kernels."""
+        ms = C.c_double()
+        check(lib.cvg_state_backward_timing(self._h, C.byref(ms)))
+        return ms.value
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -450,7 +468,7 @@ def integrate(device_function, n_dim, domain=None, n_evals=1000000, max_it=20, s
         "error": r["error"], "n_strat": n_strat, "n_cubes": r["n_cubes"],
         "n_intervals": n_intervals, "tot_nevals": r["tot_nevals"], "tot_time": tot_ms,
         "avg_it_time": r["loop_ms"] / max_it, "seed": seed, "multi_gpu": bool(multi_gpu),
-        "multi_cpu": bool(multi_cpu), "gpu_num": 1, "omp_threads_num": None, "gpu_thr": gpu_thr,
+        "multi_cpu": bool(multi_cpu), "gpu_num": 1, "omp_threads_num": 0, "gpu_thr": gpu_thr,
         "chi2_dof": r["chi2_dof"], "iterations": list(zip(I.tolist(), np.sqrt(S2).tolist())),
         "evals_per_s": r["tot_nevals"] / (r["loop_ms"] * 1e-3) if r["loop_ms"] > 0 else float("nan"),
         "precision": precision, "compile_ms": r["compile_ms"], "setup_ms": r["setup_ms"],

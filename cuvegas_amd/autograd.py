@@ -15,6 +15,7 @@ PyTorch is used only for the autograd plumbing; theta is copied to the host.
 """
 from __future__ import annotations
 
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -34,12 +35,15 @@ class _VegasIntegralFn(torch.autograd.Function):
                        "tot_nevals": r["tot_nevals"]}
         ctx.integ = integ
         ctx.module = module
+        module._pending.add(integ)
         return theta.new_tensor(r["result"])
 
     @staticmethod
     def backward(ctx, grad_out: torch.Tensor):
-        _, _, g, gv = ctx.integ.backward()
-        ctx.module.last_grad_var = gv
+        integ, module = ctx.integ, ctx.module
+        _, _, g, gv = integ.backward()
+        module._pending.discard(integ)
+        module.last_grad_var = gv
         grad = torch.as_tensor(g, dtype=grad_out.dtype, device=grad_out.device)
         return grad_out * grad, None
 
@@ -66,8 +70,17 @@ class DifferentiableIntegral:
         self.last = None
         self.last_grad_var = None
         self.state: Optional[Integrator] = None
+        # integrators of forwards whose backward has not run yet
+        self._pending: "weakref.WeakSet[Integrator]" = weakref.WeakSet()
 
     def _new_state(self, theta: np.ndarray) -> Integrator:
+        # the previous forward's device buffers are released now, unless a
+        # graph whose backward has not run yet still needs them (it replays on
+        # that integrator, which is freed with the graph)
+        if self.state is not None:
+            old, self.state = self.state, None
+            if old not in self._pending:
+                old.close()
         integ = Integrator(self.source, self.n_dim, self.n_eval_it, theta=theta,
                            options=self.options, program=self._program, **self.kw)
         self._program = integ.program

@@ -14,6 +14,7 @@ import json
 import os
 import sys
 import time
+import uuid
 from typing import Dict, Sequence, Tuple
 
 from .batch import Batch, BatchProgram
@@ -32,7 +33,9 @@ def _cache_path() -> str:
 def _signature(source, items, precision, n_intervals, max_dim, device) -> str:
     h = hashlib.sha256()
     h.update(source.encode())
-    h.update(json.dumps([[int(x) for x in it[:2]] for it in items]).encode())
+    # every field of an item: dimension, budget, parameter offset and an
+    # explicit n_strat all change the cube count, the windows or the cost
+    h.update(json.dumps([[int(x) for x in it] for it in items]).encode())
     h.update(f"{precision}|{n_intervals}|{max_dim}|{device}".encode())
     return h.hexdigest()[:32]
 
@@ -85,7 +88,9 @@ def autotune_batch(source: str, items: Sequence, theta: Sequence[float], *, max_
         try:
             os.makedirs(os.path.dirname(path), exist_ok=True)
             cache[sig] = out
-            tmp = f"{path}.{os.getpid()}.tmp"  # whole-file replace: readers never see a partial file
+            # whole-file replace: readers never see a partial file; the name is
+            # unique per call, so threads of one process do not share it
+            tmp = f"{path}.{os.getpid()}.{uuid.uuid4().hex}.tmp"
             with open(tmp, "w") as f:
                 json.dump(cache, f)
             os.replace(tmp, path)

@@ -141,35 +141,62 @@ __device__ vg_real integrand(vg_real* x, const vg_real* th, int d) {
 """
 
 
-def serving_workload(n: int = 512, seed: int = 3):
-    """The 512-integral workload: d ~ U{2..12}; family uniform among Gaussian peak
-    (sigma ~ U(0.02,0.2), centre ~ U(0.3,0.7)^d), oscillatory (u ~ U(0,1),
-    a_i ~ U(0,5)) and corner peak (a_i ~ U(0.5,3)); per-iteration budget
-    log-uniform in [1e4, 1e7]; numpy default_rng(seed).
-    Returns (items, theta, exact, family names)."""
-    from . import integrands as I
+def draw_serving_items(n: int = 512, seed: int = 3):
+    """Parameter draws of the serving-style workload, as plain dicts: d ~
+    U{2..12}; family uniform among Gaussian peak (sigma ~ U(0.02,0.2), centre ~
+    U(0.3,0.7)^d), oscillatory (u ~ U(0,1), a_i ~ U(0,5)) and corner peak
+    (a_i ~ U(0.5,3)); per-iteration budget log-uniform in [1e4, 1e7]; numpy
+    default_rng(seed), in that order per item.  The 512-integral draw of seed 3
+    is committed as a fixture (tests/golden/workloads.json, key batch_512) and
+    read from there by the benchmark and the tests."""
     rng = np.random.default_rng(seed)
-    items, theta, exact, fams = [], [], [], []
+    out = []
     for _ in range(n):
         d = int(rng.integers(2, 13))
         fam = ["gaussian", "oscillatory", "corner"][int(rng.integers(0, 3))]
         neval = int(round(math.exp(rng.uniform(math.log(1e4), math.log(1e7)))))
+        if fam == "gaussian":
+            p = {"sigma": float(rng.uniform(0.02, 0.2)), "centre": rng.uniform(0.3, 0.7, d).tolist()}
+        elif fam == "oscillatory":
+            p = {"u": float(rng.uniform(0, 1)), "a": rng.uniform(0, 5, d).tolist()}
+        else:
+            p = {"a": rng.uniform(0.5, 3.0, d).tolist()}
+        out.append({"d": d, "family": fam, "neval": neval, **p})
+    return out
+
+
+def workload_from_items(entries):
+    """Batch items, the parameter block of FAMILY_SOURCE and the closed forms
+    for a list of item dicts ({"d", "family", "neval"} plus the family's
+    parameters, as draw_serving_items returns and the fixture holds).
+    Returns (items, theta, exact, family names); item i's parameters start at
+    theta[items[i][2]]: [0, sigma, norm, -1/(2 sigma^2), centre[d]],
+    [1, u, a[d]] or [2, a[d]]."""
+    from . import integrands as I
+    items, theta, exact, fams = [], [], [], []
+    for e in entries:
+        d, fam, neval = int(e["d"]), e["family"], int(e["neval"])
         off = len(theta)
         if fam == "gaussian":
-            sigma = float(rng.uniform(0.02, 0.2))
-            centre = rng.uniform(0.3, 0.7, d).tolist()
+            sigma, centre = float(e["sigma"]), [float(c) for c in e["centre"]]
             norm = (1.0 / (sigma * math.sqrt(2.0 * math.pi))) ** d
             theta += [0.0, sigma, norm, -1.0 / (2.0 * sigma * sigma), *centre]
             exact.append(math.prod(I._erf_mass(c, sigma) for c in centre))
         elif fam == "oscillatory":
-            u = float(rng.uniform(0, 1))
-            a = rng.uniform(0, 5, d).tolist()
+            u, a = float(e["u"]), [float(v) for v in e["a"]]
             theta += [1.0, u, *a]
             exact.append(I.oscillatory(d, u, a).exact)
-        else:
-            a = rng.uniform(0.5, 3.0, d).tolist()
+        elif fam == "corner":
+            a = [float(v) for v in e["a"]]
             theta += [2.0, *a]
             exact.append(I.corner_peak(d, a).exact)
+        else:
+            raise ValueError(f"unknown family {fam!r}")
         items.append((d, neval, off))
         fams.append(fam)
     return items, theta, np.array(exact), fams
+
+
+def serving_workload(n: int = 512, seed: int = 3):
+    """draw_serving_items(n, seed) as (items, theta, exact, family names)."""
+    return workload_from_items(draw_serving_items(n, seed))

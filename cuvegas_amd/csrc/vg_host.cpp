@@ -103,14 +103,32 @@ struct cvg_program {
     hipModule_t mod = nullptr;
     hipFunction_t fill = nullptr, dump = nullptr, grad = nullptr, grad_reduce = nullptr, bfill = nullptr;
     int fill_blocks_per_cu = 0;
+    // guards the module fields: states created from several threads over one
+    // program load it once
+    std::mutex load_mu;
 
     int load() {
         int dev;
         VG_HIP(hipGetDevice(&dev));
+        std::lock_guard<std::mutex> lock(load_mu);
         if (mod && dev == device) return CVG_OK;
         if (mod) return fail(CVG_ERR_INVALID, "program was loaded on another device");
         const double t0 = now_ms();
         VG_HIP(hipModuleLoadData(&mod, code.data()));
+        const int rc = bind_functions();
+        if (rc) {
+            // leave the program unloaded, as before the call
+            (void)hipModuleUnload(mod);
+            mod = nullptr;
+            fill = dump = grad = grad_reduce = bfill = nullptr;
+            return rc;
+        }
+        if (fill_blocks_per_cu < 1) fill_blocks_per_cu = 1;
+        device = dev;
+        compile_ms += now_ms() - t0;
+        return CVG_OK;
+    }
+    int bind_functions() {
         if (batch) {
             VG_HIP(hipModuleGetFunction(&bfill, mod, "vg_bfill"));
             VG_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&fill_blocks_per_cu, bfill, block, 0));
@@ -123,9 +141,6 @@ struct cvg_program {
             }
             VG_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&fill_blocks_per_cu, fill, block, 0));
         }
-        if (fill_blocks_per_cu < 1) fill_blocks_per_cu = 1;
-        device = dev;
-        compile_ms += now_ms() - t0;
         return CVG_OK;
     }
     ~cvg_program() {
@@ -207,6 +222,8 @@ struct cvg_state {
     double* bw_part = nullptr;  // [2 + 2 n_theta][bw_units] unit partials, then the totals
     int bw_units = 0, bw_rows_per_unit = 0;
     void* bw_theta = nullptr;   // parameters of a replay at another theta (n_theta doubles)
+    hipEvent_t bw_ev0 = nullptr, bw_ev1 = nullptr;
+    double bw_replay_ms = 0.0;  // device time of the last backward's replay kernels
     std::vector<char> recorded;  // per iteration: snapshot taken
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // optional per-iteration timing (eager launches): fill and whole iteration
@@ -221,6 +238,8 @@ struct cvg_state {
         if (graph) (void)hipGraphDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (bw_ev0) (void)hipEventDestroy(bw_ev0);
+        if (bw_ev1) (void)hipEventDestroy(bw_ev1);
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         void* bufs[] = {x_edges, map64, map32, weights, counts, refine_scratch, nh, nh_sum, dh,
                         JS, run_cube, win_first, win_last, part, bsum, boff, sync, s_total,
@@ -406,11 +425,14 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
                     bin_bytes + map_bytes <= kLdsBytes;
 
     // Launch-shape knobs in the options string (consumed here, not passed to
-    // hipRTC): lds_bins=0|1, lds_map=0|1, block=64..1024 (multiple of 64),
-    // pair=0|1 (two samples per lane step in the sampling loop), pipe=0|1
-    // (one sample per step; 1: the next sample's map entries are requested
-    // before the current sample is evaluated).
+    // hipRTC): lds_bins=0|1, lds_map=0|1, bin32=0|1, block=64..1024 (multiple
+    // of 64), pair=0|1 (two samples per lane step in the sampling loop),
+    // pipe=0|1 (one sample per step; 1: the next sample's map entries are
+    // requested before the current sample is evaluated), grad=0|1.  All are
+    // read first and applied in a fixed order (bin32, then lds_bins, then
+    // lds_map), so the result does not depend on the order of the tokens.
     std::string rtc_extra;
+    int opt_lds_bins = -1, opt_lds_map = -1, opt_bin32 = -1;
     if (options) {
         std::istringstream ss(options);
         std::string tok;
@@ -419,20 +441,12 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
             const std::string key = eq == std::string::npos ? tok : tok.substr(0, eq);
             const int val = eq == std::string::npos ? 1 : std::atoi(tok.c_str() + eq + 1);
             if (key == "lds_bins") {
-                if (val && (long long)prog->nbins * (prog->bin32 ? 8 : 12) > kLdsBytes)
-                    return fail(CVG_ERR_INVALID, "the bins of one dimension do not fit in LDS");
-                prog->lds_bins = val ? 1 : 0;
-                if (!prog->lds_bins) prog->lds_map = 0;
+                opt_lds_bins = val ? 1 : 0;
             } else if (key == "lds_map") {
-                if (val && (prog->precision != CVG_FP32C || !prog->lds_bins || bin_bytes + map_bytes > kLdsBytes))
-                    return fail(CVG_ERR_INVALID, "map does not fit in LDS next to the bins");
-                prog->lds_map = val ? 1 : 0;
+                opt_lds_map = val ? 1 : 0;
             } else if (key == "bin32") {
                 if (val && prog->precision != CVG_FP32C) return fail(CVG_ERR_INVALID, "bin32 is for the FP32 form");
-                prog->bin32 = val ? 1 : 0;
-                bin_bytes = dn * (prog->bin32 ? 8 : 12);
-                set_bin_dims(prog->bin32 ? 8 : 12);
-                if (!prog->lds_bins || bin_bytes + map_bytes > kLdsBytes) prog->lds_map = 0;
+                opt_bin32 = val ? 1 : 0;
             } else if (key == "grad") {
                 if (shape->n_theta < 1) return fail(CVG_ERR_INVALID, "grad needs n_theta >= 1");
                 prog->has_grad = val ? 1 : 0;
@@ -450,6 +464,23 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
                 rtc_extra += tok + " ";
             }
         }
+    }
+    if (opt_bin32 >= 0) {
+        prog->bin32 = opt_bin32;
+        bin_bytes = dn * (prog->bin32 ? 8 : 12);
+        set_bin_dims(prog->bin32 ? 8 : 12);
+        if (!prog->lds_bins || bin_bytes + map_bytes > kLdsBytes) prog->lds_map = 0;
+    }
+    if (opt_lds_bins >= 0) {
+        if (opt_lds_bins && (long long)prog->nbins * (prog->bin32 ? 8 : 12) > kLdsBytes)
+            return fail(CVG_ERR_INVALID, "the bins of one dimension do not fit in LDS");
+        prog->lds_bins = opt_lds_bins;
+        if (!prog->lds_bins) prog->lds_map = 0;
+    }
+    if (opt_lds_map >= 0) {
+        if (opt_lds_map && (prog->precision != CVG_FP32C || !prog->lds_bins || bin_bytes + map_bytes > kLdsBytes))
+            return fail(CVG_ERR_INVALID, "map does not fit in LDS next to the bins");
+        prog->lds_map = opt_lds_map;
     }
     options = rtc_extra.c_str();
     const double t0 = now_ms();
@@ -1019,6 +1050,8 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         if ((rc = dalloc(&st->bw_map32, (long long)d * n))) return rc;
         if ((rc = dalloc(&st->bw_fold_list, 1))) return rc;
         if ((rc = dalloc(&st->bw_fold_count, 2))) return rc;
+        VG_HIP(hipEventCreate(&st->bw_ev0));
+        VG_HIP(hipEventCreate(&st->bw_ev1));
     }
     // parameters for the replay: given theta, or the forward's
     void* th = st->theta;
@@ -1045,6 +1078,7 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
     if (st->p.has_domain)
         for (int i = 0; i < d; ++i) vol *= st->domain_host[d + i];
     unsigned seed = st->p.seed, sid = st->p.stream_id;
+    st->bw_replay_ms = 0.0;
     for (int k : its) {
         const long long slot = k - st->p.skip;
         VG_HIP(hipMemcpyAsync(st->bw_x, st->snap_x + slot * nx, sizeof(double) * nx, hipMemcpyDeviceToDevice, st->stream));
@@ -1060,13 +1094,18 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         void* args[] = {(void*)&map, (void*)&snh, (void*)&st->bw_nh_sum, (void*)&C, (void*)&rpu, (void*)&itu,
                         (void*)&V, (void*)&st->domain, (void*)&vol, (void*)&th, (void*)&seed, (void*)&sid,
                         (void*)&st->bw_part};
+        VG_HIP(hipEventRecord(st->bw_ev0, st->stream));
         VG_HIP(hipModuleLaunchKernel(prog->grad, (unsigned)nu, 1, 1, 64, 1, 1, 0, st->stream, args, nullptr));
         double* out = st->bw_part + (long long)nu * NV;
         void* rargs[] = {(void*)&st->bw_part, (void*)&nu, (void*)&out};
         VG_HIP(hipModuleLaunchKernel(prog->grad_reduce, (unsigned)NV, 1, 1, 1024, 1, 1, 0, st->stream, rargs,
                                      nullptr));
+        VG_HIP(hipEventRecord(st->bw_ev1, st->stream));
         VG_HIP(hipMemcpyAsync(tot.data(), out, sizeof(double) * NV, hipMemcpyDeviceToHost, st->stream));
         VG_HIP(hipStreamSynchronize(st->stream));
+        float kms = 0.0f;
+        VG_HIP(hipEventElapsedTime(&kms, st->bw_ev0, st->bw_ev1));
+        st->bw_replay_ms += kms;
         const double w = 1.0 / hist[4 * (size_t)k + 1];  // forward weight, held fixed
         W += w;
         val += w * tot[0];
@@ -1083,6 +1122,12 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         if (grad_var) grad_var[q] = gv[q] / (W * W);
     }
     return st->check_err();
+}
+
+int cvg_state_backward_timing(cvg_state* st, double* replay_ms) {
+    if (!st || !replay_ms) return fail(CVG_ERR_INVALID, "NULL argument");
+    *replay_ms = st->bw_replay_ms;
+    return CVG_OK;
 }
 
 }  // extern "C"
@@ -1510,6 +1555,8 @@ int cvg_batch_create(cvg_program* prog, const cvg_params* common, const cvg_batc
         b->tot_xe += (long long)it.n_dim * (n + 1);
         b->tot_map += (long long)it.n_dim * n;
         if (b->tot_win * 64 > 2147483647LL * 16) return fail(CVG_ERR_CAPACITY, "batch too large");
+        // the sampling kernel indexes maps and bins with 32-bit (dim, bin) offsets
+        if (b->tot_map > 2147483647LL) return fail(CVG_ERR_CAPACITY, "batch too large: more than 2^31 map entries");
     }
     b->n_tasks = (int)task_item.size();
     b->n_refine = (int)rd_item.size();

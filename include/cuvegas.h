@@ -198,6 +198,10 @@ int cvg_state_set_record(cvg_state* st, int enable);
 int cvg_state_get_snapshot(cvg_state* st, int it, double* x_edges, int* nh);
 int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double* grad_var,
                        double* value, double* value_var);
+/* Device time (ms, HIP events on the state's stream) of the last backward
+ * call's replay kernels (the per-iteration replay and its unit reduction),
+ * summed over the recorded iterations. */
+int cvg_state_backward_timing(cvg_state* st, double* replay_ms);
 
 /* Batches (no reference counterpart): many independent integrals, each with
  * its own dimension, budget, strata, map and parameter block, advanced in

@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
-F_ONE, F_LINEAR, F_POLY, F_GAUSS, F_DGAUSS, F_OSC, F_CORNER, F_MIXTURE = range(8)
+F_ONE, F_LINEAR, F_POLY, F_GAUSS, F_DGAUSS, F_OSC, F_CORNER, F_MIXTURE, F_GAUSSC = range(9)
 
 
 def build():

@@ -465,12 +465,14 @@ ORC_EXPORT void orc_refine_dim(int n, double alpha, const double* weights, const
 }
 
 /* ------------------------------------------------------------------------ */
-/* Integrands (C restatements of the sources in cuvegas_amd/integrands.py).  */
+/* Integrands (C restatements of the sources in cuvegas_amd/integrands.py;   */
+/* ORC_F_GAUSSC and the oscillatory and corner forms also restate the        */
+/* families of the batched source, cuvegas_amd/batch.py FAMILY_SOURCE).      */
 /* ------------------------------------------------------------------------ */
 
 enum {
     ORC_F_ONE = 0, ORC_F_LINEAR = 1, ORC_F_POLY = 2, ORC_F_GAUSS = 3, ORC_F_DGAUSS = 4,
-    ORC_F_OSC = 5, ORC_F_CORNER = 6, ORC_F_MIXTURE = 7
+    ORC_F_OSC = 5, ORC_F_CORNER = 6, ORC_F_MIXTURE = 7, ORC_F_GAUSSC = 8
 };
 
 static const double ORC_PI = 3.14159265358979323846;
@@ -554,6 +556,11 @@ static double f64_eval(int id, const double* x, int d, const double* p) {
         return f;
     }
     case ORC_F_MIXTURE: return mix64(x, d, p, NULL);
+    case ORC_F_GAUSSC: { /* p = {norm, m, c_0..c_{d-1}}: norm * exp(m * sum (x_i - c_i)^2) */
+        double q = 0.0;
+        for (int i = 0; i < d; ++i) { double a = x[i] - p[2 + i]; q = fma(a, a, q); }
+        return p[0] * orc_exp(q * p[1]);
+    }
     }
     return NAN;
 }
@@ -589,6 +596,11 @@ static float f32_eval(int id, const float* x, int d, const float* p) {
         return f;
     }
     case ORC_F_MIXTURE: return mix32(x, d, p, NULL);
+    case ORC_F_GAUSSC: {
+        float q = 0.0f;
+        for (int i = 0; i < d; ++i) { float a = x[i] - p[2 + i]; q = fmaf(a, a, q); }
+        return p[0] * expf(q * p[1]);
+    }
     }
     return NAN;
 }

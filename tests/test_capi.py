@@ -4,6 +4,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -160,3 +161,21 @@ def test_program_cache_reuses_compiled_programs():
     assert len(api._PROGRAMS) == api._PROGRAMS_MAX
     assert all(v is not a for v in api._PROGRAMS.values())
     api._PROGRAMS.clear()
+
+
+def test_batch_workload_fixture_matches_its_draw():
+    """The committed 512-integral workload is the seed-3 draw of the serving
+    family generator, and its items and parameter blocks are built as the
+    batched integrand reads them."""
+    import json
+    import os
+    from cuvegas_amd.batch import draw_serving_items, workload_from_items
+    with open(os.path.join(os.path.dirname(__file__), "golden", "workloads.json")) as f:
+        fixture = json.load(f)["batch_512"]
+    assert draw_serving_items(512, 3) == fixture
+    items, theta, exact, fams = workload_from_items(fixture)
+    assert len(items) == 512 and set(fams) == {"gaussian", "oscillatory", "corner"}
+    for (d, N, off), fam, e in zip(items, fams, fixture):
+        assert d == e["d"] and N == e["neval"] and 2 <= d <= 12 and 10_000 <= N <= 10_000_000
+        assert theta[off] == {"gaussian": 0.0, "oscillatory": 1.0, "corner": 2.0}[fam]
+    assert np.all(np.isfinite(exact))

@@ -142,11 +142,14 @@ def test_gradient_over_domain_vs_finite_differences(prec):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_gradient_vs_closed_form(prec):
-    """Gradient within 4 sigma of the closed form per component, and the
-    squared pulls average below 2 (32 parameters)."""
+    """The 6-D mixture configuration (5e7 evaluations over 10 iterations: 5e6
+    per iteration, 1024 bins, n_strat 10 by the reference's formula, 5
+    iterations averaged and replayed): every one of the 32 gradient components
+    within 3 sigma of the closed-form erf gradient, the squared pulls
+    averaging below 2, the value within 4 sigma of its closed form."""
     th = _theta()
     _say("grad-closed-form", prec)
-    st, itg = _state(prec, th, N=2_000_000, iters=10, skip=5)
+    st, itg = _state(prec, th, N=5_000_000, iters=10, skip=5, nbins=1024, seed=0)
     st.run(10)
     r = st.result()
     v, vv, g, gv = st.backward()
@@ -156,7 +159,8 @@ def test_gradient_vs_closed_form(prec):
     assert abs(v - exact_v) < 4 * math.sqrt(vv)
     pulls = (g - exact_g) / np.sqrt(gv)
     print("max |pull|", np.abs(pulls).max(), "within 3 sigma", np.mean(np.abs(pulls) < 3))
-    assert np.all(np.abs(pulls) < 4)
+    assert st.n_strat == 10 and st.n_cubes == 10 ** 6
+    assert np.all(np.abs(pulls) < 3), pulls
     assert np.mean(pulls ** 2) < 2.0
 
 
@@ -180,3 +184,51 @@ def test_torch_autograd_function():
     theta.grad = None
     (2.0 * mod(theta)).backward()
     assert np.allclose(theta.grad.numpy(), 2.0 * mod.state.backward()[2])
+
+
+def test_forward_releases_superseded_state():
+    """A new forward closes the previous forward's integrator once no pending
+    backward needs it; an integrator whose backward is still pending stays
+    usable (loss = F(t1) + F(t2) differentiates both)."""
+    import torch
+    from cuvegas_amd import integrands as I
+    from cuvegas_amd.autograd import DifferentiableIntegral
+    th = _theta()
+    _say("torch-autograd-release")
+    itg = I.gaussian_mixture_param(6, th)
+    mod = DifferentiableIntegral(itg.source, 6, th.size, 100_000, max_it=4, skip=2, n_intervals=64, seed=4)
+    t1 = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+    t2 = torch.tensor(th * 1.01, dtype=torch.float64, requires_grad=True)
+    loss = mod(t1) + mod(t2)
+    first = next(iter(mod._pending - {mod.state}))
+    assert first._h.value  # pending: not closed
+    loss.backward()
+    assert t1.grad is not None and t2.grad is not None and torch.all(torch.isfinite(t1.grad))
+    kept = mod.state
+    (mod(t1)).backward()
+    assert not kept._h.value  # superseded, no backward pending: released at the new forward
+    assert mod.state._h.value
+
+
+def test_parametrised_state_save_restore(tmp_path):
+    """A parametrised integrator's archive keeps theta: the restored integrator
+    continues with the saved parameters (same next iteration as the original),
+    and an integrator with another parameter count refuses the archive."""
+    from cuvegas_amd import Integrator
+    from cuvegas_amd import integrands as I
+    th = _theta()
+    _say("grad-save-restore")
+    itg = I.gaussian_mixture_param(6, th)
+    a = Integrator(itg.source, 6, 100_000, n_intervals=64, max_it=5, skip=2, seed=3, theta=th)
+    a.run(3)
+    path = str(tmp_path / "mix.npz")
+    a.save(path)
+    a.run(1)
+    b = Integrator.from_state(path, itg.source)
+    assert np.array_equal(b.theta, th)
+    b.run(1)
+    assert np.array_equal(a.history()[0], b.history()[0]) and np.array_equal(a.get_nh(), b.get_nh())
+    with pytest.raises(ValueError):
+        Integrator(I.gaussian_peak(6).source, 6, 100_000, n_intervals=64, max_it=5, skip=2).restore(path)
+    a.close()
+    b.close()

@@ -146,3 +146,110 @@ def test_batch_rejects_bad_items():
     b.run(2)
     assert np.isfinite(b.results()["result"][0])
     b.close()
+
+
+def _fixture_items():
+    import json
+    import os
+    from cuvegas_amd.batch import workload_from_items
+    with open(os.path.join(os.path.dirname(__file__), "golden", "workloads.json")) as f:
+        return workload_from_items(json.load(f)["batch_512"])
+
+
+def _resolvable(items, theta, fams):
+    """A Gaussian peak is resolvable when uniform sampling expects at least 20
+    points inside it: N * (sigma sqrt(2 pi))^d >= 20; the other families always are."""
+    return np.array([fam != "gaussian" or N * (theta[off + 1] * math.sqrt(2 * math.pi)) ** d >= 20
+                     for (d, N, off), fam in zip(items, fams)])
+
+
+def _oracle_params(orc, items, theta, fams, i, n_strat, prec, K):
+    """Oracle parameters of batch item i (Philox stream i, seed 0)."""
+    d, N, off = items[i]
+    fam = fams[i]
+    if fam == "gaussian":
+        fid, par = orc.F_GAUSSC, theta[off + 2:off + 4 + d]
+    elif fam == "oscillatory":
+        fid, par = orc.F_OSC, theta[off + 1:off + 2 + d]
+    else:
+        fid, par = orc.F_CORNER, theta[off + 1:off + 1 + d]
+    return orc.make_params(d, 256, n_strat, N, 1, fid, list(par), skip=0, K=K, seed=0, stream=i,
+                           precision=0 if prec == "fp64" else 1, threads=8)
+
+
+def _oracle_item(orc, items, theta, fams, i, n_strat, prec, K):
+    """Oracle first iteration of batch item i."""
+    p, keep = _oracle_params(orc, items, theta, fams, i, n_strat, prec, K)
+    return orc.run(p, keep, n_strat ** items[i][0])
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_batch512_full_configuration(orc, prec):
+    """The 512-integral serving workload at its configuration, read from the
+    committed fixture (d 2..12, Gaussian / oscillatory / corner peak, budgets
+    1e4..1e7 per iteration, 256 bins, 10 iterations in one hipGraph replay, 5
+    averaged).  After the first iteration, one item of each family (the
+    smallest budget of at least 1e5 with d >= 4, resolvable) is checked against the
+    oracle's first iteration on the same Philox stream: bin-exact allocation
+    and per-cube sums for the Gaussian (FP64: the project's exp on both sides)
+    and the corner peak; the oscillatory per-cube sums within the bound that
+    cos' ulp differences allow.  At the end: invariants of every item and the
+    pulls of the resolvable items against their closed forms."""
+    from cuvegas_amd.batch import FAMILY_SOURCE, Batch, BatchProgram
+    items, theta, exact, fams = _fixture_items()
+    _say("batch512-full", prec)
+    assert len(items) == 512
+    K = 16
+    prog = BatchProgram(FAMILY_SOURCE, 12, 256, prec, run_len=K)
+    b = Batch(prog, items, theta, max_it=10, skip=5, seed=0)
+    b.run(1, use_graph=True)
+    ok = _resolvable(items, theta, fams)
+    for fam in ("gaussian", "oscillatory", "corner"):
+        cand = [i for i, ((d, N, _), f) in enumerate(zip(items, fams))
+                if f == fam and d >= 4 and N >= 100_000 and ok[i]]
+        i = min(cand, key=lambda k: items[k][1])
+        _say("batch512-item", prec, fam, i, items[i])
+        d, N, _ = items[i]
+        _, ns, C = b.item_info(i)
+        o = _oracle_item(orc, items, theta, fams, i, ns, prec, K)
+        JF, JF2 = b.get_cube_sums(i)
+        I0, S20, ne0 = b.history(i)
+        assert ne0[0] == o["hist_n"][0]
+        exact_sums = fam == "corner" or (fam == "gaussian" and prec == "fp64")
+        if exact_sums:
+            assert np.array_equal(JF, o["JF_first"]) and np.array_equal(JF2, o["JF2_first"]), (fam, i)
+            assert np.array_equal(b.get_nh(i), o["nh_first"]), (fam, i)
+            assert I0[0] == o["hist_I"][0] and S20[0] == o["hist_S2"][0]
+        else:
+            # per sample, J f differs by a few ulps of the FP32 value (relative
+            # 4e-7) or, for the FP64 cos, by 2 ulps of 1 (|f| <= 1, J = 1 under
+            # the initial uniform map); the FP64 sums of n values differ by at
+            # most 2 (n - 1) u sum|x| from their rounding plus those differences
+            nh0 = np.full(C, max(2, int(N / C)), dtype=np.int32)
+            p, keep = _oracle_params(orc, items, theta, fams, i, ns, prec, K)
+            x0 = np.tile(np.linspace(0.0, 1.0, 257), (d, 1))
+            _, _, _, ojf = orc.sample_dump(p, keep, x0, nh0, 0, int(nh0.sum()))
+            start = np.concatenate([[0], np.cumsum(nh0)[:-1]])
+            A = np.add.reduceat(np.abs(ojf), start)
+            dv = 4e-7 * A if prec == "fp32" else 4.5e-16 * nh0
+            bound1 = 2.3e-16 * nh0 * A + dv
+            assert np.all(np.abs(JF - o["JF_first"]) <= bound1), (fam, i)
+            assert abs(I0[0] - o["hist_I"][0]) <= 2 * math.pow(1.0 / ns, d) * np.sum(bound1 / nh0) + \
+                1e-15 * abs(o["hist_I"][0])
+    b.run(9, use_graph=True)
+    r = b.results()
+    b_nh = [b.get_nh(i) for i in range(512)]
+    infos = [b.item_info(i) for i in range(512)]
+    for i, ((d, N, _), (_, ns, C)) in enumerate(zip(items, infos)):
+        assert b_nh[i].min() >= 2 and b_nh[i].sum() <= N + 2 * C, i
+        I_k, S2, ne = b.history(i)
+        assert len(ne) == 10 and r["nevals"][i] == ne.sum(), i
+    for i in (0, 255, 511):
+        x = b.get_map(i)
+        assert np.all(np.diff(x, axis=1) > 0) and np.all(x[:, 0] == 0) and np.all(x[:, -1] == 1)
+    b.close()
+    assert np.all(np.isfinite(r["result"])) and np.all(r["error"] > 0)
+    pull = (r["result"] - exact) / r["error"]
+    assert ok.sum() >= 400
+    assert np.sqrt(np.mean(pull[ok] ** 2)) < 1.6, np.sqrt(np.mean(pull[ok] ** 2))
+    assert np.mean(np.abs(pull[ok]) < 3) > 0.95

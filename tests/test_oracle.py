@@ -292,3 +292,75 @@ def test_run_polynomial_fp32_and_domain(orc):
     r = orc.run(p, keep, ns ** d)
     exact = 4.0 * (2.0 + 1.0 - 0.5)
     assert abs(r["result"] - exact) < 4 * r["error"] + 1e-12
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_run_closed_form_oscillatory(orc, prec, seed):
+    """5-D cos(2 pi u + sum a_i x_i), a_i ~ U(0,5): the oracle integrator (both
+    forms) within 4 sigma of cos(2 pi u + sum a_i / 2) prod sin(a_i/2)/(a_i/2),
+    chi2/dof < 3.  A dropped phase term or a wrong factor 1/2 inside the
+    product moves the result by far more than 4 sigma."""
+    from cuvegas_amd import integrands as I
+    d, n, ns, N, iters = 5, 128, 4, 40000, 10
+    rng = np.random.default_rng(20 + seed)
+    itg = I.oscillatory(d, float(rng.uniform(0, 1)), rng.uniform(0, 5, d).tolist())
+    p, keep = orc.make_params(d, n, ns, N, iters, orc.F_OSC, list(itg.params), seed=seed, precision=prec)
+    r = orc.run(p, keep, ns ** d, snapshots=False)
+    assert abs(r["result"] - itg.exact) < 4 * r["error"], (r["result"], itg.exact, r["error"])
+    assert r["chi2_dof"] < 3.0
+    assert r["error"] < 2e-3  # |f| <= 1: at most the plain MC error of 5 x 4e4 samples, ~2.2e-3
+    # the value itself: the phase term 2 pi u alone moves it by more than 10 sigma
+    wrong = math.cos(0.5 * sum(itg.params[1:])) * math.prod(
+        math.sin(a / 2) / (a / 2) for a in itg.params[1:])
+    if abs(wrong - itg.exact) > 20 * r["error"]:
+        assert abs(r["result"] - wrong) > 10 * r["error"]
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_run_closed_form_corner_peak(orc, prec, seed):
+    """4-D corner peak (1 + sum a_i x_i)^-(d+1), a_i ~ U(0.5,3): the oracle
+    integrator within 4 sigma of the inclusion-exclusion closed form, chi2/dof
+    < 3, relative error below 1e-3.  The closed form itself is pinned by
+    quadrature (test_closed_forms_of_library)."""
+    from cuvegas_amd import integrands as I
+    d, n, ns, N, iters = 4, 128, 6, 40000, 10
+    rng = np.random.default_rng(40 + seed)
+    itg = I.corner_peak(d, rng.uniform(0.5, 3.0, d).tolist())
+    p, keep = orc.make_params(d, n, ns, N, iters, orc.F_CORNER, list(itg.params), seed=seed, precision=prec)
+    r = orc.run(p, keep, ns ** d, snapshots=False)
+    assert abs(r["result"] - itg.exact) < 4 * r["error"], (r["result"], itg.exact, r["error"])
+    assert r["chi2_dof"] < 3.0
+    assert r["error"] / itg.exact < 1e-3
+
+
+def test_corner_peak_one_dimension_closed_form(orc):
+    """d = 1: (1 + a x)^-2 integrates to 1 / (1 + a) over [0, 1]; the oracle's
+    FP64 run at 2e5 samples per iteration gets it to 4 sigma and 2e-4."""
+    a = 2.5
+    p, keep = orc.make_params(1, 64, 50, 200000, 8, orc.F_CORNER, [a], seed=4)
+    r = orc.run(p, keep, 50, snapshots=False)
+    assert abs(r["result"] - 1.0 / (1.0 + a)) < 4 * r["error"]
+    assert abs(r["result"] * (1.0 + a) - 1.0) < 2e-4
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_run_closed_form_gaussian_per_dimension_centres(orc, prec):
+    """The batch family's Gaussian peak (a centre per dimension): 3-D, sigma
+    0.12, centres near the faces, within 4 sigma of prod_i of the normal mass
+    over [0, 1] and within 2e-3 relative.  One centre shared by every
+    dimension changes the exact value by far more than that."""
+    d, n, ns, N, iters = 3, 128, 8, 40000, 10
+    sigma, c = 0.12, [0.15, 0.6, 0.8]
+    norm = (1.0 / (sigma * math.sqrt(2 * math.pi))) ** d
+    m = -1.0 / (2.0 * sigma * sigma)
+    mass = [0.5 * (math.erf((1 - ci) / (sigma * math.sqrt(2))) + math.erf(ci / (sigma * math.sqrt(2))))
+            for ci in c]
+    exact = math.prod(mass)
+    p, keep = orc.make_params(d, n, ns, N, iters, orc.F_GAUSSC, [norm, m, *c], seed=1, precision=prec)
+    r = orc.run(p, keep, ns ** d, snapshots=False)
+    assert abs(r["result"] - exact) < 4 * r["error"], (r["result"], exact, r["error"])
+    assert abs(r["result"] / exact - 1) < 2e-3 and r["chi2_dof"] < 3.0
+    shared = (0.5 * (math.erf(0.5 / (sigma * math.sqrt(2))) * 2)) ** d  # every centre at 0.5
+    assert abs(shared - exact) > 20 * r["error"]
