@@ -74,8 +74,10 @@ def run_form(precision, itg, n_eval_it, warmup, steps, n_strat, nbins, seed, use
         "fill_ms_avg": float(np.mean(fill_ms[warmup:])),
         "evals_per_fill_avg": float(np.mean(ne[warmup:])),
         "result": r["result"], "error": r["error"], "chi2_dof": r["chi2_dof"],
+        "error_unbiased": r["error_unbiased"], "chi2_dof_unbiased": r["chi2_dof_unbiased"],
         "rel_err_vs_exact": abs(r["result"] - itg.exact) / abs(itg.exact),
         "pull": (r["result"] - itg.exact) / r["error"],
+        "pull_unbiased": (r["result"] - itg.exact) / r["error_unbiased"],
         "time_to_1e-4_ms": t_to, "iteration_reached_1e-4": it_to,
         "tot_nevals": int(r["tot_nevals"]), "compile_ms": r["compile_ms"],
         "setup_s": setup_s,
@@ -178,6 +180,10 @@ def batch_bench(args, sequential=True):
                     "completion_p50_ms": batch_wall * 1e3, "completion_p99_ms": batch_wall * 1e3,
                     "completion": "lockstep: all 512 complete with the last iteration's launch"},
         "accuracy": {"pull_rms_resolvable": float(np.sqrt(np.mean(pull[ok] ** 2))),
+                     "pull_rms_resolvable_unbiased": float(np.sqrt(np.mean(
+                         ((r["result"] - exact) / r["error_unbiased"])[ok] ** 2))),
+                     "chi2_dof_median": float(np.median(r["chi2_dof"])),
+                     "chi2_dof_unbiased_median": float(np.median(r["chi2_dof_unbiased"])),
                      "n_resolvable": int(ok.sum()),
                      "frac_within_3sigma_resolvable": float(np.mean(np.abs(pull[ok]) < 3)),
                      "frac_within_3sigma": float(np.mean(np.abs(pull) < 3)),

@@ -56,6 +56,8 @@ class cvg_result(C.Structure):
         ("n_cubes", C.c_int),
         ("n_strat", C.c_int),
         ("n_intervals", C.c_int),
+        ("error_unbiased", C.c_double),
+        ("chi2_dof_unbiased", C.c_double),
     ]
 
 
@@ -111,6 +113,7 @@ SIGNATURES = {
     "cvg_state_set_timing": (C.c_int, [_P, C.c_int]),
     "cvg_state_get_timing": (C.c_int, [_P, _PD, _PD, C.c_int]),
     "cvg_state_history": (C.c_int, [_P, _PD, _PD, _PLL, C.c_int]),
+    "cvg_state_history_unbiased": (C.c_int, [_P, _PD, C.c_int]),
     "cvg_state_n_cubes": (C.c_longlong, [_P]),
     "cvg_state_get_map": (C.c_int, [_P, _PD]),
     "cvg_state_set_map": (C.c_int, [_P, _PD]),
@@ -121,7 +124,7 @@ SIGNATURES = {
     "cvg_state_get_bins": (C.c_int, [_P, _PD, _PU]),
     "cvg_state_set_bins": (C.c_int, [_P, _PD, _PU]),
     "cvg_state_set_iteration": (C.c_int, [_P, C.c_int]),
-    "cvg_state_set_history": (C.c_int, [_P, _PD, _PD, _PLL, C.c_int]),
+    "cvg_state_set_history": (C.c_int, [_P, _PD, _PD, _PLL, _PD, C.c_int]),
     "cvg_state_fill": (C.c_int, [_P]),
     "cvg_state_refine": (C.c_int, [_P]),
     "cvg_state_reallocate": (C.c_int, [_P, _PD, C.c_double]),
@@ -136,7 +139,7 @@ SIGNATURES = {
                                    C.c_int, C.POINTER(_P)]),
     "cvg_batch_destroy": (None, [_P]),
     "cvg_batch_run": (C.c_int, [_P, C.c_int, C.c_int]),
-    "cvg_batch_results": (C.c_int, [_P, _PD, _PD, _PD, _PLL, _PD]),
+    "cvg_batch_results": (C.c_int, [_P, _PD, _PD, _PD, _PLL, _PD, _PD, _PD]),
     "cvg_batch_history": (C.c_int, [_P, C.c_int, _PD, _PD, _PLL, C.c_int]),
     "cvg_batch_item_info": (C.c_int, [_P, C.c_int, _PI, _PI, _PLL]),
     "cvg_batch_get_map": (C.c_int, [_P, C.c_int, _PD]),

@@ -197,6 +197,13 @@ class Integrator:
         k = check(lib.cvg_state_history(self._h, _dptr(I), _dptr(S2), _llptr(ne), n))
         return I[:k], S2[:k], ne[:k]
 
+    def history_unbiased(self) -> np.ndarray:
+        """Per-iteration variance of I_k from the unbiased per-cube variances
+        sigma_h^2 / (n_h - 1) (the reference's sigma_k^2 divides by n_h)."""
+        S2u = np.zeros(self.max_it)
+        k = check(lib.cvg_state_history_unbiased(self._h, _dptr(S2u), self.max_it))
+        return S2u[:k]
+
     def set_timing(self, enable: bool = True) -> None:
         check(lib.cvg_state_set_timing(self._h, 1 if enable else 0))
 
@@ -253,14 +260,20 @@ class Integrator:
     def set_iteration(self, it: int) -> None:
         check(lib.cvg_state_set_iteration(self._h, int(it)))
 
-    def set_history(self, I, sigma2, nevals) -> None:
+    def set_history(self, I, sigma2, nevals, sigma2_unbiased=None) -> None:
         """Restore the per-iteration record; the next iteration is len(I)."""
         I = np.ascontiguousarray(I, dtype=np.float64)
         S2 = np.ascontiguousarray(sigma2, dtype=np.float64)
         ne = np.ascontiguousarray(nevals, dtype=np.int64)
         if not (I.shape == S2.shape == ne.shape and I.ndim == 1):
             raise ValueError("I, sigma2 and nevals must be 1-D arrays of one length")
-        check(lib.cvg_state_set_history(self._h, _dptr(I), _dptr(S2), _llptr(ne), int(I.size)))
+        S2u = None
+        if sigma2_unbiased is not None:
+            S2u = np.ascontiguousarray(sigma2_unbiased, dtype=np.float64)
+            if S2u.shape != I.shape:
+                raise ValueError("sigma2_unbiased must have the length of I")
+        check(lib.cvg_state_set_history(self._h, _dptr(I), _dptr(S2), _llptr(ne),
+                                        None if S2u is None else _dptr(S2u), int(I.size)))
 
     # -- on-disk state -------------------------------------------------------
     # An .npz archive of plain arrays (np.load(path, allow_pickle=False) reads
@@ -282,7 +295,8 @@ class Integrator:
                  n_strat=p.n_strat, n_eval_it=p.n_eval_it, max_it=p.max_it, skip=p.skip,
                  alpha=p.alpha, beta=p.beta, seed=p.seed, stream_id=p.stream_id,
                  precision=p.precision, run_len=p.run_len, domain=dom, x_edges=self.get_map(),
-                 nh=self.get_nh(), I=I, sigma2=S2, nevals=ne, n_theta=p.n_theta,
+                 nh=self.get_nh(), I=I, sigma2=S2, nevals=ne, sigma2_unbiased=self.history_unbiased(),
+                 n_theta=p.n_theta,
                  theta=self.theta if self.theta is not None else np.zeros(0))
 
     def restore(self, path) -> "Integrator":
@@ -302,7 +316,8 @@ class Integrator:
                                  f"{self.params.n_theta}")
             self.set_map(z["x_edges"])
             self.set_nh(z["nh"])
-            self.set_history(z["I"], z["sigma2"], z["nevals"])
+            self.set_history(z["I"], z["sigma2"], z["nevals"],
+                             z["sigma2_unbiased"] if "sigma2_unbiased" in z else None)
             if n_theta > 0:
                 self.set_theta(z["theta"])
         return self
@@ -470,6 +485,7 @@ def integrate(device_function, n_dim, domain=None, n_evals=1000000, max_it=20, s
         "avg_it_time": r["loop_ms"] / max_it, "seed": seed, "multi_gpu": bool(multi_gpu),
         "multi_cpu": bool(multi_cpu), "gpu_num": 1, "omp_threads_num": 0, "gpu_thr": gpu_thr,
         "chi2_dof": r["chi2_dof"], "iterations": list(zip(I.tolist(), np.sqrt(S2).tolist())),
+        "error_unbiased": r["error_unbiased"], "chi2_dof_unbiased": r["chi2_dof_unbiased"],
         "evals_per_s": r["tot_nevals"] / (r["loop_ms"] * 1e-3) if r["loop_ms"] > 0 else float("nan"),
         "precision": precision, "compile_ms": r["compile_ms"], "setup_ms": r["setup_ms"],
         "exact": exact,

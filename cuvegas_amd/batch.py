@@ -73,10 +73,13 @@ class Batch:
     def results(self) -> dict:
         n = self.n_items
         res, err, chi2 = np.zeros(n), np.zeros(n), np.zeros(n)
+        erru, chi2u = np.zeros(n), np.zeros(n)
         ne = np.zeros(n, dtype=np.int64)
         ms = C.c_double()
-        check(lib.cvg_batch_results(self._h, _dptr(res), _dptr(err), _dptr(chi2), _llptr(ne), C.byref(ms)))
-        return {"result": res, "error": err, "chi2_dof": chi2, "nevals": ne, "loop_ms": ms.value}
+        check(lib.cvg_batch_results(self._h, _dptr(res), _dptr(err), _dptr(chi2), _llptr(ne), C.byref(ms),
+                                    _dptr(erru), _dptr(chi2u)))
+        return {"result": res, "error": err, "chi2_dof": chi2, "nevals": ne, "loop_ms": ms.value,
+                "error_unbiased": erru, "chi2_dof_unbiased": chi2u}
 
     def history(self, item: int):
         I, S2 = np.zeros(self.max_it), np.zeros(self.max_it)

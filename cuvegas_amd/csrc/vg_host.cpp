@@ -616,7 +616,7 @@ int cvg_state_create(cvg_program* prog, const cvg_params* params, cvg_state** ou
     if ((rc = dalloc(&st->win_last, st->win_cap))) return rc;
     // block partials, then one per 1024 blocks (the group level of the trees / scans)
     const long long ng = (st->nb + 1023) / 1024;
-    if ((rc = dalloc(&st->part, 3 * (st->nb + ng)))) return rc;
+    if ((rc = dalloc(&st->part, (long long)VG_NRES * (st->nb + ng)))) return rc;
     if ((rc = dalloc(&st->bsum, st->nb + ng))) return rc;
     if ((rc = dalloc(&st->boff, st->nb + ng))) return rc;
     if ((rc = dalloc(&st->sync, 2))) return rc;
@@ -624,7 +624,7 @@ int cvg_state_create(cvg_program* prog, const cvg_params* params, cvg_state** ou
     if ((rc = dalloc(&st->it_p, 1))) return rc;
     if ((rc = dalloc(&st->dsum, 1))) return rc;
     if ((rc = dalloc(&st->err, 1))) return rc;
-    if ((rc = dalloc(&st->hist, 4LL * params->max_it))) return rc;
+    if ((rc = dalloc(&st->hist, (long long)VG_HIST * params->max_it))) return rc;
     if ((rc = dalloc(&st->domain, 2LL * d))) return rc;
     st->fold_cap = (int)(st->s_cap / (64LL * 64 * K) + 2);
     if ((rc = dalloc(&st->fold_list, st->fold_cap))) return rc;
@@ -770,49 +770,90 @@ int cvg_state_get_timing(cvg_state* st, double* fill_ms, double* iter_ms, int ca
     return n;
 }
 
+// The iteration record on the host: VG_HIST doubles per iteration.
+static int read_history(cvg_state* st, std::vector<double>& h) {
+    const int n = st->iterations_done;
+    h.assign((size_t)VG_HIST * (n > 0 ? n : 1), 0.0);
+    if (n > 0) {
+        VG_HIP(hipMemcpyAsync(h.data(), st->hist, sizeof(double) * VG_HIST * n, hipMemcpyDeviceToHost,
+                              st->stream));
+        VG_HIP(hipStreamSynchronize(st->stream));
+    }
+    return CVG_OK;
+}
+
 int cvg_state_history(cvg_state* st, double* I, double* sigma2, long long* nevals, int cap) {
     if (!st) return fail(CVG_ERR_INVALID, "state is NULL");
     const int n = st->iterations_done;
-    std::vector<double> h(4 * (size_t)(n > 0 ? n : 1));
-    if (n > 0) {
-        VG_HIP(hipMemcpyAsync(h.data(), st->hist, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, st->stream));
-        VG_HIP(hipStreamSynchronize(st->stream));
-    }
+    std::vector<double> h;
+    int rc = read_history(st, h);
+    if (rc) return rc;
     for (int i = 0; i < n && i < cap; ++i) {
-        if (I) I[i] = h[4 * i + 0];
-        if (sigma2) sigma2[i] = h[4 * i + 1];
-        if (nevals) nevals[i] = (long long)h[4 * i + 3];
+        if (I) I[i] = h[VG_HIST * i + 0];
+        if (sigma2) sigma2[i] = h[VG_HIST * i + 1];
+        if (nevals) nevals[i] = (long long)h[VG_HIST * i + 3];
     }
     return n;
+}
+
+int cvg_state_history_unbiased(cvg_state* st, double* sigma2_unbiased, int cap) {
+    if (!st || !sigma2_unbiased) return fail(CVG_ERR_INVALID, "NULL argument");
+    const int n = st->iterations_done;
+    std::vector<double> h;
+    int rc = read_history(st, h);
+    if (rc) return rc;
+    for (int i = 0; i < n && i < cap; ++i) sigma2_unbiased[i] = h[VG_HIST * i + 4];
+    return n;
+}
+
+// Inverse-variance weighted mean of iterations [skip, n) of a record of
+// VG_HIST doubles per iteration (reference combine, cuda_wrapper.cu:758-766
+// and :869), its error and chi2/dof; with the unbiased per-iteration
+// variances, the error of that same weighted mean and chi2/dof (no reference
+// counterpart).
+struct vg_combined {
+    double result = 0.0, error = 0.0, chi2_dof = 0.0, error_unb = 0.0, chi2_dof_unb = 0.0;
+    long long tot = 0;
+};
+static vg_combined combine_record(const double* h, int n, int skip) {
+    vg_combined c;
+    double res = 0.0, sig = 0.0;
+    int Kc = 0;
+    for (int i = skip; i < n; ++i) {
+        res += h[VG_HIST * i] / h[VG_HIST * i + 1];
+        sig += 1.0 / h[VG_HIST * i + 1];
+        ++Kc;
+    }
+    if (Kc > 0) res /= sig;
+    double chi2 = 0.0, chi2u = 0.0, varu = 0.0;
+    for (int i = skip; i < n; ++i) {
+        const double dd = h[VG_HIST * i] - res, s2 = h[VG_HIST * i + 1], s2u = h[VG_HIST * i + 4];
+        chi2 += dd * dd / s2;
+        chi2u += dd * dd / s2u;
+        varu += s2u / (s2 * s2);  // w_k^2 sigma_unb,k^2 with w_k = 1 / sigma_k^2
+    }
+    for (int i = 0; i < n; ++i) c.tot += (long long)h[VG_HIST * i + 3];
+    c.result = res;
+    c.error = 1.0 / std::sqrt(sig);
+    c.chi2_dof = Kc > 1 ? chi2 / (Kc - 1) : 0.0;
+    c.error_unb = Kc > 0 ? std::sqrt(varu) / sig : 0.0;
+    c.chi2_dof_unb = Kc > 1 ? chi2u / (Kc - 1) : 0.0;
+    return c;
 }
 
 int cvg_state_result(cvg_state* st, cvg_result* out) {
     if (!st || !out) return fail(CVG_ERR_INVALID, "state or out is NULL");
     const int n = st->iterations_done;
-    std::vector<double> I(n > 0 ? n : 1), S2(n > 0 ? n : 1);
-    std::vector<long long> ne(n > 0 ? n : 1);
-    int rc = cvg_state_history(st, I.data(), S2.data(), ne.data(), n);
-    if (rc < 0) return rc;
-    // Reference combine (cuda_wrapper.cu:758-766, 869).
-    double res = 0.0, sig = 0.0;
-    int Kc = 0;
-    for (int i = st->p.skip; i < n; ++i) {
-        res += I[i] / S2[i];
-        sig += 1.0 / S2[i];
-        ++Kc;
-    }
-    if (Kc > 0) res /= sig;
-    double chi2 = 0.0;
-    for (int i = st->p.skip; i < n; ++i) {
-        const double dd = I[i] - res;
-        chi2 += dd * dd / S2[i];
-    }
-    long long tot = 0;
-    for (int i = 0; i < n; ++i) tot += ne[i];
-    out->result = res;
-    out->error = 1.0 / std::sqrt(sig);
-    out->chi2_dof = Kc > 1 ? chi2 / (Kc - 1) : 0.0;
-    out->tot_nevals = tot;
+    std::vector<double> h;
+    int rc = read_history(st, h);
+    if (rc) return rc;
+    const vg_combined c = combine_record(h.data(), n, st->p.skip);
+    out->result = c.result;
+    out->error = c.error;
+    out->chi2_dof = c.chi2_dof;
+    out->error_unbiased = c.error_unb;
+    out->chi2_dof_unbiased = c.chi2_dof_unb;
+    out->tot_nevals = c.tot;
     out->loop_ms = st->loop_ms;
     out->compile_ms = st->prog->compile_ms;
     out->setup_ms = st->setup_ms;
@@ -907,17 +948,18 @@ int cvg_state_set_bins(cvg_state* st, const double* weights, const unsigned int*
 }
 
 int cvg_state_set_history(cvg_state* st, const double* I, const double* sigma2, const long long* nevals,
-                          int count) {
+                          const double* sigma2_unbiased, int count) {
     if (!st || count < 0) return fail(CVG_ERR_INVALID, "bad argument");
     if (count > 0 && (!I || !sigma2 || !nevals)) return fail(CVG_ERR_INVALID, "NULL argument");
     if (count > st->p.max_it) return fail(CVG_ERR_INVALID, "count exceeds the state's max_it");
     if (count > 0) {
-        std::vector<double> h(4 * (size_t)count);
+        std::vector<double> h((size_t)VG_HIST * count);
         for (int i = 0; i < count; ++i) {
-            h[4 * i + 0] = I[i];
-            h[4 * i + 1] = sigma2[i];
-            h[4 * i + 2] = 0.0;  // D of a restored iteration is not kept
-            h[4 * i + 3] = (double)nevals[i];
+            h[VG_HIST * i + 0] = I[i];
+            h[VG_HIST * i + 1] = sigma2[i];
+            h[VG_HIST * i + 2] = 0.0;  // D of a restored iteration is not kept
+            h[VG_HIST * i + 3] = (double)nevals[i];
+            h[VG_HIST * i + 4] = sigma2_unbiased ? sigma2_unbiased[i] : sigma2[i];
         }
         int rc = copy_h2d(st, st->hist, h.data(), sizeof(double) * h.size());
         if (rc) return rc;
@@ -1067,7 +1109,7 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         }
         th = st->bw_theta;
     }
-    std::vector<double> hist(4 * (size_t)st->iterations_done);
+    std::vector<double> hist((size_t)VG_HIST * st->iterations_done);
     VG_HIP(hipMemcpyAsync(hist.data(), st->hist, sizeof(double) * hist.size(), hipMemcpyDeviceToHost, st->stream));
     const long long nx = (long long)d * (n + 1);
     std::vector<double> tot((size_t)NV);
@@ -1106,7 +1148,7 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         float kms = 0.0f;
         VG_HIP(hipEventElapsedTime(&kms, st->bw_ev0, st->bw_ev1));
         st->bw_replay_ms += kms;
-        const double w = 1.0 / hist[4 * (size_t)k + 1];  // forward weight, held fixed
+        const double w = 1.0 / hist[(size_t)VG_HIST * k + 1];  // forward weight, held fixed
         W += w;
         val += w * tot[0];
         vvar += w * w * tot[1];
@@ -1580,14 +1622,14 @@ int cvg_batch_create(cvg_program* prog, const cvg_params* common, const cvg_batc
     if ((rc = dalloc(&b->run_cube, b->tot_win * 64))) return rc;
     if ((rc = dalloc(&b->win_first, b->tot_win))) return rc;
     if ((rc = dalloc(&b->win_last, b->tot_win))) return rc;
-    if ((rc = dalloc(&b->part, 3 * b->tot_blk))) return rc;
+    if ((rc = dalloc(&b->part, (long long)VG_NRES * b->tot_blk))) return rc;
     if ((rc = dalloc(&b->bsum, b->tot_blk))) return rc;
     if ((rc = dalloc(&b->boff, b->tot_blk))) return rc;
     if ((rc = dalloc(&b->dsum, n_items))) return rc;
     if ((rc = dalloc(&b->s_item, n_items))) return rc;
     if ((rc = dalloc(&b->it_p, 1))) return rc;
     if ((rc = dalloc(&b->err, 1))) return rc;
-    if ((rc = dalloc(&b->hist, 4LL * n_items * common->max_it))) return rc;
+    if ((rc = dalloc(&b->hist, (long long)VG_HIST * n_items * common->max_it))) return rc;
     if ((rc = dalloc(&b->x_all, b->tot_xe))) return rc;
     if ((rc = dalloc(&b->map64, b->tot_map))) return rc;
     if ((rc = dalloc(&b->map32, b->tot_map))) return rc;
@@ -1669,44 +1711,33 @@ int cvg_batch_history(cvg_batch* b, int item, double* I, double* sigma2, long lo
     if (!b || item < 0 || item >= b->n_items) return fail(CVG_ERR_INVALID, "bad item");
     const int n = b->iterations_done;
     if (n == 0) return 0;
-    std::vector<double> h(4 * (size_t)n);
-    VG_HIP(hipMemcpy(h.data(), b->hist + (size_t)item * b->p.max_it * 4, sizeof(double) * h.size(),
+    VG_HIP(hipStreamSynchronize(b->stream));
+    std::vector<double> h((size_t)VG_HIST * n);
+    VG_HIP(hipMemcpy(h.data(), b->hist + (size_t)item * b->p.max_it * VG_HIST, sizeof(double) * h.size(),
                      hipMemcpyDeviceToHost));
     for (int i = 0; i < n && i < cap; ++i) {
-        if (I) I[i] = h[4 * i];
-        if (sigma2) sigma2[i] = h[4 * i + 1];
-        if (nevals) nevals[i] = (long long)h[4 * i + 3];
+        if (I) I[i] = h[VG_HIST * i];
+        if (sigma2) sigma2[i] = h[VG_HIST * i + 1];
+        if (nevals) nevals[i] = (long long)h[VG_HIST * i + 3];
     }
     return n;
 }
 
 int cvg_batch_results(cvg_batch* b, double* result, double* error, double* chi2_dof, long long* nevals,
-                      double* loop_ms) {
+                      double* loop_ms, double* error_unbiased, double* chi2_dof_unbiased) {
     if (!b) return fail(CVG_ERR_INVALID, "batch is NULL");
     VG_HIP(hipStreamSynchronize(b->stream));
     const int n = b->iterations_done, cap = b->p.max_it;
-    std::vector<double> h(4 * (size_t)b->n_items * cap);
+    std::vector<double> h((size_t)VG_HIST * b->n_items * cap);
     VG_HIP(hipMemcpy(h.data(), b->hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
     for (int i = 0; i < b->n_items; ++i) {
-        const double* row = h.data() + (size_t)i * cap * 4;
-        double res = 0.0, sig = 0.0, chi2 = 0.0;
-        int Kc = 0;
-        long long tot = 0;
-        for (int k = 0; k < n; ++k) tot += (long long)row[4 * k + 3];
-        for (int k = b->p.skip; k < n; ++k) {
-            res += row[4 * k] / row[4 * k + 1];
-            sig += 1.0 / row[4 * k + 1];
-            ++Kc;
-        }
-        if (Kc > 0) res /= sig;
-        for (int k = b->p.skip; k < n; ++k) {
-            const double dd = row[4 * k] - res;
-            chi2 += dd * dd / row[4 * k + 1];
-        }
-        if (result) result[i] = res;
-        if (error) error[i] = 1.0 / std::sqrt(sig);
-        if (chi2_dof) chi2_dof[i] = Kc > 1 ? chi2 / (Kc - 1) : 0.0;
-        if (nevals) nevals[i] = tot;
+        const vg_combined c = combine_record(h.data() + (size_t)i * cap * VG_HIST, n, b->p.skip);
+        if (result) result[i] = c.result;
+        if (error) error[i] = c.error;
+        if (chi2_dof) chi2_dof[i] = c.chi2_dof;
+        if (nevals) nevals[i] = c.tot;
+        if (error_unbiased) error_unbiased[i] = c.error_unb;
+        if (chi2_dof_unbiased) chi2_dof_unbiased[i] = c.chi2_dof_unb;
     }
     if (loop_ms) *loop_ms = b->loop_ms;
     return CVG_OK;

@@ -5,6 +5,15 @@
 #define VG_ERR_HIST 1      // more iterations than the history buffer holds
 #define VG_ERR_CAPACITY 2  // allocation exceeded the sample buffers
 
+// Per-cube result sums reduced every iteration (block partials hold VG_NRES
+// doubles per block): sum I_h, sum sigma_h^2 / n_h (the reference's variance),
+// D = sum d_h, and sum sigma_h^2 / (n_h - 1) (the unbiased per-cube variance
+// of the same estimate, reported beside the reference's).
+#define VG_NRES 4
+// Per-iteration record (doubles per iteration): I_k, sigma_k^2, D, samples,
+// unbiased sigma_k^2.
+#define VG_HIST 5
+
 // Per-cube results from complete per-cube sums: vg_launch_bigfold must have
 // folded the window partials of every cube that crosses a sampling window.
 hipError_t vg_launch_results(hipStream_t st, int C, double V, double beta, const int* nh,
@@ -16,7 +25,7 @@ hipError_t vg_launch_results_refine(hipStream_t st, int C, double V, double beta
                                     double* x_edges, double* weights, unsigned* counts, double* scratch,
                                     double2* map64, float2* map32);
 long long vg_results_refine_lds(int n);
-// part holds 3 * (nparts + ceil(nparts / 1024)) doubles; sync is one zeroed
+// part holds VG_NRES * (nparts + ceil(nparts / 1024)) doubles; sync is one zeroed
 // counter that the launch leaves zeroed.
 hipError_t vg_launch_finalize(hipStream_t st, long long nparts, double* part, unsigned* sync,
                               const long long* s_total, const int* it_p, int hist_cap, double* hist,

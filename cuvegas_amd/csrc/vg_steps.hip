@@ -99,14 +99,16 @@ __device__ inline bool vg_last_block(unsigned* cnt, int* flag) {
 }
 
 // ---------------------------------------------------------------------------
-// Per-cube results (reference `results`, cuda_wrapper.cu:26-49).  part[3b+q]:
-// block partial of q = 0: sum I_h, 1: sum sigma_h^2 / n_h, 2: sum d_h.
+// Per-cube results (reference `results`, cuda_wrapper.cu:26-49).  part[VG_NRES b + q]:
+// block partial of q = 0: sum I_h, 1: sum sigma_h^2 / n_h, 2: sum d_h,
+// 3: sum sigma_h^2 / (n_h - 1) (no reference counterpart).
 // ---------------------------------------------------------------------------
 // Results of cube h from complete per-cube sums (the window folds ran before).
 __device__ inline void vg_cube_result_sums(long long h, int C, double V, double beta,
                                            const int* __restrict__ nh, const double2* __restrict__ JS,
-                                           double* __restrict__ dh, double& sr, double& ss, double& dd) {
-    sr = 0.0; ss = 0.0; dd = 0.0;
+                                           double* __restrict__ dh, double& sr, double& ss, double& dd,
+                                           double& su) {
+    sr = 0.0; ss = 0.0; dd = 0.0; su = 0.0;
     if (h >= C) return;
     const int neval = nh[h];
     const double2 js = JS[h];
@@ -115,6 +117,7 @@ __device__ inline void vg_cube_result_sums(long long h, int C, double V, double 
     const double Sig2 = (j2 / neval * V * V) - (Ih * Ih);
     sr = Ih;
     ss = Sig2 / neval;
+    su = Sig2 / (neval - 1);  // n_h >= 2 always
     const double u = V * j1 / neval;
     double t = V * V * j2 / neval - u * u;
     if (t < 0) t = 0;
@@ -123,19 +126,20 @@ __device__ inline void vg_cube_result_sums(long long h, int C, double V, double 
 }
 
 // Results of the 1024 cubes of block b: d_h, and the block partials of
-// (sum I_h, sum sigma_h^2 / n_h, sum d_h) in part[3b + q].  256 threads.
+// (sum I_h, sum sigma_h^2 / n_h, sum d_h, sum sigma_h^2 / (n_h - 1)) in
+// part[VG_NRES b + q].  256 threads.
 __device__ inline void vg_results_block(long long b, int C, double V, double beta,
                                         const int* __restrict__ nh, const double2* __restrict__ JS,
                                         double* __restrict__ dh, double* __restrict__ part, double* red) {
     const long long base = b * VG_TREE + threadIdx.x;
-    double r[4][3];
+    double r[4][VG_NRES];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-        vg_cube_result_sums(base + 256 * j, C, V, beta, nh, JS, dh, r[j][0], r[j][1], r[j][2]);
+        vg_cube_result_sums(base + 256 * j, C, V, beta, nh, JS, dh, r[j][0], r[j][1], r[j][2], r[j][3]);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < VG_NRES; ++q) {
         const double tot = vg_tree1024_256(r[0][q], r[1][q], r[2][q], r[3][q], red);
-        if (threadIdx.x == 0) part[3 * b + q] = tot;
+        if (threadIdx.x == 0) part[VG_NRES * b + q] = tot;
     }
 }
 
@@ -239,8 +243,9 @@ hipError_t vg_launch_bigfold(hipStream_t st, const int* list, const int* count, 
 }
 
 // Remaining tree levels over the block partials, then the iteration record:
-// hist[4*it + {0,1,2,3}] = {I_k, sigma_k^2, D, samples}; *dsum = D.
-// Block g reduces chunk g (1024 block partials) into part2[3g + q]; the last
+// hist[VG_HIST it + {0..4}] = {I_k, sigma_k^2, D, samples, unbiased sigma_k^2};
+// *dsum = D.  Block g reduces chunk g (1024 block partials) into
+// part2[VG_NRES g + q]; the last
 // block to finish reduces part2 level by level, in place.  The sums are those
 // of the sequential definition (chunks of 1024, strided pairwise trees,
 // repeated over the partials).
@@ -252,17 +257,17 @@ vg_finalize_kernel(long long nparts, const double* __restrict__ part, double* __
     __shared__ double red[256];
     __shared__ int last;
     {
-        double v[4][3];
+        double v[4][VG_NRES];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const long long i = (long long)blockIdx.x * VG_TREE + threadIdx.x + 256 * j;
 #pragma unroll
-            for (int q = 0; q < 3; ++q) v[j][q] = i < nparts ? part[3 * i + q] : 0.0;
+            for (int q = 0; q < VG_NRES; ++q) v[j][q] = i < nparts ? part[VG_NRES * i + q] : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < VG_NRES; ++q) {
             const double tot = vg_tree1024_256(v[0][q], v[1][q], v[2][q], v[3][q], red);
-            if (threadIdx.x == 0) part2[3 * blockIdx.x + q] = tot;
+            if (threadIdx.x == 0) part2[VG_NRES * blockIdx.x + q] = tot;
         }
     }
     if (!vg_last_block(sync, &last)) return;
@@ -270,18 +275,18 @@ vg_finalize_kernel(long long nparts, const double* __restrict__ part, double* __
     while (cur > 1) {
         const long long nch = (cur + VG_TREE - 1) / VG_TREE;
         for (long long c = 0; c < nch; ++c) {
-            double v[4][3];
+            double v[4][VG_NRES];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const long long i = c * VG_TREE + threadIdx.x + 256 * j;
 #pragma unroll
-                for (int q = 0; q < 3; ++q) v[j][q] = i < cur ? part2[3 * i + q] : 0.0;
+                for (int q = 0; q < VG_NRES; ++q) v[j][q] = i < cur ? part2[VG_NRES * i + q] : 0.0;
             }
             __syncthreads();  // every read of this chunk precedes the writes below
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
+            for (int q = 0; q < VG_NRES; ++q) {
                 const double tot = vg_tree1024_256(v[0][q], v[1][q], v[2][q], v[3][q], red);
-                if (threadIdx.x == 0) part2[3 * c + q] = tot;
+                if (threadIdx.x == 0) part2[VG_NRES * c + q] = tot;
             }
             __syncthreads();
         }
@@ -291,10 +296,11 @@ vg_finalize_kernel(long long nparts, const double* __restrict__ part, double* __
         const int it = *it_p;
         *dsum = part2[2];
         if (it < hist_cap) {
-            hist[4 * it + 0] = part2[0];
-            hist[4 * it + 1] = part2[1];
-            hist[4 * it + 2] = part2[2];
-            hist[4 * it + 3] = (double)*s_total;
+            hist[VG_HIST * it + 0] = part2[0];
+            hist[VG_HIST * it + 1] = part2[1];
+            hist[VG_HIST * it + 2] = part2[2];
+            hist[VG_HIST * it + 3] = (double)*s_total;
+            hist[VG_HIST * it + 4] = part2[3];
         } else {
             atomicOr(err, VG_ERR_HIST);
         }
@@ -737,7 +743,7 @@ hipError_t vg_launch_finalize(hipStream_t st, long long nparts, double* part, un
                               double* dsum, int* err) {
     const long long ng = (nparts + VG_TREE - 1) / VG_TREE;
     hipLaunchKernelGGL(vg_finalize_kernel, dim3((unsigned)ng), dim3(256), 0, st, nparts, part,
-                       part + 3 * nparts, sync, s_total, it_p, hist_cap, hist, dsum, err);
+                       part + VG_NRES * nparts, sync, s_total, it_p, hist_cap, hist, dsum, err);
     return hipGetLastError();
 }
 
@@ -892,11 +898,11 @@ vg_bresults_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ b
     const long long bb = blockIdx.x - n_refine;
     const vg_bitem I = items[blk_item[bb]];
     const long long base = (bb - I.blk_off) * VG_TREE + threadIdx.x;
-    double r[4][3];
+    double r[4][VG_NRES];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const long long h = base + 256 * j;
-        double sr = 0.0, ss = 0.0, dd = 0.0;
+        double sr = 0.0, ss = 0.0, dd = 0.0, su = 0.0;
         if (h < I.C) {
             const int neval = nh[I.cube_off + h];
             const double j1 = JF[I.cube_off + h], j2 = JF2[I.cube_off + h];
@@ -904,23 +910,24 @@ vg_bresults_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ b
             const double Sig2 = (j2 / neval * I.V * I.V) - (Ih * Ih);
             sr = Ih;
             ss = Sig2 / neval;
+            su = Sig2 / (neval - 1);
             const double u = I.V * j1 / neval;
             double t = I.V * I.V * j2 / neval - u * u;
             if (t < 0) t = 0;
             dd = vg_pow(t, beta);
             dh[I.cube_off + h] = dd;
         }
-        r[j][0] = sr; r[j][1] = ss; r[j][2] = dd;
+        r[j][0] = sr; r[j][1] = ss; r[j][2] = dd; r[j][3] = su;
     }
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < VG_NRES; ++q) {
         const double tot = vg_tree1024_256(r[0][q], r[1][q], r[2][q], r[3][q], red);
-        if (threadIdx.x == 0) part[3 * bb + q] = tot;
+        if (threadIdx.x == 0) part[VG_NRES * bb + q] = tot;
     }
 }
 
 // One block per item: remaining tree levels over the item's block partials
-// (in place), then hist[(item * cap + it) * 4 + q] and dsum[item].
+// (in place), then hist[(item * cap + it) * VG_HIST + q] and dsum[item].
 __global__ void __launch_bounds__(256)
 vg_bfinalize_kernel(const vg_bitem* __restrict__ items, double* __restrict__ part_all,
                     const long long* __restrict__ s_item, const int* __restrict__ it_p, int hist_cap,
@@ -928,23 +935,23 @@ vg_bfinalize_kernel(const vg_bitem* __restrict__ items, double* __restrict__ par
     __shared__ double red[256];
     const int item = blockIdx.x;
     const vg_bitem I = items[item];
-    double* part = part_all + 3 * I.blk_off;
+    double* part = part_all + VG_NRES * I.blk_off;
     long long cur = I.nblk;
     while (cur > 1) {
         const long long nch = (cur + VG_TREE - 1) / VG_TREE;
         for (long long c = 0; c < nch; ++c) {
-            double v[4][3];
+            double v[4][VG_NRES];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const long long i = c * VG_TREE + threadIdx.x + 256 * j;
 #pragma unroll
-                for (int q = 0; q < 3; ++q) v[j][q] = i < cur ? part[3 * i + q] : 0.0;
+                for (int q = 0; q < VG_NRES; ++q) v[j][q] = i < cur ? part[VG_NRES * i + q] : 0.0;
             }
             __syncthreads();
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
+            for (int q = 0; q < VG_NRES; ++q) {
                 const double tot = vg_tree1024_256(v[0][q], v[1][q], v[2][q], v[3][q], red);
-                if (threadIdx.x == 0) part[3 * c + q] = tot;
+                if (threadIdx.x == 0) part[VG_NRES * c + q] = tot;
             }
             __syncthreads();
         }
@@ -954,11 +961,12 @@ vg_bfinalize_kernel(const vg_bitem* __restrict__ items, double* __restrict__ par
         const int it = *it_p;
         dsum[item] = part[2];
         if (it < hist_cap) {
-            double* hrow = hist + ((long long)item * hist_cap + it) * 4;
+            double* hrow = hist + ((long long)item * hist_cap + it) * VG_HIST;
             hrow[0] = part[0];
             hrow[1] = part[1];
             hrow[2] = part[2];
             hrow[3] = (double)s_item[item];
+            hrow[4] = part[3];
         } else {
             atomicOr(err, VG_ERR_HIST);
         }

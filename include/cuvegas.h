@@ -77,6 +77,12 @@ typedef struct cvg_result {
     int n_cubes;
     int n_strat;
     int n_intervals;
+    /* No reference counterpart: the same weighted mean's error and chi2/dof
+     * with the unbiased per-cube variance sigma_h^2 / (n_h - 1) in place of the
+     * reference's sigma_h^2 / n_h (which is low by n_h / (n_h - 1), a factor
+     * of 2 for the many cubes holding the minimum of 2 samples). */
+    double error_unbiased;
+    double chi2_dof_unbiased;
 } cvg_result;
 
 /* The 14 values the reference's integrate_cuda returns (cuda_wrapper.cu:869). */
@@ -145,6 +151,9 @@ int cvg_state_set_timing(cvg_state* st, int enable);
 int cvg_state_get_timing(cvg_state* st, double* fill_ms, double* iter_ms, int cap);
 /* Per-iteration estimates: I_k, sigma_k^2 and samples; returns the count. */
 int cvg_state_history(cvg_state* st, double* I, double* sigma2, long long* nevals, int cap);
+/* Per-iteration variance of I_k from the unbiased per-cube variances
+ * (sum_h sigma_h^2 / (n_h - 1)); no reference counterpart. Returns the count. */
+int cvg_state_history_unbiased(cvg_state* st, double* sigma2_unbiased, int cap);
 /* Number of cubes of a state. */
 long long cvg_state_n_cubes(const cvg_state* st);
 
@@ -166,12 +175,13 @@ int cvg_state_set_bins(cvg_state* st, const double* weights, const unsigned int*
 int cvg_state_set_iteration(cvg_state* st, int it);
 /* Restore the per-iteration record of a saved run (no reference counterpart;
  * the reference keeps no state between calls): iterations [0, count) get
- * I_k, sigma_k^2 and samples, and both the device iteration counter and the
+ * I_k, sigma_k^2, samples and the unbiased sigma_k^2 (NULL: sigma_k^2 again),
+ * and both the device iteration counter and the
  * count of iterations done become count, so the next run continues with
  * iteration count and result() averages the restored iterations >= skip.
  * Together with set_map / set_nh this resumes or warm-starts a run. */
 int cvg_state_set_history(cvg_state* st, const double* I, const double* sigma2, const long long* nevals,
-                          int count);
+                          const double* sigma2_unbiased, int count);
 
 /* Single steps, for callers that drive their own loop. */
 int cvg_state_fill(cvg_state* st);      /* one sampling pass with the current map */
@@ -230,9 +240,12 @@ int cvg_batch_create(cvg_program* prog, const cvg_params* common, const cvg_batc
                      int n_items, const double* theta, int n_theta, cvg_batch** out);
 void cvg_batch_destroy(cvg_batch* b);
 int cvg_batch_run(cvg_batch* b, int n_iter, int use_graph);
-/* Per item: weighted mean, error, chi2/dof, evaluations; loop_ms: device time. */
+/* Per item: weighted mean, error, chi2/dof, evaluations; loop_ms: device time;
+ * error and chi2/dof from the unbiased per-cube variances (as in cvg_result).
+ * Any output pointer may be NULL. */
 int cvg_batch_results(cvg_batch* b, double* result, double* error, double* chi2_dof,
-                      long long* nevals, double* loop_ms);
+                      long long* nevals, double* loop_ms, double* error_unbiased,
+                      double* chi2_dof_unbiased);
 int cvg_batch_history(cvg_batch* b, int item, double* I, double* sigma2, long long* nevals, int cap);
 int cvg_batch_item_info(cvg_batch* b, int item, int* n_dim, int* n_strat, long long* n_cubes);
 int cvg_batch_get_map(cvg_batch* b, int item, double* x_edges);

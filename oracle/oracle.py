@@ -120,7 +120,7 @@ def results(V, beta, nh, JF, JF2):
     JF = np.ascontiguousarray(JF, dtype=np.float64)
     JF2 = np.ascontiguousarray(JF2, dtype=np.float64)
     dh = np.zeros(nh.size)
-    sums = np.zeros(3)
+    sums = np.zeros(4)
     _lib.orc_results(nh.size, V, beta, nh.ctypes.data_as(_PI), _d(JF), _d(JF2), _d(dh), _d(sums))
     return dh, sums
 

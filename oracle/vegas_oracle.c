@@ -378,19 +378,23 @@ ORC_EXPORT int orc_cube_sums(long long C, const long long* nh_sum, long long K,
 
 /* ------------------------------------------------------------------------ */
 /* Per-cube results (cuda_wrapper.cu:26-49), same expression order.          */
-/* sums[0] = sum I_h, sums[1] = sum sigma_h^2/n_h, sums[2] = D = sum d_h.    */
+/* sums[0] = sum I_h, sums[1] = sum sigma_h^2/n_h, sums[2] = D = sum d_h,    */
+/* sums[3] = sum sigma_h^2/(n_h - 1) (the unbiased per-cube variance, an     */
+/* addition reported beside the reference's).                                */
 /* ------------------------------------------------------------------------ */
 
 ORC_EXPORT void orc_results(long long C, double V, double beta, const int* nh,
                             const double* JF, const double* JF2, double* dh, double* sums) {
     double* sr = (double*)malloc(sizeof(double) * (size_t)C);
     double* ss = (double*)malloc(sizeof(double) * (size_t)C);
+    double* su = (double*)malloc(sizeof(double) * (size_t)C);
     for (long long i = 0; i < C; ++i) {
         int neval = nh[i];
         double Ih = JF[i] / neval * V;
         double Sig2 = (JF2[i] / neval * V * V) - (Ih * Ih);
         sr[i] = Ih;
         ss[i] = Sig2 / neval;
+        su[i] = Sig2 / (neval - 1);
         double u = V * JF[i] / neval;
         double t = V * V * JF2[i] / neval - u * u;
         if (t < 0) t = 0;
@@ -399,7 +403,8 @@ ORC_EXPORT void orc_results(long long C, double V, double beta, const int* nh,
     sums[0] = orc_tree_sum(sr, C);
     sums[1] = orc_tree_sum(ss, C);
     sums[2] = orc_tree_sum(dh, C);
-    free(sr); free(ss);
+    sums[3] = orc_tree_sum(su, C);
+    free(sr); free(ss); free(su);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -812,7 +817,7 @@ ORC_EXPORT int orc_run(const orc_params* P, orc_out* O) {
         }
         cube_acc_free(&A);
 
-        double sums[3];
+        double sums[4];
         orc_results(C, V, P->beta, nh, JF, JF2, dh, sums);
         O->hist_I[it] = sums[0];
         O->hist_S2[it] = sums[1];

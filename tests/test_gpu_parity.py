@@ -148,6 +148,7 @@ def test_first_iteration_bit_exact(cv, orc, prec, layout, K):
     assert np.array_equal(st2.get_nh(), orc.alloc(odh, sums[2], N))
     I_k, S2, ne = st2.history()
     assert I_k[0] == sums[0] and S2[0] == sums[1] and ne[0] == S
+    assert st2.history_unbiased()[0] == sums[3]
     assert np.max(np.abs(st2.get_map() - orc.refine(alpha, ow, oc, x0))) < mtol
     if layout != "mixed":  # end to end against the oracle integrator from scratch
         r = orc.run(p, keep, st.n_cubes)
@@ -382,6 +383,7 @@ def test_first_iteration_bit_exact_shapes(cv, orc, prec, shape):
     assert np.array_equal(st.get_nh(), orc.alloc(odh, sums[2], N))
     I_k, S2, ne = st.history()
     assert I_k[0] == sums[0] and S2[0] == sums[1] and ne[0] == S
+    assert st.history_unbiased()[0] == sums[3]
     st.close()
 
 
@@ -415,6 +417,7 @@ def test_gaussian_first_iteration_bit_exact_fp64(cv, orc, case):
     assert np.array_equal(st.get_nh(), orc.alloc(odh, sums[2], N))
     I_k, S2, ne = st.history()
     assert I_k[0] == sums[0] and S2[0] == sums[1] and ne[0] == S
+    assert st.history_unbiased()[0] == sums[3]
     if case == "gauss4_config":
         # the rest of the configuration's 10 iterations: consistent with the
         # closed form (the 8-D peak of width 0.05 needs far more than 1e6
@@ -457,6 +460,7 @@ def test_many_blocks_bit_exact(cv, orc, prec):
     assert np.array_equal(st.get_nh(), onh)
     I_k, S2, ne = st.history()
     assert I_k[0] == sums[0] and S2[0] == sums[1] and ne[0] == S
+    assert st.history_unbiased()[0] == sums[3]
     # the new allocation's layout: every sample of the next pass lands in the
     # cube the oracle's prefix sums assign it to
     S1 = int(onh.sum())
@@ -522,13 +526,17 @@ def test_large_shape_properties(cv):
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_double_gaussian_full_size(cv, prec):
     """The 8-D double Gaussian at its full size (n = 1024, n_strat = 8: 16.8 M
-    cubes, 1e8 samples per iteration, 20 iterations, 5 skipped): the averaged
-    result is within 4 sigma and 1e-4 relative of the closed form with
-    chi2/dof < 2, and the invariants of every size hold (n_h >= 2, sum of n_h
-    <= N + 2C, total evaluations = sum of the iterations', map edges strictly
-    increasing from 0 to 1, per-cube sums of squares >= 0)."""
+    cubes, 1e8 samples per iteration) at the benchmark's run length (5
+    skipped + 20 averaged iterations): the averaged result is within 4 sigma
+    and 1e-4 relative of the closed form, chi2/dof < 2 with the unbiased
+    per-cube variances (the reference's sigma_h^2 / n_h understates the
+    variance of the many 2-sample cubes by half, so its chi2/dof runs near 1.8
+    in either form: below 2 in about 60% of seeds), the unbiased error at
+    least the reference's, and the invariants of every size hold (n_h >= 2,
+    sum of n_h <= N + 2C, total evaluations = sum of the iterations', map
+    edges strictly increasing from 0 to 1, per-cube sums of squares >= 0)."""
     from cuvegas_amd import integrands as I
-    d, n, ns, N, iters = 8, 1024, 8, 100_000_000, 20
+    d, n, ns, N, iters = 8, 1024, 8, 100_000_000, 25
     itg = I.double_gaussian(d)
     _say("double-gaussian-full-size", prec)
     st = _integrator(cv, itg, d, n, ns, N, iters, prec, skip=5)
@@ -543,7 +551,7 @@ def test_double_gaussian_full_size(cv, prec):
     assert C == 8 ** 8
     assert abs(r["result"] - itg.exact) < 4 * r["error"], (r["result"], itg.exact, r["error"])
     assert abs(r["result"] - itg.exact) <= 1e-4 * abs(itg.exact)
-    assert r["chi2_dof"] < 2.0
+    assert r["chi2_dof_unbiased"] < 2.0 and r["error_unbiased"] >= r["error"]
     assert nh.min() >= 2 and nh.sum() <= N + 2 * C
     assert r["tot_nevals"] == ne.sum() and len(ne) == iters
     assert np.all(np.diff(x, axis=1) > 0) and np.all(x[:, 0] == 0) and np.all(x[:, -1] == 1)

@@ -199,6 +199,26 @@ def test_results_constant_integrand(orc):
                            np.array([a * a + b * b]))
     assert dh[0] == pytest.approx(0.25 * (a - b) ** 2 / 4, rel=1e-15)
     assert sums[1] == pytest.approx(0.25 * (a - b) ** 2 / 4 / 2, rel=1e-15)
+    assert sums[3] == pytest.approx(0.25 * (a - b) ** 2 / 4, rel=1e-15)  # / (n_h - 1) = / 1
+
+
+def test_results_unbiased_variance_is_the_sample_variance_of_the_mean(orc):
+    """sums[3] = sum_h sigma_h^2 / (n_h - 1) is, cube by cube, the unbiased
+    estimate of the variance of I_h = V mean(f): the sample variance
+    (statistics.variance, divisor n - 1) of the V f values over n_h; sums[1]
+    (the reference's) is that times (n_h - 1) / n_h."""
+    import statistics
+    rng = np.random.default_rng(4)
+    nh = np.array([2, 3, 5, 17, 2, 9], dtype=np.int32)
+    V = 0.125
+    vals = [rng.normal(1.0, 0.7, n) for n in nh]
+    JF = np.array([v.sum() for v in vals])
+    JF2 = np.array([(v * v).sum() for v in vals])
+    _, sums = orc.results(V, 0.75, nh, JF, JF2)
+    unb = sum(statistics.variance((V * v).tolist()) / n for v, n in zip(vals, nh))
+    ref = sum(statistics.pvariance((V * v).tolist()) / n for v, n in zip(vals, nh))
+    assert sums[3] == pytest.approx(unb, rel=1e-12)
+    assert sums[1] == pytest.approx(ref, rel=1e-12)
 
 
 def test_alloc_invariants(orc):
