@@ -133,6 +133,7 @@ SIGNATURES = {
     "cvg_state_get_snapshot": (C.c_int, [_P, C.c_int, _PD, _PI]),
     "cvg_state_backward": (C.c_int, [_P, _PD, _PD, _PD, _PD, _PD]),
     "cvg_state_backward_timing": (C.c_int, [_P, _PD]),
+    "cvg_state_backward_noalloc": (C.c_int, [_P, _PD, _PD, _PD, _PD, _PD]),
     "cvg_batch_program_create": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(cvg_params), C.c_char_p,
                                            C.POINTER(_P)]),
     "cvg_batch_create": (C.c_int, [_P, C.POINTER(cvg_params), C.POINTER(cvg_batch_item), C.c_int, _PD,

@@ -22,6 +22,7 @@
 
 #include "../../include/cuvegas.h"
 #include "vg_steps.h"
+#include "vg_internal.h"
 
 #include "vg_embed.inc"  // kVgFillSrc, kVgPhiloxSrc, kVgMathSrc (generated from vg_fill.hip, vg_philox.h, vg_math.h)
 
@@ -352,6 +353,121 @@ int cvg_state::launch_realloc_copy() {
     VG_HIP(vg_launch_layout(stream, (int)C, nh, bsum, boff, sync + 1, s_total, s_cap, it_p, k_shift, run_cap, nh_sum,
                             run_cube, err, fold_list, fold_count, fold_cap, big_list, big_cap));
     return CVG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Backward replay (no reference counterpart), shared by cvg_state_backward
+// (buffers owned by the state) and cvg_state_backward_noalloc (buffers in the
+// library's static scratch array, vg_noalloc.hip).
+// ---------------------------------------------------------------------------
+int vg_fail(int code, const char* msg) { return fail(code, msg); }
+
+int vg_state_device(const cvg_state* st) { return st ? st->device : -1; }
+
+int vg_backward_layout(const cvg_state* st, vg_bw_layout* out) {
+    if (!st || !out) return fail(CVG_ERR_INVALID, "state is NULL");
+    const cvg_program* prog = st->prog;
+    if (!prog->has_grad || !prog->grad)
+        return fail(CVG_ERR_INVALID, "program was compiled without the gradient kernel (option grad)");
+    bool any = false;
+    for (int k = st->p.skip; k < st->iterations_done; ++k) any = any || st->recorded[k];
+    if (!any) return fail(CVG_ERR_INVALID, "no recorded iterations: enable recording before running");
+    const int P = st->p.n_theta, d = st->p.n_dim, n = st->p.n_intervals;
+    // rows of 64 cubes, cut into chunks so that at most kGradMaxUnits units
+    // (kGradSplit per chunk) hold partials
+    const long long rows = (st->C + 63) / 64;
+    const long long rpu = (rows * kGradSplit + kGradMaxUnits - 1) / kGradMaxUnits;
+    out->rows_per_unit = (int)(rpu > 0 ? rpu : 1);
+    out->units = (int)((rows + out->rows_per_unit - 1) / out->rows_per_unit) * kGradSplit;
+    out->part = ((long long)out->units + 1) * (2LL + 2LL * P);
+    out->theta = P;
+    out->nh_sum = st->C + 1;
+    out->x = (long long)d * (n + 1);
+    out->map = (long long)d * n;
+    return CVG_OK;
+}
+
+int vg_backward_run(cvg_state* st, const double* theta, const vg_bw_buffers& b, const vg_bw_layout& L,
+                    hipEvent_t ev0, hipEvent_t ev1, double* grad, double* grad_var, double* value,
+                    double* value_var) {
+    cvg_program* prog = st->prog;
+    const int P = st->p.n_theta, d = st->p.n_dim, n = st->p.n_intervals;
+    const int NV = 2 + 2 * P;
+    std::vector<int> its;
+    for (int k = st->p.skip; k < st->iterations_done; ++k)
+        if (st->recorded[k]) its.push_back(k);
+    // parameters for the replay: given theta, or the forward's
+    void* th = st->theta;
+    if (theta) {
+        if (st->p.precision == CVG_FP32C) {
+            std::vector<float> f(theta, theta + P);
+            VG_HIP(hipMemcpyAsync(b.theta, f.data(), 4 * (size_t)P, hipMemcpyHostToDevice, st->stream));
+            VG_HIP(hipStreamSynchronize(st->stream));
+        } else {
+            VG_HIP(hipMemcpyAsync(b.theta, theta, 8 * (size_t)P, hipMemcpyHostToDevice, st->stream));
+        }
+        th = b.theta;
+    }
+    std::vector<double> hist((size_t)VG_HIST * st->iterations_done);
+    VG_HIP(hipMemcpyAsync(hist.data(), st->hist, sizeof(double) * hist.size(), hipMemcpyDeviceToHost, st->stream));
+    const long long nx = (long long)d * (n + 1);
+    std::vector<double> tot((size_t)NV);
+    double W = 0.0, val = 0.0, vvar = 0.0;
+    std::vector<double> g((size_t)P, 0.0), gv((size_t)P, 0.0);
+    int C = (int)st->C;
+    double vol = 1.0;
+    if (st->p.has_domain)
+        for (int i = 0; i < d; ++i) vol *= st->domain_host[d + i];
+    unsigned seed = st->p.seed, sid = st->p.stream_id;
+    double replay_ms = 0.0;
+    for (int k : its) {
+        const long long slot = k - st->p.skip;
+        VG_HIP(hipMemcpyAsync(b.x, st->snap_x + slot * nx, sizeof(double) * nx, hipMemcpyDeviceToDevice, st->stream));
+        VG_HIP(vg_launch_tables(st->stream, d, n, b.x, b.map64, b.map32));
+        const int* snh = st->snap_nh + slot * st->C;
+        VG_HIP(vg_launch_layout(st->stream, C, snh, st->bsum, st->boff, st->sync + 1, b.s_total, (1LL << 62),
+                                st->it_p, st->k_shift, 0, b.nh_sum, nullptr, st->err, b.fold_list,
+                                b.fold_count, 1, nullptr, 0));
+        const void* map = st->p.precision == CVG_FP32C ? (const void*)b.map32 : (const void*)b.map64;
+        unsigned itu = (unsigned)k;
+        double V = st->V;
+        int rpu = L.rows_per_unit, nu = L.units;
+        double* part = b.part;
+        long long* nhs = b.nh_sum;
+        void* args[] = {(void*)&map, (void*)&snh, (void*)&nhs, (void*)&C, (void*)&rpu, (void*)&itu,
+                        (void*)&V, (void*)&st->domain, (void*)&vol, (void*)&th, (void*)&seed, (void*)&sid,
+                        (void*)&part};
+        if (ev0) VG_HIP(hipEventRecord(ev0, st->stream));
+        VG_HIP(hipModuleLaunchKernel(prog->grad, (unsigned)nu, 1, 1, 64, 1, 1, 0, st->stream, args, nullptr));
+        double* out = part + (long long)nu * NV;
+        void* rargs[] = {(void*)&part, (void*)&nu, (void*)&out};
+        VG_HIP(hipModuleLaunchKernel(prog->grad_reduce, (unsigned)NV, 1, 1, 1024, 1, 1, 0, st->stream, rargs,
+                                     nullptr));
+        if (ev1) VG_HIP(hipEventRecord(ev1, st->stream));
+        VG_HIP(hipMemcpyAsync(tot.data(), out, sizeof(double) * NV, hipMemcpyDeviceToHost, st->stream));
+        VG_HIP(hipStreamSynchronize(st->stream));
+        if (ev0 && ev1) {
+            float kms = 0.0f;
+            VG_HIP(hipEventElapsedTime(&kms, ev0, ev1));
+            replay_ms += kms;
+        }
+        const double w = 1.0 / hist[(size_t)VG_HIST * k + 1];  // forward weight, held fixed
+        W += w;
+        val += w * tot[0];
+        vvar += w * w * tot[1];
+        for (int q = 0; q < P; ++q) {
+            g[q] += w * tot[2 + q];
+            gv[q] += w * w * tot[2 + P + q];
+        }
+    }
+    if (ev0 && ev1) st->bw_replay_ms = replay_ms;
+    if (value) *value = val / W;
+    if (value_var) *value_var = vvar / (W * W);
+    for (int q = 0; q < P; ++q) {
+        if (grad) grad[q] = g[q] / W;
+        if (grad_var) grad_var[q] = gv[q] / (W * W);
+    }
+    return st->check_err();
 }
 
 // ---------------------------------------------------------------------------
@@ -1065,105 +1181,35 @@ int cvg_state_get_snapshot(cvg_state* st, int it, double* x_edges, int* nh) {
 
 int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double* grad_var,
                        double* value, double* value_var) {
-    if (!st) return fail(CVG_ERR_INVALID, "state is NULL");
-    cvg_program* prog = st->prog;
-    if (!prog->has_grad || !prog->grad)
-        return fail(CVG_ERR_INVALID, "program was compiled without the gradient kernel (option grad)");
-    const int P = st->p.n_theta, d = st->p.n_dim, n = st->p.n_intervals;
-    const int NV = 2 + 2 * P;
-    int rc;
-    std::vector<int> its;
-    for (int k = st->p.skip; k < st->iterations_done; ++k)
-        if (st->recorded[k]) its.push_back(k);
-    if (its.empty()) return fail(CVG_ERR_INVALID, "no recorded iterations: enable recording before running");
+    vg_bw_layout L;
+    int rc = vg_backward_layout(st, &L);
+    if (rc) return rc;
     if (!st->bw_part) {
-        // rows of 64 cubes, cut into chunks so that at most kGradMaxUnits units
-        // (kGradSplit per chunk) hold partials
-        const long long rows = (st->C + 63) / 64;
-        const long long rpu = (rows * kGradSplit + kGradMaxUnits - 1) / kGradMaxUnits;
-        st->bw_rows_per_unit = (int)(rpu > 0 ? rpu : 1);
-        st->bw_units = (int)((rows + st->bw_rows_per_unit - 1) / st->bw_rows_per_unit) * kGradSplit;
-        if ((rc = dalloc(&st->bw_part, ((long long)st->bw_units + 1) * NV))) return rc;
-        if ((rc = dalloc((double**)&st->bw_theta, P))) return rc;
-        if ((rc = dalloc(&st->bw_nh_sum, st->C + 1))) return rc;
+        // the state's own replay buffers, allocated by the first call and
+        // reused by every later one
+        if ((rc = dalloc(&st->bw_part, L.part))) return rc;
+        if ((rc = dalloc((double**)&st->bw_theta, L.theta))) return rc;
+        if ((rc = dalloc(&st->bw_nh_sum, L.nh_sum))) return rc;
         if ((rc = dalloc(&st->bw_s_total, 1))) return rc;
-        if ((rc = dalloc(&st->bw_x, (long long)d * (n + 1)))) return rc;
-        if ((rc = dalloc(&st->bw_map64, (long long)d * n))) return rc;
-        if ((rc = dalloc(&st->bw_map32, (long long)d * n))) return rc;
+        if ((rc = dalloc(&st->bw_x, L.x))) return rc;
+        if ((rc = dalloc(&st->bw_map64, L.map))) return rc;
+        if ((rc = dalloc(&st->bw_map32, L.map))) return rc;
         if ((rc = dalloc(&st->bw_fold_list, 1))) return rc;
         if ((rc = dalloc(&st->bw_fold_count, 2))) return rc;
         VG_HIP(hipEventCreate(&st->bw_ev0));
         VG_HIP(hipEventCreate(&st->bw_ev1));
     }
-    // parameters for the replay: given theta, or the forward's
-    void* th = st->theta;
-    if (theta) {
-        // the state's replay buffer: no allocation per call, nothing to free on
-        // an error return
-        if (st->p.precision == CVG_FP32C) {
-            std::vector<float> f(theta, theta + P);
-            VG_HIP(hipMemcpyAsync(st->bw_theta, f.data(), 4 * (size_t)P, hipMemcpyHostToDevice, st->stream));
-            VG_HIP(hipStreamSynchronize(st->stream));
-        } else {
-            VG_HIP(hipMemcpyAsync(st->bw_theta, theta, 8 * (size_t)P, hipMemcpyHostToDevice, st->stream));
-        }
-        th = st->bw_theta;
-    }
-    std::vector<double> hist((size_t)VG_HIST * st->iterations_done);
-    VG_HIP(hipMemcpyAsync(hist.data(), st->hist, sizeof(double) * hist.size(), hipMemcpyDeviceToHost, st->stream));
-    const long long nx = (long long)d * (n + 1);
-    std::vector<double> tot((size_t)NV);
-    double W = 0.0, val = 0.0, vvar = 0.0;
-    std::vector<double> g((size_t)P, 0.0), gv((size_t)P, 0.0);
-    int C = (int)st->C;
-    double vol = 1.0;
-    if (st->p.has_domain)
-        for (int i = 0; i < d; ++i) vol *= st->domain_host[d + i];
-    unsigned seed = st->p.seed, sid = st->p.stream_id;
-    st->bw_replay_ms = 0.0;
-    for (int k : its) {
-        const long long slot = k - st->p.skip;
-        VG_HIP(hipMemcpyAsync(st->bw_x, st->snap_x + slot * nx, sizeof(double) * nx, hipMemcpyDeviceToDevice, st->stream));
-        VG_HIP(vg_launch_tables(st->stream, d, n, st->bw_x, st->bw_map64, st->bw_map32));
-        const int* snh = st->snap_nh + slot * st->C;
-        VG_HIP(vg_launch_layout(st->stream, C, snh, st->bsum, st->boff, st->sync + 1, st->bw_s_total, (1LL << 62), st->it_p,
-                                st->k_shift, 0, st->bw_nh_sum, nullptr, st->err, st->bw_fold_list,
-                                st->bw_fold_count, 1, nullptr, 0));
-        const void* map = st->p.precision == CVG_FP32C ? (const void*)st->bw_map32 : (const void*)st->bw_map64;
-        unsigned itu = (unsigned)k;
-        double V = st->V;
-        int rpu = st->bw_rows_per_unit, nu = st->bw_units;
-        void* args[] = {(void*)&map, (void*)&snh, (void*)&st->bw_nh_sum, (void*)&C, (void*)&rpu, (void*)&itu,
-                        (void*)&V, (void*)&st->domain, (void*)&vol, (void*)&th, (void*)&seed, (void*)&sid,
-                        (void*)&st->bw_part};
-        VG_HIP(hipEventRecord(st->bw_ev0, st->stream));
-        VG_HIP(hipModuleLaunchKernel(prog->grad, (unsigned)nu, 1, 1, 64, 1, 1, 0, st->stream, args, nullptr));
-        double* out = st->bw_part + (long long)nu * NV;
-        void* rargs[] = {(void*)&st->bw_part, (void*)&nu, (void*)&out};
-        VG_HIP(hipModuleLaunchKernel(prog->grad_reduce, (unsigned)NV, 1, 1, 1024, 1, 1, 0, st->stream, rargs,
-                                     nullptr));
-        VG_HIP(hipEventRecord(st->bw_ev1, st->stream));
-        VG_HIP(hipMemcpyAsync(tot.data(), out, sizeof(double) * NV, hipMemcpyDeviceToHost, st->stream));
-        VG_HIP(hipStreamSynchronize(st->stream));
-        float kms = 0.0f;
-        VG_HIP(hipEventElapsedTime(&kms, st->bw_ev0, st->bw_ev1));
-        st->bw_replay_ms += kms;
-        const double w = 1.0 / hist[(size_t)VG_HIST * k + 1];  // forward weight, held fixed
-        W += w;
-        val += w * tot[0];
-        vvar += w * w * tot[1];
-        for (int q = 0; q < P; ++q) {
-            g[q] += w * tot[2 + q];
-            gv[q] += w * w * tot[2 + P + q];
-        }
-    }
-    if (value) *value = val / W;
-    if (value_var) *value_var = vvar / (W * W);
-    for (int q = 0; q < P; ++q) {
-        if (grad) grad[q] = g[q] / W;
-        if (grad_var) grad_var[q] = gv[q] / (W * W);
-    }
-    return st->check_err();
+    vg_bw_buffers b;
+    b.part = st->bw_part;
+    b.theta = st->bw_theta;
+    b.nh_sum = st->bw_nh_sum;
+    b.s_total = st->bw_s_total;
+    b.x = st->bw_x;
+    b.map64 = st->bw_map64;
+    b.map32 = st->bw_map32;
+    b.fold_list = st->bw_fold_list;
+    b.fold_count = st->bw_fold_count;
+    return vg_backward_run(st, theta, b, L, st->bw_ev0, st->bw_ev1, grad, grad_var, value, value_var);
 }
 
 int cvg_state_backward_timing(cvg_state* st, double* replay_ms) {

@@ -208,6 +208,20 @@ int cvg_state_set_record(cvg_state* st, int enable);
 int cvg_state_get_snapshot(cvg_state* st, int it, double* x_edges, int* nh);
 int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double* grad_var,
                        double* value, double* value_var);
+/* cvg_state_backward without any allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as a
+ * training step captured in a HIP graph or a serving loop kept off the
+ * allocator.  Same arguments, kernels and results, bit for bit, as
+ * cvg_state_backward; the replay buffers live in a static 64 MiB device array
+ * of the library (one copy per GPU, held until the process ends) instead of
+ * buffers the state allocates on its first backward.  Returns
+ * CVG_ERR_CAPACITY, launching nothing, when the state's replay needs more
+ * (about 8 (C + 1) + 8 (2 + 2 n_theta) (units + 1) bytes, units <= 65,536:
+ * 41 MB for a million cubes and 32 parameters; up to about 4.5 M cubes).  Calls that overlap on one
+ * GPU share the array and race; the replay itself does not time its kernels
+ * (cvg_state_backward_timing keeps the last timed call's figure). */
+int cvg_state_backward_noalloc(cvg_state* st, const double* theta, double* grad, double* grad_var,
+                               double* value, double* value_var);
 /* Device time (ms, HIP events on the state's stream) of the last backward
  * call's replay kernels (the per-iteration replay and its unit reduction),
  * summed over the recorded iterations. */
