@@ -474,12 +474,6 @@ vg_dump(const vg_map_t* __restrict__ gmap, const long long* __restrict__ nh_sum,
 // launch's work units, summed in unit order by vg_grad_reduce.
 // ---------------------------------------------------------------------------
 constexpr int NT = VG_NTHETA;
-// Cubes with more samples than this are replayed by the whole wave (samples
-// strided over the lanes); smaller ones by one lane each.
-#ifndef VG_GRAD_LMAX
-#define VG_GRAD_LMAX 8
-#endif
-constexpr int VG_GRAD_LANE_MAX = VG_GRAD_LMAX;
 // The map is read through L1/L2: the LDS holds the transpose buffers (a
 // 48 KB FP32 map staged in LDS as well measured slower).
 
@@ -602,122 +596,128 @@ __device__ inline void vg_wave_tsum(double* tb, int lane, double a0, const doubl
     }
 }
 
-// Work units of the backward launch: the cubes are cut into chunks of
-// rows_per_unit rows of 64, and each chunk into VG_GRAD_SPLIT units; unit
-// (chunk c, slot s) replays the chunk's wave-replayed cubes (more than
-// VG_GRAD_LANE_MAX samples) whose index among them is s mod VG_GRAD_SPLIT, and
-// slot 0 also the chunk's other cubes.  Large cubes cluster where the
-// integrand peaks, so spreading them over the slots keeps the costliest unit
-// near the cost of a few large cubes.
-#ifndef VG_GRAD_SPLIT
-#define VG_GRAD_SPLIT 4
-#endif
-
-// One wave (one block) per work unit; the hardware hands units to SIMDs as
-// they free up.  Within a row, each wave-replayed cube is replayed by the
-// whole wave (samples strided over the lanes) and its lanes' partial sums are
-// combined by two transposed sums, after which the lane holding value k
-// applies the cube expressions to it; then the row's other cubes are
-// replayed one per lane and their contributions summed across the wave the
-// same way.  Everything goes into the unit's LDS accumulator (entry 0, 1:
-// value terms; 2 + p, 2 + NT + p: gradient terms, each written by one lane)
-// in that fixed order, and the unit writes it to part[q][unit] (value q of
-// 2 + 2 NT, unit-contiguous); vg_grad_reduce sums the units in index order.
-// Two waves per SIMD (at most 256 VGPRs; one wave per SIMD without spills
-// measured slower in both forms).
+// Work layout (built on the device for each recorded allocation, see
+// vg_launch_grad_order): the cubes sorted by sample count, largest first;
+// sorted cube c owns lanes [lane_start[c], lane_start[c + 1]) of the launch,
+// L_c of them (a power of two, about n / 16, 1 for cubes of up to 16
+// samples), and wave w covers lanes [64 w, 64 w + 64), which start at cube
+// wave_first[w].  Lane j of a cube replays its samples j, j + L, j + 2L, ...,
+// so no lane replays more than about 32 samples; the cube's lanes then combine
+// their sums by a butterfly over the cube's aligned block of L lanes, the
+// first applies the cube expressions, and the wave's contributions are summed
+// by two transposed sums into the unit's LDS accumulator (entry 0, 1: value
+// terms; 2 + p, 2 + NT + p: gradient terms, each written by one lane), in a
+// fixed order.  A work unit (one wave, one block) takes slots_per_unit
+// consecutive waves of the layout and writes its accumulator to part[q][unit]
+// (value q of 2 + 2 NT, unit-contiguous); vg_grad_reduce sums the units in
+// index order.  The largest cubes come first, so the hardware starts the
+// longest units first.  Two waves per SIMD (at most 256 VGPRs).
 extern "C" __global__ void __launch_bounds__(64, 2)
-vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh,
-        const long long* __restrict__ nh_sum, int C, int rows_per_unit, unsigned it, double V,
-        const double* __restrict__ domain, double domain_vol, const vg_real* __restrict__ theta,
-        unsigned seed, unsigned stream, double* __restrict__ part) {
+vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh, const long long* __restrict__ nh_sum,
+        const int* __restrict__ order, const int* __restrict__ lane_start, const int* __restrict__ wave_first,
+        int C, int slots_per_unit, unsigned it, double V, const double* __restrict__ domain, double domain_vol,
+        const vg_real* __restrict__ theta, unsigned seed, unsigned stream, double* __restrict__ part) {
     constexpr int NV = 2 + 2 * NT;
     __shared__ double wacc[NV];
     __shared__ double tb[64 * VG_TW];
+    __shared__ int lmap[64];
     const vg_map_t* map = gmap;
     const int lane = threadIdx.x;
     for (int q = lane; q < NV; q += 64) wacc[q] = 0.0;
-    const int slot = (int)(blockIdx.x % VG_GRAD_SPLIT);
-    const long long row0 = (long long)(blockIdx.x / VG_GRAD_SPLIT) * rows_per_unit;
-    int nbig = 0;  // wave-replayed cubes of the chunk seen so far
-    for (int r = 0; r < rows_per_unit; ++r) {
-        const long long g0 = (row0 + r) * 64;
-        if (g0 >= C) break;
-        const long long h = g0 + lane;
-        const int n = h < C ? nh[h] : 0;
-        const bool own = slot == 0 && h < C && n <= VG_GRAD_LANE_MAX;
-        unsigned long long big = __ballot(h < C && n > VG_GRAD_LANE_MAX);
-        while (big) {
-            const int j = __ffsll((long long)big) - 1;
-            big &= big - 1;
-            if ((nbig++) % VG_GRAD_SPLIT != slot) continue;
-            const long long hb = g0 + j;
-            const int nb = __shfl(n, j, 64);
-            const long long s0 = nh_sum[hb];
-            double sf = 0.0, sf2 = 0.0, g1[NT], g2[NT];
+    const long long total = lane_start[C];
+    const long long waves = (total + 63) / 64;
+    for (int r = 0; r < slots_per_unit; ++r) {
+        const long long w = (long long)blockIdx.x * slots_per_unit + r;
+        if (w >= waves) break;
+        // lane -> sorted cube: the wave's cubes [c0, cend) write their lanes' slots
+        const int c0 = wave_first[w], cend = wave_first[w + 1];
+        const long long base = 64 * w;
+        if (c0 + lane < cend) {
+            const int s0 = (int)(lane_start[c0 + lane] - base), s1 = (int)(lane_start[c0 + lane + 1] - base);
+            for (int t = s0; t < s1; ++t) lmap[t] = c0 + lane;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const bool act = base + lane < total;
+        const int c = act ? lmap[lane] : c0;
+        const int ls = lane_start[c];
+        const int L = act ? lane_start[c + 1] - ls : 1;
+        const int j = act ? (int)(base + lane - ls) : 0;
+        const int h = order[c];
+        const int n = act ? nh[h] : 0;
+        // every lane has read its slot before the next slot's writes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double sf = 0.0, sf2 = 0.0, g1[NT], g2[NT];
 #pragma unroll
-            for (int p = 0; p < NT; ++p) { g1[p] = 0.0; g2[p] = 0.0; }
-            for (int e = lane; e < nb; e += 64) {
-                double jf, jg[NT];
-                vg_grad_sample(hb, s0 + e, it, map, domain, domain_vol, theta, seed, stream, jf, jg);
-                sf = sf + jf;
-                sf2 = fma(jf, jf, sf2);
+        for (int p = 0; p < NT; ++p) { g1[p] = 0.0; g2[p] = 0.0; }
+        const long long s0 = nh_sum[h];
+        for (int e = j; e < n; e += L) {
+            double jf, jg[NT];
+            vg_grad_sample(h, s0 + e, it, map, domain, domain_vol, theta, seed, stream, jf, jg);
+            sf = sf + jf;
+            sf2 = fma(jf, jf, sf2);
 #pragma unroll
-                for (int p = 0; p < NT; ++p) { g1[p] = g1[p] + jg[p]; g2[p] = fma(jg[p], jg[p], g2[p]); }
-            }
-            // the cube's sums of J f and (J f)^2 (value k = 0), of J df_p and
-            // (J df_p)^2 (k = 1 + p); the lane holding k applies the cube
-            // expressions to it
-            double c1[VG_NCH], c2[VG_NCH];
-            vg_wave_tsum(tb, lane, sf, g1, c1);
-            vg_wave_tsum(tb, lane, sf2, g2, c2);
+            for (int p = 0; p < NT; ++p) { g1[p] = g1[p] + jg[p]; g2[p] = fma(jg[p], jg[p], g2[p]); }
+        }
+        // The cube's terms split into parts linear in the sums of squares,
+        // which each lane forms from its own partial sums,
+        //   sigma_h^2 / n_h:  V^2 sum (J f)^2 / n^2        (minus I_h^2 / n)
+        //   Var_h,p:          V^2 sum (J df_p)^2 / (n (n-1))  (minus G_h,p^2 / (n-1)),
+        // and parts in the squares of the sums, which need the cube's whole
+        // sums of J f and J df_p: a butterfly over the cube's aligned block of
+        // L lanes (the wave's first cube has the most lanes) gives them to
+        // every lane of the block, and the first lane adds I_h, G_h,p and the
+        // squared terms.
+        const double inv_n = act ? 1.0 / n : 0.0;
+        const double q2 = V * V * inv_n;
+        sf2 = sf2 * q2 * inv_n;
+        const double q2g = act ? q2 / (n - 1) : 0.0;
 #pragma unroll
-            for (int c = 0; c < VG_NCH; ++c) {
-                const int k = c * VG_TW + lane;
-                if (lane >= VG_TW || k > NT) continue;
-                if (k == 0) {
-                    const double Ih = c1[c] / nb * V;
-                    const double Sig2 = (c2[c] / nb * V * V) - (Ih * Ih);
-                    wacc[0] = wacc[0] + Ih;
-                    wacc[1] = wacc[1] + Sig2 / nb;
-                } else {
-                    const double Gh = c1[c] * (V / nb);
-                    const double Gv = c2[c] * (V * V / nb) - Gh * Gh;
-                    wacc[1 + k] = wacc[1 + k] + Gh;
-                    wacc[1 + NT + k] = wacc[1 + NT + k] + Gv * (1.0 / (nb - 1));
-                }
+        for (int p = 0; p < NT; ++p) g2[p] = g2[p] * q2g;
+        const int lmax = lane_start[c0 + 1] - lane_start[c0];
+        for (int o = 1; o < lmax; o <<= 1) {
+            const bool in = o < L;
+            const double t = __shfl_xor(sf, o, 64);
+            sf = in ? sf + t : sf;
+#pragma unroll
+            for (int p = 0; p < NT; ++p) {
+                const double u = __shfl_xor(g1[p], o, 64);
+                g1[p] = in ? g1[p] + u : g1[p];
             }
         }
-        if (__ballot(own)) {
-            double sf = 0.0, sf2 = 0.0, g1[NT], g2[NT];
+        if (act && j == 0) {
+            const double Ih = sf * inv_n * V;
+            sf = Ih;
+            sf2 = sf2 - Ih * Ih * inv_n;
+            const double vn = V * inv_n, in1 = 1.0 / (n - 1);
 #pragma unroll
-            for (int p = 0; p < NT; ++p) { g1[p] = 0.0; g2[p] = 0.0; }
-            if (own) {
-                const long long s0 = nh_sum[h];
-                for (int e = 0; e < n; ++e) {
-                    double jf, jg[NT];
-                    vg_grad_sample(h, s0 + e, it, map, domain, domain_vol, theta, seed, stream, jf, jg);
-                    sf = sf + jf;
-                    sf2 = fma(jf, jf, sf2);
-#pragma unroll
-                    for (int p = 0; p < NT; ++p) { g1[p] = g1[p] + jg[p]; g2[p] = fma(jg[p], jg[p], g2[p]); }
-                }
-                vg_grad_cube(n, V, sf, sf2, g1, g2);
+            for (int p = 0; p < NT; ++p) {
+                const double Gh = g1[p] * vn;
+                g1[p] = Gh;
+                g2[p] = g2[p] - Gh * Gh * in1;
             }
-            // sums of I_h and sigma_h^2 / n_h (k = 0), of G_h,p and Var_h,p (k = 1 + p)
-            double c1[VG_NCH], c2[VG_NCH];
-            vg_wave_tsum(tb, lane, sf, g1, c1);
-            vg_wave_tsum(tb, lane, sf2, g2, c2);
+        } else {
+            sf = 0.0;
 #pragma unroll
-            for (int c = 0; c < VG_NCH; ++c) {
-                const int k = c * VG_TW + lane;
-                if (lane >= VG_TW || k > NT) continue;
-                if (k == 0) {
-                    wacc[0] = wacc[0] + c1[c];
-                    wacc[1] = wacc[1] + c2[c];
-                } else {
-                    wacc[1 + k] = wacc[1 + k] + c1[c];
-                    wacc[1 + NT + k] = wacc[1 + NT + k] + c2[c];
-                }
+            for (int p = 0; p < NT; ++p) g1[p] = 0.0;
+        }
+        // sums of I_h and sigma_h^2 / n_h (k = 0), of G_h,p and Var_h,p (k = 1 + p)
+        double c1[VG_NCH], c2[VG_NCH];
+        vg_wave_tsum(tb, lane, sf, g1, c1);
+        vg_wave_tsum(tb, lane, sf2, g2, c2);
+#pragma unroll
+        for (int cc = 0; cc < VG_NCH; ++cc) {
+            const int kk = cc * VG_TW + lane;
+            if (lane >= VG_TW || kk > NT) continue;
+            if (kk == 0) {
+                wacc[0] = wacc[0] + c1[cc];
+                wacc[1] = wacc[1] + c2[cc];
+            } else {
+                wacc[1 + kk] = wacc[1 + kk] + c1[cc];
+                wacc[1 + NT + kk] = wacc[1 + NT + kk] + c2[cc];
             }
         }
     }

@@ -50,9 +50,7 @@ double now_ms() {
 
 constexpr int kLdsBytes = 160 * 1024;  // gfx950 LDS per workgroup
 constexpr int kFillBlock = 1024;
-// Backward replay: work units per chunk of cube rows (vg_grad's VG_GRAD_SPLIT)
-// and the most units one launch writes partials for.
-constexpr int kGradSplit = 4;
+// Backward replay: the most work units (waves) one launch writes partials for.
 constexpr long long kGradMaxUnits = 65536;
 constexpr int kDefaultRun = 16;
 
@@ -223,6 +221,9 @@ struct cvg_state {
     double* bw_part = nullptr;  // [2 + 2 n_theta][bw_units] unit partials, then the totals
     int bw_units = 0, bw_rows_per_unit = 0;
     void* bw_theta = nullptr;   // parameters of a replay at another theta (n_theta doubles)
+    unsigned char* bw_keys = nullptr;  // [2 C] size classes, then sorted
+    int* bw_idx = nullptr;             // cube indices, replay order, lanes, lane starts, wave starts
+    void* bw_sort_tmp = nullptr;       // radix-sort workspace
     hipEvent_t bw_ev0 = nullptr, bw_ev1 = nullptr;
     double bw_replay_ms = 0.0;  // device time of the last backward's replay kernels
     std::vector<char> recorded;  // per iteration: snapshot taken
@@ -246,7 +247,8 @@ struct cvg_state {
                         JS, run_cube, win_first, win_last, part, bsum, boff, sync, s_total,
                         it_p, dsum, err, hist, domain, theta, fold_list, fold_count, big_list, bw_fold_list,
                         bw_fold_count, snap_x, snap_nh, bw_nh_sum,
-                        bw_s_total, bw_x, bw_map64, bw_map32, bw_part, bw_theta};
+                        bw_s_total, bw_x, bw_map64, bw_map32, bw_part, bw_theta, bw_keys, bw_idx,
+                        bw_sort_tmp};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -373,17 +375,21 @@ int vg_backward_layout(const cvg_state* st, vg_bw_layout* out) {
     for (int k = st->p.skip; k < st->iterations_done; ++k) any = any || st->recorded[k];
     if (!any) return fail(CVG_ERR_INVALID, "no recorded iterations: enable recording before running");
     const int P = st->p.n_theta, d = st->p.n_dim, n = st->p.n_intervals;
-    // rows of 64 cubes, cut into chunks so that at most kGradMaxUnits units
-    // (kGradSplit per chunk) hold partials
-    const long long rows = (st->C + 63) / 64;
-    const long long rpu = (rows * kGradSplit + kGradMaxUnits - 1) / kGradMaxUnits;
-    out->rows_per_unit = (int)(rpu > 0 ? rpu : 1);
-    out->units = (int)((rows + out->rows_per_unit - 1) / out->rows_per_unit) * kGradSplit;
+    // waves of the replay layout (at most vg_grad_max_waves for any allocation
+    // of the state's sample capacity), grouped so that at most kGradMaxUnits
+    // units hold partials
+    out->waves = vg_grad_max_waves(st->C, st->s_cap);
+    const long long spu = (out->waves + kGradMaxUnits - 1) / kGradMaxUnits;
+    out->slots_per_unit = (int)(spu > 0 ? spu : 1);
+    out->units = (int)((out->waves + out->slots_per_unit - 1) / out->slots_per_unit);
     out->part = ((long long)out->units + 1) * (2LL + 2LL * P);
     out->theta = P;
     out->nh_sum = st->C + 1;
     out->x = (long long)d * (n + 1);
     out->map = (long long)d * n;
+    out->cubes = st->C;
+    out->sort_tmp = (long long)vg_grad_order_tmp_bytes((int)st->C);
+    if (out->sort_tmp <= 0) return fail(CVG_ERR_HIP, "radix sort workspace query failed");
     return CVG_OK;
 }
 
@@ -428,15 +434,18 @@ int vg_backward_run(cvg_state* st, const double* theta, const vg_bw_buffers& b, 
         VG_HIP(vg_launch_layout(st->stream, C, snh, st->bsum, st->boff, st->sync + 1, b.s_total, (1LL << 62),
                                 st->it_p, st->k_shift, 0, b.nh_sum, nullptr, st->err, b.fold_list,
                                 b.fold_count, 1, nullptr, 0));
+        VG_HIP(vg_launch_grad_order(st->stream, C, snh, b.keys, b.keys_sorted, b.idx, b.order, b.lanes,
+                                    b.lane_start, b.wave_first, b.sort_tmp, (size_t)L.sort_tmp));
         const void* map = st->p.precision == CVG_FP32C ? (const void*)b.map32 : (const void*)b.map64;
         unsigned itu = (unsigned)k;
         double V = st->V;
-        int rpu = L.rows_per_unit, nu = L.units;
+        int spu = L.slots_per_unit, nu = L.units;
         double* part = b.part;
         long long* nhs = b.nh_sum;
-        void* args[] = {(void*)&map, (void*)&snh, (void*)&nhs, (void*)&C, (void*)&rpu, (void*)&itu,
-                        (void*)&V, (void*)&st->domain, (void*)&vol, (void*)&th, (void*)&seed, (void*)&sid,
-                        (void*)&part};
+        int *order = b.order, *lane_start = b.lane_start, *wave_first = b.wave_first;
+        void* args[] = {(void*)&map, (void*)&snh, (void*)&nhs, (void*)&order, (void*)&lane_start,
+                        (void*)&wave_first, (void*)&C, (void*)&spu, (void*)&itu, (void*)&V, (void*)&st->domain,
+                        (void*)&vol, (void*)&th, (void*)&seed, (void*)&sid, (void*)&part};
         if (ev0) VG_HIP(hipEventRecord(ev0, st->stream));
         VG_HIP(hipModuleLaunchKernel(prog->grad, (unsigned)nu, 1, 1, 64, 1, 1, 0, st->stream, args, nullptr));
         double* out = part + (long long)nu * NV;
@@ -617,7 +626,6 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
         "-DVG_LDS_MAP=" + std::to_string(prog->lds_map),
         "-DVG_HAS_DOMAIN=" + std::to_string(prog->has_domain),
         "-DVG_GRAD=" + std::to_string(prog->has_grad),
-        "-DVG_GRAD_SPLIT=" + std::to_string(kGradSplit),
         "-DVG_BIN32=" + std::to_string(prog->bin32),
         "-DVG_PAIR=" + std::to_string(prog->pair),
         "-DVG_PIPE=" + std::to_string(prog->pipe),
@@ -1196,6 +1204,9 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         if ((rc = dalloc(&st->bw_map32, L.map))) return rc;
         if ((rc = dalloc(&st->bw_fold_list, 1))) return rc;
         if ((rc = dalloc(&st->bw_fold_count, 2))) return rc;
+        if ((rc = dalloc(&st->bw_keys, 2 * L.cubes))) return rc;
+        if ((rc = dalloc(&st->bw_idx, 4 * L.cubes + 2 + L.waves + 1))) return rc;
+        if ((rc = dalloc((char**)&st->bw_sort_tmp, L.sort_tmp))) return rc;
         VG_HIP(hipEventCreate(&st->bw_ev0));
         VG_HIP(hipEventCreate(&st->bw_ev1));
     }
@@ -1209,6 +1220,14 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
     b.map32 = st->bw_map32;
     b.fold_list = st->bw_fold_list;
     b.fold_count = st->bw_fold_count;
+    b.keys = st->bw_keys;
+    b.keys_sorted = st->bw_keys + L.cubes;
+    b.idx = st->bw_idx;
+    b.order = st->bw_idx + L.cubes;
+    b.lanes = st->bw_idx + 2 * L.cubes;
+    b.lane_start = st->bw_idx + 3 * L.cubes + 1;
+    b.wave_first = st->bw_idx + 4 * L.cubes + 2;
+    b.sort_tmp = st->bw_sort_tmp;
     return vg_backward_run(st, theta, b, L, st->bw_ev0, st->bw_ev1, grad, grad_var, value, value_var);
 }
 

@@ -18,12 +18,20 @@ struct vg_bw_buffers {
     float2* map32 = nullptr;       // [d n] (FP32 form)
     int* fold_list = nullptr;      // [1]
     int* fold_count = nullptr;     // [2]
+    unsigned char* keys = nullptr;         // [C] size classes of the recorded allocation
+    unsigned char* keys_sorted = nullptr;  // [C]
+    int* idx = nullptr;                    // [C] cube indices
+    int* order = nullptr;                  // [C] cubes by size, largest first
+    int* lanes = nullptr;                  // [C + 1] lanes of each sorted cube
+    int* lane_start = nullptr;             // [C + 1] their exclusive prefix sum
+    int* wave_first = nullptr;             // [waves + 1] first sorted cube of each wave
+    void* sort_tmp = nullptr;              // sort_tmp bytes of radix-sort and scan workspace
 };
 
 // Element counts of those buffers for a state, and the replay's launch shape.
 struct vg_bw_layout {
-    long long part = 0, theta = 0, nh_sum = 0, x = 0, map = 0;
-    int units = 0, rows_per_unit = 0;
+    long long part = 0, theta = 0, nh_sum = 0, x = 0, map = 0, cubes = 0, waves = 0, sort_tmp = 0;
+    int units = 0, slots_per_unit = 0;
 };
 
 // Checks that the state can be replayed (program compiled with the gradient

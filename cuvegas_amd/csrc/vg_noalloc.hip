@@ -67,9 +67,12 @@ extern "C" int cvg_state_backward_noalloc(cvg_state* st, const double* theta, do
     int rc = vg_backward_layout(st, &L);
     if (rc) return rc;
     // the buffers of vg_bw_buffers, each on a 256-byte boundary
-    const long long sizes[9] = {round256(8 * L.part),  round256(8 * L.theta + 8), round256(8 * L.nh_sum),
-                                round256(8),           round256(8 * L.x),         round256(16 * L.map),
-                                round256(8 * L.map),   round256(4),               round256(8)};
+    const long long sizes[17] = {round256(8 * L.part),  round256(8 * L.theta + 8), round256(8 * L.nh_sum),
+                                 round256(8),           round256(8 * L.x),         round256(16 * L.map),
+                                 round256(8 * L.map),   round256(4),               round256(8),
+                                 round256(L.cubes),     round256(L.cubes),         round256(4 * L.cubes),
+                                 round256(4 * L.cubes), round256(4 * L.cubes + 4), round256(4 * L.cubes + 4),
+                                 round256(4 * L.waves + 4), round256(L.sort_tmp)};
     long long total = 0;
     for (long long b : sizes) total += b;
     if (total > kNoallocBytes)
@@ -98,5 +101,21 @@ extern "C" int cvg_state_backward_noalloc(cvg_state* st, const double* theta, do
     b.fold_list = reinterpret_cast<int*>(p);
     p += sizes[7];
     b.fold_count = reinterpret_cast<int*>(p);
+    p += sizes[8];
+    b.keys = reinterpret_cast<unsigned char*>(p);
+    p += sizes[9];
+    b.keys_sorted = reinterpret_cast<unsigned char*>(p);
+    p += sizes[10];
+    b.idx = reinterpret_cast<int*>(p);
+    p += sizes[11];
+    b.order = reinterpret_cast<int*>(p);
+    p += sizes[12];
+    b.lanes = reinterpret_cast<int*>(p);
+    p += sizes[13];
+    b.lane_start = reinterpret_cast<int*>(p);
+    p += sizes[14];
+    b.wave_first = reinterpret_cast<int*>(p);
+    p += sizes[15];
+    b.sort_tmp = p;
     return vg_backward_run(st, theta, b, L, nullptr, nullptr, grad, grad_var, value, value_var);
 }

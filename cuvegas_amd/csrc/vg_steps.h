@@ -93,3 +93,16 @@ hipError_t vg_launch_batch_iteration_steps(hipStream_t st, const vg_bitem* items
 hipError_t vg_launch_batch_refine(hipStream_t st, const vg_bitem* items, int n_refine, const int* rd_item,
                                   const int* rd_dim, int n, double alpha, int init, double* x_all,
                                   double* weights, unsigned* counts, double2* map64, float2* map32);
+
+// Work layout of the backward replay (vg_grad) for the recorded allocation nh:
+// order (C ints) the cubes sorted by sample count, largest first; lanes (C + 1
+// ints) and lane_start (C + 1 ints, exclusive prefix) the lanes of each sorted
+// cube; wave_first (vg_grad_max_waves(C, S) + 1 ints) the first sorted cube of
+// each wave, then C.  keys, keys_sorted: C bytes; idx: C ints; tmp:
+// vg_grad_order_tmp_bytes(C) bytes (0: the size query failed).  S bounds the
+// allocation's samples.
+size_t vg_grad_order_tmp_bytes(int C);
+long long vg_grad_max_waves(long long C, long long S);
+hipError_t vg_launch_grad_order(hipStream_t st, int C, const int* nh, unsigned char* keys,
+                                unsigned char* keys_sorted, int* idx, int* order, int* lanes, int* lane_start,
+                                int* wave_first, void* tmp, size_t tmp_bytes);

@@ -7,6 +7,7 @@
 // so the stratification weights D = sum d_h and the integer allocation n_h are
 // reproducible bit for bit.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 #include "vg_math.h"
 #include "vg_steps.h"
 
@@ -1301,4 +1302,106 @@ hipError_t vg_launch_batch_refine(hipStream_t st, const vg_bitem* items, int n_r
     hipLaunchKernelGGL(vg_brefine_kernel, dim3(n_refine), dim3(init ? 256 : VG_TREE), lds, st, items,
                        rd_item, rd_dim, n, alpha, init, x_all, weights, counts, map64, map32);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Work layout of the backward replay (vg_grad).  The cubes of a recorded
+// allocation are sorted by sample count, largest first, stably (ties in cube
+// order).  The sort key is a size class: n itself up to 128, then 16 classes
+// per octave (sizes within a class differ by at most 1/16 of an octave), 255
+// from 31,744 samples on; one 8-bit radix pass sorts by it.  Cube c of the
+// sorted order gets L_c lanes, a power of two (1 up to VG_GRAD_T samples, about
+// n / VG_GRAD_T beyond, at most 64) that depends only on the class, so L does
+// not increase along the order: every cube's lanes start at a multiple of its
+// L, and no cube's lanes straddle a wave of 64.  lane_start is the exclusive
+// prefix sum of L over the order (lane_start[C] = all lanes); wave_first[w] is
+// the first cube of wave w (64 w is always a cube's first lane), and
+// wave_first[waves] = C.
+// ---------------------------------------------------------------------------
+#define VG_GRAD_T 16
+
+__device__ inline unsigned vg_size_class(unsigned n) {
+    if (n <= 128u) return n;
+    const unsigned o = 31u - (unsigned)__clz((int)n);  // n > 128: o >= 7
+    if (o > 14u) return 255u;
+    return 128u + 16u * (o - 7u) + ((n >> (o - 4u)) & 15u);
+}
+
+// Lanes of the cubes of class k: L = 2^ceil(log2(ceil(n_lo / VG_GRAD_T))), at
+// most 64, for n_lo the smallest size of the class.
+__device__ inline int vg_class_lanes(unsigned k) {
+    unsigned nlo = k;
+    if (k > 128u) {
+        const unsigned o = 7u + (k - 128u) / 16u, f = (k - 128u) % 16u;
+        nlo = (16u + f) << (o - 4u);
+    }
+    const unsigned q = (nlo + VG_GRAD_T - 1) / VG_GRAD_T;
+    unsigned L = 1;
+    while (L < q && L < 64u) L <<= 1;
+    return (int)L;
+}
+
+__global__ void __launch_bounds__(256)
+vg_grad_keys_kernel(int C, const int* __restrict__ nh, unsigned char* __restrict__ key, int* __restrict__ idx) {
+    for (long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x; h < C;
+         h += (long long)gridDim.x * blockDim.x) {
+        key[h] = (unsigned char)vg_size_class((unsigned)nh[h]);
+        idx[h] = (int)h;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+vg_grad_lanes_kernel(int C, const unsigned char* __restrict__ key_sorted, int* __restrict__ lanes) {
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c <= C;
+         c += (long long)gridDim.x * blockDim.x)
+        lanes[c] = c < C ? vg_class_lanes(key_sorted[c]) : 0;
+}
+
+__global__ void __launch_bounds__(256)
+vg_grad_waves_kernel(int C, const int* __restrict__ lane_start, int* __restrict__ wave_first) {
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < C;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int s = lane_start[c];
+        if ((s & 63) == 0) wave_first[s >> 6] = (int)c;
+        if (c == C - 1) wave_first[(lane_start[C] + 63) >> 6] = C;
+    }
+}
+
+size_t vg_grad_order_tmp_bytes(int C) {
+    size_t sort_bytes = 0, scan_bytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, sort_bytes, (const unsigned char*)nullptr,
+                                                     (unsigned char*)nullptr, (const int*)nullptr,
+                                                     (int*)nullptr, C, 0, 8) != hipSuccess)
+        return 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int*)nullptr, (int*)nullptr, C + 1) !=
+        hipSuccess)
+        return 0;
+    return sort_bytes > scan_bytes ? sort_bytes : scan_bytes;
+}
+
+hipError_t vg_launch_grad_order(hipStream_t st, int C, const int* nh, unsigned char* keys,
+                                unsigned char* keys_sorted, int* idx, int* order, int* lanes, int* lane_start,
+                                int* wave_first, void* tmp, size_t tmp_bytes) {
+    const long long blocks = ((long long)C + 256) / 256;
+    const unsigned grid = (unsigned)(blocks < 4096 ? blocks : 4096);
+    hipLaunchKernelGGL(vg_grad_keys_kernel, dim3(grid), dim3(256), 0, st, C, nh, keys, idx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t bytes = tmp_bytes;
+    e = hipcub::DeviceRadixSort::SortPairsDescending(tmp, bytes, (const unsigned char*)keys, keys_sorted,
+                                                     (const int*)idx, order, C, 0, 8, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(vg_grad_lanes_kernel, dim3(grid), dim3(256), 0, st, C, keys_sorted, lanes);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    bytes = tmp_bytes;
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, (const int*)lanes, lane_start, C + 1, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(vg_grad_waves_kernel, dim3(grid), dim3(256), 0, st, C, lane_start, wave_first);
+    return hipGetLastError();
+}
+
+long long vg_grad_max_waves(long long C, long long S) {
+    // L = 1 for cubes of at most VG_GRAD_T samples, below 4 n / VG_GRAD_T beyond
+    const long long lanes = C + (4 * S + VG_GRAD_T - 1) / VG_GRAD_T;
+    return (lanes + 63) / 64;
 }
