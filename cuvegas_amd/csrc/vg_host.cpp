@@ -224,7 +224,7 @@ struct cvg_state {
     unsigned char* bw_keys = nullptr;  // [2 C] size classes, then sorted
     int* bw_idx = nullptr;             // cube indices, replay order, lanes, lane starts, wave starts
     void* bw_sort_tmp = nullptr;       // radix-sort workspace
-    hipEvent_t bw_ev0 = nullptr, bw_ev1 = nullptr;
+    std::vector<hipEvent_t> bw_evs;  // 2 per recordable iteration: around its replay kernels
     double bw_replay_ms = 0.0;  // device time of the last backward's replay kernels
     std::vector<char> recorded;  // per iteration: snapshot taken
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -240,8 +240,8 @@ struct cvg_state {
         if (graph) (void)hipGraphDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
-        if (bw_ev0) (void)hipEventDestroy(bw_ev0);
-        if (bw_ev1) (void)hipEventDestroy(bw_ev1);
+        for (hipEvent_t e : bw_evs)
+            if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         void* bufs[] = {x_edges, map64, map32, weights, counts, refine_scratch, nh, nh_sum, dh,
                         JS, run_cube, win_first, win_last, part, bsum, boff, sync, s_total,
@@ -382,7 +382,8 @@ int vg_backward_layout(const cvg_state* st, vg_bw_layout* out) {
     const long long spu = (out->waves + kGradMaxUnits - 1) / kGradMaxUnits;
     out->slots_per_unit = (int)(spu > 0 ? spu : 1);
     out->units = (int)((out->waves + out->slots_per_unit - 1) / out->slots_per_unit);
-    out->part = ((long long)out->units + 1) * (2LL + 2LL * P);
+    const long long slots = st->p.max_it - st->p.skip;
+    out->part = ((long long)out->units + (slots > 0 ? slots : 1)) * (2LL + 2LL * P);
     out->theta = P;
     out->nh_sum = st->C + 1;
     out->x = (long long)d * (n + 1);
@@ -394,8 +395,7 @@ int vg_backward_layout(const cvg_state* st, vg_bw_layout* out) {
 }
 
 int vg_backward_run(cvg_state* st, const double* theta, const vg_bw_buffers& b, const vg_bw_layout& L,
-                    hipEvent_t ev0, hipEvent_t ev1, double* grad, double* grad_var, double* value,
-                    double* value_var) {
+                    const hipEvent_t* evs, double* grad, double* grad_var, double* value, double* value_var) {
     cvg_program* prog = st->prog;
     const int P = st->p.n_theta, d = st->p.n_dim, n = st->p.n_intervals;
     const int NV = 2 + 2 * P;
@@ -417,7 +417,7 @@ int vg_backward_run(cvg_state* st, const double* theta, const vg_bw_buffers& b, 
     std::vector<double> hist((size_t)VG_HIST * st->iterations_done);
     VG_HIP(hipMemcpyAsync(hist.data(), st->hist, sizeof(double) * hist.size(), hipMemcpyDeviceToHost, st->stream));
     const long long nx = (long long)d * (n + 1);
-    std::vector<double> tot((size_t)NV);
+    std::vector<double> tot((size_t)NV * its.size());
     double W = 0.0, val = 0.0, vvar = 0.0;
     std::vector<double> g((size_t)P, 0.0), gv((size_t)P, 0.0);
     int C = (int)st->C;
@@ -425,8 +425,10 @@ int vg_backward_run(cvg_state* st, const double* theta, const vg_bw_buffers& b, 
     if (st->p.has_domain)
         for (int i = 0; i < d; ++i) vol *= st->domain_host[d + i];
     unsigned seed = st->p.seed, sid = st->p.stream_id;
-    double replay_ms = 0.0;
-    for (int k : its) {
+    const int nu = L.units;
+    double* const totals = b.part + (long long)nu * NV;  // NV per replayed iteration
+    for (size_t idx = 0; idx < its.size(); ++idx) {
+        const int k = its[idx];
         const long long slot = k - st->p.skip;
         VG_HIP(hipMemcpyAsync(b.x, st->snap_x + slot * nx, sizeof(double) * nx, hipMemcpyDeviceToDevice, st->stream));
         VG_HIP(vg_launch_tables(st->stream, d, n, b.x, b.map64, b.map32));
@@ -439,37 +441,41 @@ int vg_backward_run(cvg_state* st, const double* theta, const vg_bw_buffers& b, 
         const void* map = st->p.precision == CVG_FP32C ? (const void*)b.map32 : (const void*)b.map64;
         unsigned itu = (unsigned)k;
         double V = st->V;
-        int spu = L.slots_per_unit, nu = L.units;
+        int spu = L.slots_per_unit, nuk = nu;
         double* part = b.part;
         long long* nhs = b.nh_sum;
         int *order = b.order, *lane_start = b.lane_start, *wave_first = b.wave_first;
         void* args[] = {(void*)&map, (void*)&snh, (void*)&nhs, (void*)&order, (void*)&lane_start,
                         (void*)&wave_first, (void*)&C, (void*)&spu, (void*)&itu, (void*)&V, (void*)&st->domain,
                         (void*)&vol, (void*)&th, (void*)&seed, (void*)&sid, (void*)&part};
-        if (ev0) VG_HIP(hipEventRecord(ev0, st->stream));
+        if (evs) VG_HIP(hipEventRecord(evs[2 * idx], st->stream));
         VG_HIP(hipModuleLaunchKernel(prog->grad, (unsigned)nu, 1, 1, 64, 1, 1, 0, st->stream, args, nullptr));
-        double* out = part + (long long)nu * NV;
-        void* rargs[] = {(void*)&part, (void*)&nu, (void*)&out};
+        double* out = totals + (long long)idx * NV;
+        void* rargs[] = {(void*)&part, (void*)&nuk, (void*)&out};
         VG_HIP(hipModuleLaunchKernel(prog->grad_reduce, (unsigned)NV, 1, 1, 1024, 1, 1, 0, st->stream, rargs,
                                      nullptr));
-        if (ev1) VG_HIP(hipEventRecord(ev1, st->stream));
-        VG_HIP(hipMemcpyAsync(tot.data(), out, sizeof(double) * NV, hipMemcpyDeviceToHost, st->stream));
-        VG_HIP(hipStreamSynchronize(st->stream));
-        if (ev0 && ev1) {
+        if (evs) VG_HIP(hipEventRecord(evs[2 * idx + 1], st->stream));
+    }
+    VG_HIP(hipMemcpyAsync(tot.data(), totals, sizeof(double) * tot.size(), hipMemcpyDeviceToHost, st->stream));
+    VG_HIP(hipStreamSynchronize(st->stream));
+    double replay_ms = 0.0;
+    for (size_t idx = 0; idx < its.size(); ++idx) {
+        if (evs) {
             float kms = 0.0f;
-            VG_HIP(hipEventElapsedTime(&kms, ev0, ev1));
+            VG_HIP(hipEventElapsedTime(&kms, evs[2 * idx], evs[2 * idx + 1]));
             replay_ms += kms;
         }
-        const double w = 1.0 / hist[(size_t)VG_HIST * k + 1];  // forward weight, held fixed
+        const double* t = tot.data() + idx * NV;
+        const double w = 1.0 / hist[(size_t)VG_HIST * its[idx] + 1];  // forward weight, held fixed
         W += w;
-        val += w * tot[0];
-        vvar += w * w * tot[1];
+        val += w * t[0];
+        vvar += w * w * t[1];
         for (int q = 0; q < P; ++q) {
-            g[q] += w * tot[2 + q];
-            gv[q] += w * w * tot[2 + P + q];
+            g[q] += w * t[2 + q];
+            gv[q] += w * w * t[2 + P + q];
         }
     }
-    if (ev0 && ev1) st->bw_replay_ms = replay_ms;
+    if (evs) st->bw_replay_ms = replay_ms;
     if (value) *value = val / W;
     if (value_var) *value_var = vvar / (W * W);
     for (int q = 0; q < P; ++q) {
@@ -1207,8 +1213,8 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
         if ((rc = dalloc(&st->bw_keys, 2 * L.cubes))) return rc;
         if ((rc = dalloc(&st->bw_idx, 4 * L.cubes + 2 + L.waves + 1))) return rc;
         if ((rc = dalloc((char**)&st->bw_sort_tmp, L.sort_tmp))) return rc;
-        VG_HIP(hipEventCreate(&st->bw_ev0));
-        VG_HIP(hipEventCreate(&st->bw_ev1));
+        st->bw_evs.assign(2 * (size_t)(st->p.max_it - st->p.skip), nullptr);
+        for (hipEvent_t& e : st->bw_evs) VG_HIP(hipEventCreate(&e));
     }
     vg_bw_buffers b;
     b.part = st->bw_part;
@@ -1228,7 +1234,7 @@ int cvg_state_backward(cvg_state* st, const double* theta, double* grad, double*
     b.lane_start = st->bw_idx + 3 * L.cubes + 1;
     b.wave_first = st->bw_idx + 4 * L.cubes + 2;
     b.sort_tmp = st->bw_sort_tmp;
-    return vg_backward_run(st, theta, b, L, st->bw_ev0, st->bw_ev1, grad, grad_var, value, value_var);
+    return vg_backward_run(st, theta, b, L, st->bw_evs.data(), grad, grad_var, value, value_var);
 }
 
 int cvg_state_backward_timing(cvg_state* st, double* replay_ms) {

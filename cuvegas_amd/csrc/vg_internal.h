@@ -9,7 +9,8 @@ struct cvg_state;
 // counters, which the scan kernels clear themselves), so none of them has to
 // start at zero.
 struct vg_bw_buffers {
-    double* part = nullptr;        // [units + 1][2 + 2 n_theta]: unit partials, then the totals
+    double* part = nullptr;        // [units + max_it - skip][2 + 2 n_theta]: unit partials, then the
+                                   // totals of each recorded iteration
     void* theta = nullptr;         // replay parameters (n_theta values of the working precision)
     long long* nh_sum = nullptr;   // [C + 1] prefix sums of a recorded allocation
     long long* s_total = nullptr;  // [1] samples of that allocation
@@ -42,12 +43,13 @@ int vg_backward_layout(const cvg_state* st, vg_bw_layout* out);
 int vg_state_device(const cvg_state* st);
 
 // The replay of the recorded iterations through the buffers b (laid out by
-// vg_backward_layout): value, gradient and their variances.  ev0 / ev1, when
-// not null, time the replay kernels of each iteration (summed into the
-// state's last-backward time).
+// vg_backward_layout): value, gradient and their variances.  Every recorded
+// iteration is enqueued on the state's stream, then one copy brings all their
+// totals back.  evs, when not null, holds 2 (max_it - skip) events that time
+// the replay kernels of each iteration (summed into the state's
+// last-backward time).
 int vg_backward_run(cvg_state* st, const double* theta, const vg_bw_buffers& b, const vg_bw_layout& L,
-                    hipEvent_t ev0, hipEvent_t ev1, double* grad, double* grad_var, double* value,
-                    double* value_var);
+                    const hipEvent_t* evs, double* grad, double* grad_var, double* value, double* value_var);
 
 // Sets the calling thread's last-error message (cvg_last_error) and returns code.
 int vg_fail(int code, const char* msg);

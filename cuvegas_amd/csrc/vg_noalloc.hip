@@ -117,5 +117,5 @@ extern "C" int cvg_state_backward_noalloc(cvg_state* st, const double* theta, do
     b.wave_first = reinterpret_cast<int*>(p);
     p += sizes[15];
     b.sort_tmp = p;
-    return vg_backward_run(st, theta, b, L, nullptr, nullptr, grad, grad_var, value, value_var);
+    return vg_backward_run(st, theta, b, L, nullptr, grad, grad_var, value, value_var);
 }
