@@ -128,6 +128,17 @@ def cpu_baseline(itg, family, nbins, n_strat, n_eval_it, seed):
             "omp_num_threads": budget or None}
 
 
+def _device_cu_clock():
+    """(compute units, clock in Hz) of the current device."""
+    import ctypes as C
+    from cuvegas_amd import lib
+    n_cu, clk = C.c_int(0), C.c_int(0)
+    arch = C.create_string_buffer(64)
+    if lib.cvg_device_info(C.byref(n_cu), C.byref(clk), arch, 64) != 0:
+        sys.exit("no HIP device: " + lib.cvg_last_error().decode())
+    return n_cu.value, clk.value * 1e3
+
+
 def _workloads():
     with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
         return json.load(f)
@@ -262,6 +273,14 @@ def mix6_bench(args, neval=None):
         bw_dev = st.backward_timing()
         replayed = int(ne[skip:].sum())
         pull = (g - exact_grad) / np.sqrt(gv)
+        # roofline of the replay kernel (vg_grad and its unit reduction): the
+        # replayed samples' algorithmic slots over the kernels' device time
+        from cuvegas_amd.roofline import slots_per_replayed_sample, valu_peak
+        s_grad = slots_per_replayed_sample(d, "gaussian_mixture", int(theta.size))
+        peak = valu_peak(*_device_cu_clock())
+        grad_roof = {"bound": "valu", "kernel": "vg_grad", "slots_per_sample": s_grad,
+                     "achieved": s_grad * replayed / (bw_dev * 1e-3) / 1e12, "peak": peak / 1e12,
+                     "unit": "Tslot/s", "frac": s_grad * replayed / (bw_dev * 1e-3) / peak}
         forms[prec] = {
             "precision": prec, "n_strat": st.n_strat, "n_cubes": st.n_cubes,
             "forward_evals_per_s": float(ne.sum() / (iter_ms.sum() * 1e-3)),
@@ -274,6 +293,7 @@ def mix6_bench(args, neval=None):
             "grad_pull_rms": float(np.sqrt(np.mean(pull ** 2))),
             "grad_pull_max": float(np.max(np.abs(pull))), "n_theta": int(theta.size),
             "grad_within_3sigma": bool(np.all(np.abs(pull) < 3)),
+            "backward_roofline": grad_roof,
         }
         _log(prec, json.dumps(forms[prec]))
         st.close()

@@ -15,6 +15,17 @@ transcendental 4 slots plus 1 for range scaling; an LDS atomic 1 slot):
 with S_f = 2d+7 (Gaussian peak), 4d+14 (double Gaussian), d+6 (oscillatory),
 d+9 (corner peak), K(2d+7) (K-component Gaussian mixture).
 
+The backward replay of a parametrised integrand (d I / d theta, P
+parameters) redraws each recorded sample and forms J f and the P terms
+J df/dtheta_p: per replayed sample
+
+  S_grad = S - 2 d     the forward's sample work without the 2 d bin updates
+         + S_g         the integrand's gradient terms
+         + 3 P         J df_p, and its sum and sum of squares
+
+with S_g = K(5 + 2d) for the K-component Gaussian mixture (per component:
+the weight term 1, the width term 4, 2 per centre coordinate).
+
 Peak = CUs x 64 lanes x clock (lane-slots per second).
 """
 from __future__ import annotations
@@ -34,6 +45,17 @@ def slots_per_eval(d: int, family: str, components: int = 4) -> int:
     """VALU lane-slots per integrand evaluation (the rules above)."""
     s_f = INTEGRAND_SLOTS[family](d, components)
     return 60 * math.ceil(d / 4) + 16 * d + s_f + 10
+
+
+GRADIENT_SLOTS = {
+    "gaussian_mixture": lambda d, k=4: k * (5 + 2 * d),
+}
+
+
+def slots_per_replayed_sample(d: int, family: str, n_theta: int, components: int = 4) -> int:
+    """VALU lane-slots per sample of the backward replay (the rules above)."""
+    return (slots_per_eval(d, family, components) - 2 * d + GRADIENT_SLOTS[family](d, components)
+            + 3 * n_theta)
 
 
 def valu_peak(n_cu: int, clock_hz: float) -> float:

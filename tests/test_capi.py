@@ -179,3 +179,18 @@ def test_batch_workload_fixture_matches_its_draw():
         assert d == e["d"] and N == e["neval"] and 2 <= d <= 12 and 10_000 <= N <= 10_000_000
         assert theta[off] == {"gaussian": 0.0, "oscillatory": 1.0, "corner": 2.0}[fam]
     assert np.all(np.isfinite(exact))
+
+
+def test_roofline_slot_counts():
+    """The algorithmic work per evaluation of the benchmark configurations
+    (149, 304, 528 and 302 VALU lane-slots for the 4-D Gaussian, the 8-D
+    double Gaussian, the 16-D oscillatory and the 6-D mixture) and per
+    replayed sample of the mixture's 32-parameter backward (302 - 12 + 68 +
+    96 = 454), and the MI355X issue peak (256 CUs x 64 lanes x 2.4 GHz)."""
+    from cuvegas_amd.roofline import slots_per_eval, slots_per_replayed_sample, valu_peak
+    assert slots_per_eval(4, "gaussian_peak") == 149
+    assert slots_per_eval(8, "double_gaussian") == 304
+    assert slots_per_eval(16, "oscillatory") == 528
+    assert slots_per_eval(6, "gaussian_mixture") == 302
+    assert slots_per_replayed_sample(6, "gaussian_mixture", 32) == 454
+    assert valu_peak(256, 2.4e9) == pytest.approx(3.93216e13)
