@@ -20,7 +20,8 @@
 // Compile-time shape (hipRTC -D options): VG_NDIM, VG_NBINS, VG_NSTRAT, VG_K,
 // VG_BLOCK, VG_FP32 (0: FP64 sampling and evaluation; 1: FP32 sampling and
 // evaluation with FP64 accumulation), VG_LDS_BINS, VG_BIN_DIMS (dimensions
-// whose bins one pass keeps in LDS), VG_LDS_MAP, VG_HAS_DOMAIN,
+// whose bins one pass keeps in LDS), VG_BIN_COPIES and VG_COPY_DIM0 (LDS bin
+// copies of the dimensions from VG_COPY_DIM0 on), VG_LDS_MAP, VG_HAS_DOMAIN,
 // VG_PAIR (two samples per lane step), VG_PIPE (one sample per step, the next
 // sample's map entries requested before the current one is evaluated).
 #include "vg_philox.h"
@@ -31,6 +32,12 @@
 #endif
 #ifndef VG_PIPE
 #define VG_PIPE 0
+#endif
+#ifndef VG_BIN_COPIES
+#define VG_BIN_COPIES 1
+#endif
+#ifndef VG_COPY_DIM0
+#define VG_COPY_DIM0 0
 #endif
 
 #if VG_FP32
@@ -251,9 +258,21 @@ vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
     // write_sums set stores the per-cube sums.
 #if VG_LDS_BINS
     constexpr int BD = VG_BIN_DIMS;
-    __shared__ vg_bin_t lw[BD * NB];
-    __shared__ unsigned lc[BD * NB];
-    for (int i = threadIdx.x; i < BD * NB; i += VG_BLOCK) { lw[i] = (vg_bin_t)0; lc[i] = 0u; }
+    // Dimensions i >= VG_COPY_DIM0 (all bins in one pass only) keep
+    // VG_BIN_COPIES copies of their bins, lane l adding into copy l mod
+    // VG_BIN_COPIES: in the dimensions whose stratum digit a window's cubes
+    // share, all lanes add into the few bins of one stratum, and the copies
+    // divide those same-address collisions.  Copy 0 is lw / lc itself; the
+    // others follow after them, copy cp at BD NB + (cp - 1) XS (XS: the copy
+    // dimensions' bins plus an odd pad, so that one bin's copies fall in
+    // different LDS banks), and are added to copy 0 at the flush.
+    constexpr int CP = (BD == D && VG_BIN_COPIES > 1 && VG_COPY_DIM0 < D) ? VG_BIN_COPIES : 1;
+    constexpr int CD0 = CP > 1 ? VG_COPY_DIM0 : D;
+    constexpr int XS = (D - CD0) * NB + 17;
+    constexpr int NXB = (CP - 1) * XS;
+    __shared__ vg_bin_t lw[BD * NB + NXB];
+    __shared__ unsigned lc[BD * NB + NXB];
+    for (int i = threadIdx.x; i < BD * NB + NXB; i += VG_BLOCK) { lw[i] = (vg_bin_t)0; lc[i] = 0u; }
 #endif
 #if VG_LDS_MAP
     __shared__ vg_map_t lmap[D * NB];
@@ -308,7 +327,13 @@ vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
             for (int i = 0; i < D; ++i) {
 #if VG_LDS_BINS
                 const int li = BD == D ? i : i - bin_lo;  // this pass's slot of dimension i
-                if (BD == D || (unsigned)li < (unsigned)BD) {
+                if (CP > 1 && i >= CD0) {
+                    const int cp = lane % CP;
+                    const int a = cp == 0 ? li * NB + (int)o.ids[i]
+                                          : BD * NB + (cp - 1) * XS + (i - CD0) * NB + (int)o.ids[i];
+                    atomicAdd(&lw[a], (vg_bin_t)o.jf2);
+                    atomicAdd(&lc[a], 1u);
+                } else if (BD == D || (unsigned)li < (unsigned)BD) {
                     atomicAdd(&lw[li * NB + o.ids[i]], (vg_bin_t)o.jf2);
                     atomicAdd(&lc[li * NB + o.ids[i]], 1u);
                 }
@@ -419,9 +444,18 @@ vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
     __syncthreads();
     const int nb_pass = (D - bin_lo < BD ? D - bin_lo : BD) * NB;
     for (int i = threadIdx.x; i < nb_pass; i += VG_BLOCK) {
-        const unsigned c = lc[i];
+        unsigned c = lc[i];
+        vg_bin_t wsum = lw[i];
+        if (CP > 1 && i >= CD0 * NB) {  // the other copies of this bin, in copy order
+#pragma unroll
+            for (int cp = 1; cp < CP; ++cp) {
+                const int a = BD * NB + (cp - 1) * XS + (i - CD0 * NB);
+                c += lc[a];
+                wsum = wsum + lw[a];
+            }
+        }
         if (c) {
-            unsafeAtomicAdd(&weights[bin_lo * NB + i], (double)lw[i]);
+            unsafeAtomicAdd(&weights[bin_lo * NB + i], (double)wsum);
             atomicAdd(&counts[bin_lo * NB + i], c);
         }
     }

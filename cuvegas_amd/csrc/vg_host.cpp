@@ -91,7 +91,7 @@ int validate(const cvg_params* p) {
 // ---------------------------------------------------------------------------
 struct cvg_program {
     int d = 0, nbins = 0, nstrat = 0, precision = 0, K = kDefaultRun, has_domain = 0;
-    int block = kFillBlock, lds_bins = 0, lds_map = 0, bin_dims = 1;
+    int block = kFillBlock, lds_bins = 0, lds_map = 0, bin_dims = 1, bin_copies = 1, copy_dim0 = 0;
     int has_grad = 0, n_theta = 0, bin32 = 0, pair = 1, pipe = 0;
     int batch = 0, max_dim = 0, task_w = 8, fill_grid = 1;
     std::string log;
@@ -563,7 +563,7 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
     // read first and applied in a fixed order (bin32, then lds_bins, then
     // lds_map), so the result does not depend on the order of the tokens.
     std::string rtc_extra;
-    int opt_lds_bins = -1, opt_lds_map = -1, opt_bin32 = -1;
+    int opt_lds_bins = -1, opt_lds_map = -1, opt_bin32 = -1, opt_copies = -1, opt_copy_dim0 = -1;
     if (options) {
         std::istringstream ss(options);
         std::string tok;
@@ -578,6 +578,12 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
             } else if (key == "bin32") {
                 if (val && prog->precision != CVG_FP32C) return fail(CVG_ERR_INVALID, "bin32 is for the FP32 form");
                 opt_bin32 = val ? 1 : 0;
+            } else if (key == "bin_copies") {
+                if (val < 1 || val > 8) return fail(CVG_ERR_INVALID, "bin_copies must be in 1..8");
+                opt_copies = val;
+            } else if (key == "copy_dim0") {
+                if (val < 0 || val >= prog->d) return fail(CVG_ERR_INVALID, "copy_dim0 must be in 0..n_dim-1");
+                opt_copy_dim0 = val;
             } else if (key == "grad") {
                 if (shape->n_theta < 1) return fail(CVG_ERR_INVALID, "grad needs n_theta >= 1");
                 prog->has_grad = val ? 1 : 0;
@@ -613,6 +619,40 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
             return fail(CVG_ERR_INVALID, "map does not fit in LDS next to the bins");
         prog->lds_map = opt_lds_map;
     }
+    // Copies of the LDS bins in the dimensions whose stratum digit a window's
+    // cubes share (from the first dimension i with n_strat^i >= 256: a window
+    // of 64 K samples holds fewer cubes than that digit's period once the
+    // allocation has adapted) when a stratum spans few bins (n / n_strat <= 32):
+    // the lanes of a wave then add into the few bins of one stratum, and lane l
+    // adds into copy l mod copies.  As many copies (8, 4 or 2) as fit in LDS.
+    // 4-D Gaussian, 1024 bins, 70 strata: dimensions 2 and 3, 4 copies.
+    auto copies_lds = [&](int copies, int dim0) {
+        return (long long)(copies - 1) * ((long long)(prog->d - dim0) * prog->nbins + 17) * (prog->bin32 ? 8 : 12);
+    };
+    const long long lds_used = bin_bytes + (prog->lds_map ? map_bytes : 0);
+    const bool copies_ok = prog->lds_bins && prog->bin_dims == prog->d;
+    prog->bin_copies = 1;
+    prog->copy_dim0 = prog->d;
+    if (copies_ok && prog->nbins <= 32LL * prog->nstrat) {
+        long long period = 1;
+        int i0 = 0;
+        while (i0 < prog->d && period < 256) { period *= prog->nstrat; ++i0; }
+        for (int cp : {8, 4, 2}) {
+            if (i0 < prog->d && lds_used + copies_lds(cp, i0) <= kLdsBytes) {
+                prog->bin_copies = cp;
+                prog->copy_dim0 = i0;
+                break;
+            }
+        }
+    }
+    if (opt_copies >= 0 || opt_copy_dim0 >= 0) {
+        const int cp = opt_copies >= 0 ? opt_copies : (prog->bin_copies > 1 ? prog->bin_copies : 1);
+        const int i0 = opt_copy_dim0 >= 0 ? opt_copy_dim0 : (prog->copy_dim0 < prog->d ? prog->copy_dim0 : prog->d - 1);
+        if (cp > 1 && (!copies_ok || lds_used + copies_lds(cp, i0) > kLdsBytes))
+            return fail(CVG_ERR_INVALID, "bin copies need all bins in LDS and room for the copies");
+        prog->bin_copies = cp;
+        prog->copy_dim0 = cp > 1 ? i0 : prog->d;
+    }
     options = rtc_extra.c_str();
     const double t0 = now_ms();
     hiprtcProgram rtc;
@@ -629,6 +669,8 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
         "-DVG_FP32=" + std::to_string(prog->precision == CVG_FP32C ? 1 : 0),
         "-DVG_LDS_BINS=" + std::to_string(prog->lds_bins),
         "-DVG_BIN_DIMS=" + std::to_string(prog->bin_dims),
+        "-DVG_BIN_COPIES=" + std::to_string(prog->bin_copies),
+        "-DVG_COPY_DIM0=" + std::to_string(prog->copy_dim0),
         "-DVG_LDS_MAP=" + std::to_string(prog->lds_map),
         "-DVG_HAS_DOMAIN=" + std::to_string(prog->has_domain),
         "-DVG_GRAD=" + std::to_string(prog->has_grad),

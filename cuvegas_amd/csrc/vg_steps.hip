@@ -78,6 +78,43 @@ __device__ inline double vg_tree1024_256(double v0, double v1, double v2, double
     return x;
 }
 
+// Q trees of vg_tree1024_256 at once (value q of thread t: v[0..3][q] at
+// elements t, t + 256, t + 512, t + 768), with three block barriers instead
+// of 3 Q; each quantity's additions are those of vg_tree1024_256, so the
+// totals are bit-identical.  red: Q * 256 doubles of LDS; tot[q] is valid in
+// thread 0.
+template <int Q>
+__device__ inline void vg_tree1024_256q(const double (&v)[4][Q], double* red, double (&tot)[Q]) {
+    double x[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const double a = v[0][q] + v[2][q];  // stride 512: i = t
+        const double b = v[1][q] + v[3][q];  // stride 512: i = t + 256
+        x[q] = a + b;                        // stride 256: i = t
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) red[q * 256 + threadIdx.x] = x[q];
+    __syncthreads();
+    if (threadIdx.x < 128) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) red[q * 256 + threadIdx.x] = red[q * 256 + threadIdx.x] + red[q * 256 + threadIdx.x + 128];
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            double y = red[q * 256 + threadIdx.x] + red[q * 256 + threadIdx.x + 64];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const double z = __shfl_down(y, off, 64);
+                y = y + z;  // lanes >= off compute values that are never read
+            }
+            tot[q] = y;
+        }
+    }
+}
+
 // True in every thread of the block that arrives last at *cnt among the
 // gridDim.x blocks of this launch, with the other blocks' global writes
 // visible to it (agent-scope release before the count, acquire after: on
@@ -128,7 +165,7 @@ __device__ inline void vg_cube_result_sums(long long h, int C, double V, double 
 
 // Results of the 1024 cubes of block b: d_h, and the block partials of
 // (sum I_h, sum sigma_h^2 / n_h, sum d_h, sum sigma_h^2 / (n_h - 1)) in
-// part[VG_NRES b + q].  256 threads.
+// part[VG_NRES b + q].  256 threads; red: VG_NRES * 256 doubles of LDS.
 __device__ inline void vg_results_block(long long b, int C, double V, double beta,
                                         const int* __restrict__ nh, const double2* __restrict__ JS,
                                         double* __restrict__ dh, double* __restrict__ part, double* red) {
@@ -137,17 +174,18 @@ __device__ inline void vg_results_block(long long b, int C, double V, double bet
 #pragma unroll
     for (int j = 0; j < 4; ++j)
         vg_cube_result_sums(base + 256 * j, C, V, beta, nh, JS, dh, r[j][0], r[j][1], r[j][2], r[j][3]);
+    double tot[VG_NRES];
+    vg_tree1024_256q<VG_NRES>(r, red, tot);
+    if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < VG_NRES; ++q) {
-        const double tot = vg_tree1024_256(r[0][q], r[1][q], r[2][q], r[3][q], red);
-        if (threadIdx.x == 0) part[VG_NRES * b + q] = tot;
+        for (int q = 0; q < VG_NRES; ++q) part[VG_NRES * b + q] = tot[q];
     }
 }
 
 __global__ void __launch_bounds__(256)
 vg_results_kernel(int C, double V, double beta, const int* __restrict__ nh,
                   const double2* __restrict__ JS, double* __restrict__ dh, double* __restrict__ part) {
-    __shared__ double red[256];
+    __shared__ double red[VG_NRES * 256];
     vg_results_block(blockIdx.x, C, V, beta, nh, JS, dh, part, red);
 }
 
@@ -255,7 +293,7 @@ vg_finalize_kernel(long long nparts, const double* __restrict__ part, double* __
                    unsigned* __restrict__ sync, const long long* __restrict__ s_total,
                    const int* __restrict__ it_p, int hist_cap, double* __restrict__ hist,
                    double* __restrict__ dsum, int* __restrict__ err) {
-    __shared__ double red[256];
+    __shared__ double red[VG_NRES * 256];
     __shared__ int last;
     {
         double v[4][VG_NRES];
@@ -265,10 +303,11 @@ vg_finalize_kernel(long long nparts, const double* __restrict__ part, double* __
 #pragma unroll
             for (int q = 0; q < VG_NRES; ++q) v[j][q] = i < nparts ? part[VG_NRES * i + q] : 0.0;
         }
+        double tot[VG_NRES];
+        vg_tree1024_256q<VG_NRES>(v, red, tot);
+        if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < VG_NRES; ++q) {
-            const double tot = vg_tree1024_256(v[0][q], v[1][q], v[2][q], v[3][q], red);
-            if (threadIdx.x == 0) part2[VG_NRES * blockIdx.x + q] = tot;
+            for (int q = 0; q < VG_NRES; ++q) part2[VG_NRES * blockIdx.x + q] = tot[q];
         }
     }
     if (!vg_last_block(sync, &last)) return;
@@ -284,10 +323,11 @@ vg_finalize_kernel(long long nparts, const double* __restrict__ part, double* __
                 for (int q = 0; q < VG_NRES; ++q) v[j][q] = i < cur ? part2[VG_NRES * i + q] : 0.0;
             }
             __syncthreads();  // every read of this chunk precedes the writes below
+            double tot[VG_NRES];
+            vg_tree1024_256q<VG_NRES>(v, red, tot);
+            if (threadIdx.x == 0) {
 #pragma unroll
-            for (int q = 0; q < VG_NRES; ++q) {
-                const double tot = vg_tree1024_256(v[0][q], v[1][q], v[2][q], v[3][q], red);
-                if (threadIdx.x == 0) part2[VG_NRES * c + q] = tot;
+                for (int q = 0; q < VG_NRES; ++q) part2[VG_NRES * c + q] = tot[q];
             }
             __syncthreads();
         }
@@ -708,10 +748,15 @@ vg_results_refine_kernel(int C, double V, double beta, const int* __restrict__ n
                            map32 + (long long)dim * n, red);
         return;
     }
-    vg_results_block(blockIdx.x - rd, C, V, beta, nh, JS, dh, part, red);
+    // the per-cube results blocks take their tree buffer from the dynamic LDS
+    // (sized for both uses)
+    vg_results_block(blockIdx.x - rd, C, V, beta, nh, JS, dh, part, dyn);
 }
 
-long long vg_results_refine_lds(int n) { return 8LL * (2LL * n + n / VG_TREE + 2); }
+long long vg_results_refine_lds(int n) {
+    const long long refine = 8LL * (2LL * n + n / VG_TREE + 2), tree = 8LL * VG_NRES * 256;
+    return refine > tree ? refine : tree;
+}
 
 long long vg_refine_scratch_doubles(int d, int n) {
     return (long long)d * (4LL * n + 1 + n / VG_TREE + 2);
@@ -920,10 +965,11 @@ vg_bresults_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ b
         }
         r[j][0] = sr; r[j][1] = ss; r[j][2] = dd; r[j][3] = su;
     }
+    double tot[VG_NRES];
+    vg_tree1024_256q<VG_NRES>(r, dyn, tot);
+    if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < VG_NRES; ++q) {
-        const double tot = vg_tree1024_256(r[0][q], r[1][q], r[2][q], r[3][q], red);
-        if (threadIdx.x == 0) part[VG_NRES * bb + q] = tot;
+        for (int q = 0; q < VG_NRES; ++q) part[VG_NRES * bb + q] = tot[q];
     }
 }
 
@@ -933,7 +979,7 @@ __global__ void __launch_bounds__(256)
 vg_bfinalize_kernel(const vg_bitem* __restrict__ items, double* __restrict__ part_all,
                     const long long* __restrict__ s_item, const int* __restrict__ it_p, int hist_cap,
                     double* __restrict__ hist, double* __restrict__ dsum, int* __restrict__ err) {
-    __shared__ double red[256];
+    __shared__ double red[VG_NRES * 256];
     const int item = blockIdx.x;
     const vg_bitem I = items[item];
     double* part = part_all + VG_NRES * I.blk_off;
@@ -949,10 +995,11 @@ vg_bfinalize_kernel(const vg_bitem* __restrict__ items, double* __restrict__ par
                 for (int q = 0; q < VG_NRES; ++q) v[j][q] = i < cur ? part[VG_NRES * i + q] : 0.0;
             }
             __syncthreads();
+            double tot[VG_NRES];
+            vg_tree1024_256q<VG_NRES>(v, red, tot);
+            if (threadIdx.x == 0) {
 #pragma unroll
-            for (int q = 0; q < VG_NRES; ++q) {
-                const double tot = vg_tree1024_256(v[0][q], v[1][q], v[2][q], v[3][q], red);
-                if (threadIdx.x == 0) part[VG_NRES * c + q] = tot;
+                for (int q = 0; q < VG_NRES; ++q) part[VG_NRES * c + q] = tot[q];
             }
             __syncthreads();
         }
@@ -1274,7 +1321,9 @@ hipError_t vg_launch_batch_iteration_steps(hipStream_t st, const vg_bitem* items
                            items, task_item,
                            task_w0, n_tasks, task_windows, fold_list, fold_count, fold_cap, nh_sum, run_cube,
                            s_item, win_first, win_last, 6 + k_shift, JF, JF2);
-        const size_t lds = sizeof(double) * (size_t)(4LL * n + 1 + n / VG_TREE + 2);
+        // the refinement blocks' workspace, or the results blocks' tree buffer
+        size_t lds = sizeof(double) * (size_t)(4LL * n + 1 + n / VG_TREE + 2);
+        if (lds < sizeof(double) * VG_NRES * 256) lds = sizeof(double) * VG_NRES * 256;
         hipLaunchKernelGGL(vg_bresults_kernel, dim3((unsigned)(nblk + n_refine)), dim3(256), lds, st, items,
                            blk_item, nh, JF, JF2, beta, dh, part, n_refine, rd_item, rd_dim, n, alpha, x_all,
                            weights, counts, map64, map32);

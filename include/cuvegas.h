@@ -118,8 +118,11 @@ int cvg_device_info(int* n_cu, int* clock_khz, char* arch, int arch_len);
  * samples per step), lds_bins=0|1 (1, the default: per-block bins in LDS,
  * in one sampling pass per group of dimensions when d x n_intervals bins do
  * not fit; 0: every sample to the global bins), lds_map=0|1 (FP32 map table
- * in LDS), bin32=0|1, grad=0|1 (compile the backward replay kernel).  Every
- * other token is passed to hipRTC.
+ * in LDS), bin32=0|1, bin_copies=1..8 and copy_dim0=i (copies of the LDS bins
+ * of dimensions i.. that lanes add into by lane mod copies; by default 8, 4
+ * or 2 copies, as LDS allows, of the dimensions whose strata a window's cubes
+ * share when a stratum spans at most 32 bins), grad=0|1 (compile the backward
+ * replay kernel).  Every other token is passed to hipRTC.
  * On CVG_ERR_COMPILE, *out is still set so cvg_program_log can be read. */
 int cvg_program_create(const char* integrand_src, const cvg_params* shape, const char* options,
                        cvg_program** out);

@@ -157,7 +157,8 @@ def test_first_iteration_bit_exact(cv, orc, prec, layout, K):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("options", ["pair=1", "pipe=0", "pipe=1", "pipe=0 lds_map=1", "pipe=1 lds_map=1",
-                                     "pipe=1 lds_bins=0"])
+                                     "pipe=1 lds_bins=0", "pipe=1 bin_copies=4 copy_dim0=1",
+                                     "pipe=0 bin_copies=2 copy_dim0=0"])
 @pytest.mark.parametrize("K", [16, 2])
 def test_sampling_loop_variants_bit_exact(cv, orc, prec, options, K):
     """Every sampling-loop schedule (two samples per step; one per step; the
@@ -782,3 +783,38 @@ def test_python_integrate_from_threads(cv):
     assert len(out) == 8
     for res, err in out:
         assert abs(res - 1.0) < 5 * err, (res, err)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_bin_copies_default_shape_bit_exact(cv, orc, prec):
+    """A shape where the LDS bin copies are on by default (16 strata over 64
+    bins: a stratum spans 4 bins, and dimension 2's digit has a period of 256
+    cubes): bin counts exact, weights to 1e-12 and per-cube sums bit-exact
+    against the oracle, as without copies (polynomial integrand: exact in both
+    forms)."""
+    from cuvegas_amd import integrands as I
+    d, n, ns, N = 3, 64, 16, 200_000
+    itg = I.poly_peak(d)
+    _say("bin-copies-default", prec)
+    p, keep = _orc_params(orc, itg, d, n, ns, N, 1, 0 if prec == "fp64" else 1)
+    st = _integrator(cv, itg, d, n, ns, N, 2, prec)
+    nh0, x0 = st.get_nh(), st.get_map()
+    S = int(nh0.sum())
+    _, ids, _, ojf = orc.sample_dump(p, keep, x0, nh0, 0, S)
+    oJF, oJF2 = orc.cube_sums(nh0, ojf, ojf * ojf, 16)
+    oc = np.zeros((d, n), dtype=np.uint32)
+    ow = np.zeros((d, n))
+    for i in range(d):
+        oc[i] = np.bincount(ids[:, i], minlength=n)
+        np.add.at(ow[i], ids[:, i], ojf * ojf)
+    st.fill()
+    w, c = st.get_bins()
+    assert np.array_equal(c, oc)
+    assert np.max(np.abs(w - ow) / np.maximum(ow, 1e-300)) < 1e-12
+    st2 = cv.Integrator(itg.source, d, N, n_intervals=n, n_strat=ns, max_it=2, skip=1, seed=0,
+                        precision=prec, program=st.program)
+    st2.run(1, use_graph=False)
+    JF, JF2 = st2.get_cube_sums()
+    assert np.array_equal(JF, oJF) and np.array_equal(JF2, oJF2)
+    st.close()
+    st2.close()
