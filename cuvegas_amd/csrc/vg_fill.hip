@@ -497,8 +497,8 @@ vg_dump(const vg_map_t* __restrict__ gmap, const long long* __restrict__ nh_sum,
 //   vg_real integrand_grad(vg_real* x, const vg_real* theta, vg_real* grad)
 // returning f and writing df/dtheta[0..VG_NTHETA).  Samples are regenerated,
 // not stored.  Per-cube sums of J df/dtheta and their squares (each cube
-// replayed by one lane, or by a whole wave when large, see vg_grad) give the
-// stratified variance of the gradient:
+// replayed by a block of lanes sized to its sample count, see vg_grad) give
+// the stratified variance of the gradient:
 //   G_p   = sum_h V/n_h sum_s J df_p
 //   Var_p = sum_h (V^2 sum_s (J df_p)^2 / n_h - (V sum_s J df_p / n_h)^2) / (n_h - 1)
 // (unbiased per-cube sample variance; n_h >= 2 always)
@@ -567,27 +567,6 @@ __device__ inline void vg_grad_sample(long long h, long long s, unsigned it, con
     jf = (double)(f * jac);
 #pragma unroll
     for (int p = 0; p < NT; ++p) jg[p] = (double)(gr[p] * jac);
-}
-
-// Contributions of one cube (n samples, sums sf, sf2, g1, g2), with the
-// forward's results expressions for I_h and sigma_h^2 / n_h, written over the
-// sums: sf <- I_h, sf2 <- sigma_h^2 / n_h, g1[p] <- G_h,p, g2[p] <- Var_h,p.
-// The value terms keep the forward's division order (they replay I_k); the
-// gradient terms multiply by 1/n_h and 1/(n_h - 1), two divides per cube
-// instead of four per parameter.
-__device__ inline void vg_grad_cube(int n, double V, double& sf, double& sf2, double* g1, double* g2) {
-    const double Ih = sf / n * V;
-    const double Sig2 = (sf2 / n * V * V) - (Ih * Ih);
-    sf = Ih;
-    sf2 = Sig2 / n;
-    const double vn = V / n, vvn = V * V / n, in1 = 1.0 / (n - 1);
-#pragma unroll
-    for (int p = 0; p < NT; ++p) {
-        const double Gh = g1[p] * vn;
-        const double Gv = g2[p] * vvn - Gh * Gh;
-        g1[p] = Gh;
-        g2[p] = Gv * in1;
-    }
 }
 
 // Sums over the wave's 64 lanes of the NT + 1 values (a0, g[0..NT)), value
