@@ -531,7 +531,12 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
     //  * d > 8: the prefetched sample's state does not fit next to the
     //    current one (16-D oscillatory, FP64: 85 VGPRs spilled, fill 8.8 ms
     //    against 4.8 ms without the prefetch; FP32 3.42 against 3.33 ms), so
-    //    one sample per step, the FP32 map in LDS when it fits.
+    //    one sample per step.
+    //  * The FP32 map is read through the caches in every shape: staged in
+    //    LDS beside the bins it adds d LDS reads per sample to an LDS that the
+    //    bin atomics already keep busy (16-D oscillatory, 512 bins, 2e8
+    //    samples: fill 3.66 ms with the 64 KB map in LDS, 3.20 ms through the
+    //    caches; option lds_map=1 stages it).
     // 16 waves per CU for the FP32 form, 12 for the FP64 form (168 VGPRs).
     prog->block = prog->precision == CVG_FP32C ? kFillBlock : 768;
     prog->pair = 0;
@@ -552,8 +557,7 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
         if (prog->bin_dims < 1) prog->bin_dims = 1;
     };
     set_bin_dims(12);
-    prog->lds_map = prog->precision == CVG_FP32C && !prog->pipe && prog->lds_bins &&
-                    bin_bytes + map_bytes <= kLdsBytes;
+    prog->lds_map = 0;
 
     // Launch-shape knobs in the options string (consumed here, not passed to
     // hipRTC): lds_bins=0|1, lds_map=0|1, bin32=0|1, block=64..1024 (multiple
