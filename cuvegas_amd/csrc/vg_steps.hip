@@ -1146,13 +1146,11 @@ vg_bbigfold_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ t
 }
 
 // Batched counterpart of vg_scan_cubes_kernel: block b scans 1024 cubes of
-// item blk_item[b] (the items' cube ranges are not aligned for vector
-// accesses).  The block's capped cube starts go to LDS and its runs are spread
-// over the threads, each finding its cube by a binary search that starts from
-// its previous run's cube, so the run table is written in consecutive order.
-// Cubes holding more than VG_OWN_RUNS runs go to big_list (absolute run
-// indices, written by vg_big_runs_kernel) and the threads jump past their
-// runs.  counts[0]: fold list length, counts[1]: big list length.
+// item blk_item[b] (item ranges are padded to 4 cubes, so a thread's 4 cubes
+// are one 16-byte load).  A cube writes its own runs when it holds at most
+// VG_OWN_RUNS of them; larger cubes go to big_list (absolute run indices,
+// written by vg_big_runs_kernel).  counts[0]: fold list length, counts[1]:
+// big list length.
 __global__ void __launch_bounds__(256)
 vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ blk_item,
                       const int* __restrict__ nh_all, const long long* __restrict__ boff,
@@ -1161,7 +1159,6 @@ vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict_
                       long long* __restrict__ fold_list, int* __restrict__ counts, int fold_cap,
                       long long* __restrict__ big_list, int big_cap) {
     __shared__ long long wt[4];
-    __shared__ long long es[VG_TREE + 1];  // capped start of the block's cubes; es[nc] = end
     __shared__ long long slot0;
     const int item = blk_item[blockIdx.x];
     const vg_bitem I = items[item];
@@ -1171,7 +1168,6 @@ vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict_
     const long long run_cap = I.wcap * 64;
     const int C = I.C;
     const long long base = (blockIdx.x - I.blk_off) * VG_TREE;
-    const int nc = C - base < VG_TREE ? (int)(C - base) : VG_TREE;
     const long long h0 = base + 4 * threadIdx.x;
     const bool full = h0 + 4 <= C;  // item ranges are padded: 16-byte load, 2 x 16-byte stores
     long long v[4];
@@ -1212,7 +1208,6 @@ vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict_
         __syncthreads();
         slot += slot0;
     }
-    const int i0 = 4 * threadIdx.x;
     if (full) {
         longlong2* o = reinterpret_cast<longlong2*>(nh_sum + h0);
         o[0] = make_longlong2(e[0], e[1]);
@@ -1224,8 +1219,6 @@ vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict_
         if (h < C) {
             if (!full) nh_sum[h] = e[j];
             if (h == C - 1) nh_sum[C] = S;
-            es[i0 + j] = e[j];
-            if (i0 + j == nc - 1) es[nc] = e[j + 1];
             const long long n = nr[j];
             if (n > VG_OWN_RUNS) {
                 for (long long c = 0; c * VG_BIG_CHUNK < n; ++c, ++slot) {
@@ -1238,6 +1231,8 @@ vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict_
                         for (long long r = rc; r < rc + m; ++r) run_cube[r] = (int)h;
                     }
                 }
+            } else {
+                for (long long r = r0[j]; r < r0[j] + n; ++r) run_cube[r] = (int)h;
             }
             const long long st = e[j], en = e[j + 1];
             if (en > st && ((en - 1) >> (k_shift + 6)) - (st >> (k_shift + 6)) + 1 > VG_FOLD_MAX) {
@@ -1245,31 +1240,6 @@ vg_bscan_cubes_kernel(const vg_bitem* __restrict__ items, const int* __restrict_
                 if (fs < fold_cap) fold_list[fs] = ((long long)item << 32) | (long long)h;
             }
         }
-    }
-    __syncthreads();
-    const long long rb0 = (es[0] + kmask) >> k_shift;
-    long long rb1 = (es[nc] + kmask) >> k_shift;
-    if (rb1 > run_cap) rb1 = run_cap;
-    int lo = 0;  // the thread's runs increase, so its cube index only grows
-    for (long long r = rb0 + threadIdx.x; r < rb1;) {
-        const long long s = r << k_shift;
-        if (lo + 1 < nc && es[lo + 1] <= s) {
-            int a = lo + 1, b = nc;  // largest i in (lo, nc) with es[i] <= s
-            while (b - a > 1) {
-                const int mid = (a + b) >> 1;
-                if (es[mid] <= s) a = mid; else b = mid;
-            }
-            lo = a;
-        }
-        const long long cr0 = (es[lo] + kmask) >> k_shift;
-        long long cr1 = (es[lo + 1] + kmask) >> k_shift;
-        if (cr1 > run_cap) cr1 = run_cap;
-        if (cr1 - cr0 > VG_OWN_RUNS) {  // listed: jump to this thread's first run past the cube
-            r += ((cr1 - r + 255) >> 8) << 8;
-            continue;
-        }
-        run_cube[r] = (int)(base + lo);
-        r += 256;
     }
 }
 
