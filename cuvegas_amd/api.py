@@ -412,16 +412,25 @@ class Integrator:
 
 def combine(I: np.ndarray, S2: np.ndarray, skip: int):
     """Inverse-variance weighted mean of iterations >= skip, its error and chi2/dof
-    (reference cuda_wrapper.cu:758-766 and :869; chi2/dof is an addition)."""
+    (reference cuda_wrapper.cu:758-766 and :869; chi2/dof is an addition), by
+    the rule cvg_state_result applies: iterations of zero variance are exact,
+    and if there are any the result is their mean, with error and chi2/dof 0
+    (the reference's sums give inf / inf = NaN there)."""
+    I = np.asarray(I, dtype=np.float64)
+    S2 = np.asarray(S2, dtype=np.float64)
+    idx = list(range(skip, len(I)))
+    exact = [i for i in idx if S2[i] == 0.0]
+    if exact:
+        return sum(I[i] for i in exact) / len(exact), 0.0, 0.0
     res, sig, K = 0.0, 0.0, 0
-    for i in range(skip, len(I)):
+    for i in idx:
         res += I[i] / S2[i]
         sig += 1.0 / S2[i]
         K += 1
     if K > 0:
         res /= sig
     err = 1.0 / math.sqrt(sig) if sig > 0 else math.inf
-    chi2 = sum((I[i] - res) ** 2 / S2[i] for i in range(skip, len(I)))
+    chi2 = sum((I[i] - res) ** 2 / S2[i] for i in idx)
     return res, err, (chi2 / (K - 1) if K > 1 else 0.0)
 
 

@@ -993,6 +993,19 @@ struct vg_combined {
 };
 static vg_combined combine_record(const double* h, int n, int skip) {
     vg_combined c;
+    for (int i = 0; i < n; ++i) c.tot += (long long)h[VG_HIST * i + 3];
+    // Iterations of zero variance (an integrand the map and strata make
+    // constant, e.g. f = 1) are exact: their weight 1 / sigma_k^2 is infinite,
+    // and the reference's sums give inf / inf = NaN.  Here the result is the
+    // mean of those iterations, with error and chi2/dof 0.
+    double zsum = 0.0;
+    int nz = 0;
+    for (int i = skip; i < n; ++i)
+        if (h[VG_HIST * i + 1] == 0.0) { zsum += h[VG_HIST * i]; ++nz; }
+    if (nz > 0) {
+        c.result = zsum / nz;
+        return c;
+    }
     double res = 0.0, sig = 0.0;
     int Kc = 0;
     for (int i = skip; i < n; ++i) {
@@ -1008,7 +1021,6 @@ static vg_combined combine_record(const double* h, int n, int skip) {
         chi2u += dd * dd / s2u;
         varu += s2u / (s2 * s2);  // w_k^2 sigma_unb,k^2 with w_k = 1 / sigma_k^2
     }
-    for (int i = 0; i < n; ++i) c.tot += (long long)h[VG_HIST * i + 3];
     c.result = res;
     c.error = 1.0 / std::sqrt(sig);
     c.chi2_dof = Kc > 1 ? chi2 / (Kc - 1) : 0.0;

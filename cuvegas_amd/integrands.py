@@ -253,10 +253,12 @@ __device__ vg_real integrand(vg_real* x) {
 # --- the reference's integrand library (src/commons/functions.cu) -------------
 
 def roosarnold(n_dim: int) -> Integrand:
+    """(1/d) sum |4 x_i - 2|; each term through the single-precision fabsf, as
+    the reference writes it (functions.cu:15-23)."""
     src = """
 __device__ vg_real integrand(vg_real* x) {
     vg_real aux = (vg_real)0;
-    for (int i = 0; i < VG_NDIM; ++i) aux = aux + fabs((vg_real)4 * x[i] - (vg_real)2);
+    for (int i = 0; i < VG_NDIM; ++i) aux = aux + (vg_real)fabsf((float)((vg_real)4 * x[i] - (vg_real)2));
     return aux / (vg_real)VG_NDIM;
 }
 """
@@ -275,10 +277,12 @@ __device__ vg_real integrand(vg_real* x) {
 
 
 def hellekalek(n_dim: int) -> Integrand:
+    """prod (x_i - 1/2) / sqrt(12), the root taken in single precision as the
+    reference does (functions.cu:35-43)."""
     src = """
 __device__ vg_real integrand(vg_real* x) {
     vg_real f = (vg_real)1;
-    for (int i = 0; i < VG_NDIM; ++i) f = f * ((x[i] - (vg_real)0.5) / sqrt((vg_real)12));
+    for (int i = 0; i < VG_NDIM; ++i) f = f * ((x[i] - (vg_real)0.5) / (vg_real)sqrtf(12.0f));
     return f;
 }
 """

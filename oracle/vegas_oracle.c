@@ -436,7 +436,7 @@ ORC_EXPORT long long orc_prefix(long long C, const int* nh, long long* nh_sum) {
 
 ORC_EXPORT void orc_refine_dim(int n, double alpha, const double* weights, const uint32_t* counts,
                                const double* x_in, double* x_out) {
-    double* w = (double*)malloc(sizeof(double) * (size_t)n);
+    double* w = (double*)calloc((size_t)n, sizeof(double));
     double* d = (double*)malloc(sizeof(double) * (size_t)n);
     double* cum = (double*)malloc(sizeof(double) * (size_t)n);
     for (int k = 0; k < n; ++k) w[k] = counts[k] != 0 ? weights[k] / (double)counts[k] : weights[k];
@@ -844,23 +844,35 @@ ORC_EXPORT int orc_run(const orc_params* P, orc_out* O) {
         }
     }
 
-    /* weighted average over iterations skip..iters-1 (cuda_wrapper.cu:750-766,869) */
-    double res = 0.0, sig = 0.0;
-    int K = 0;
-    for (int it = P->skip; it < P->iters; ++it) {
-        res += O->hist_I[it] / O->hist_S2[it];
-        sig += 1.0 / O->hist_S2[it];
-        ++K;
+    /* weighted average over iterations skip..iters-1 (cuda_wrapper.cu:750-766,869);
+     * iterations of zero variance are exact (the library's rule, where the
+     * reference's sums give inf / inf = NaN): their mean, error and chi2 0 */
+    double zsum = 0.0;
+    int nz = 0;
+    for (int it = P->skip; it < P->iters; ++it)
+        if (O->hist_S2[it] == 0.0) { zsum += O->hist_I[it]; ++nz; }
+    if (nz > 0) {
+        O->result = zsum / nz;
+        O->error = 0.0;
+        O->chi2_dof = 0.0;
+    } else {
+        double res = 0.0, sig = 0.0;
+        int K = 0;
+        for (int it = P->skip; it < P->iters; ++it) {
+            res += O->hist_I[it] / O->hist_S2[it];
+            sig += 1.0 / O->hist_S2[it];
+            ++K;
+        }
+        if (K > 0) res /= sig;
+        O->result = res;
+        O->error = 1.0 / sqrt(sig);
+        double chi2 = 0.0;
+        for (int it = P->skip; it < P->iters; ++it) {
+            double dd = O->hist_I[it] - res;
+            chi2 += dd * dd / O->hist_S2[it];
+        }
+        O->chi2_dof = K > 1 ? chi2 / (K - 1) : 0.0;
     }
-    if (K > 0) res /= sig;
-    O->result = res;
-    O->error = 1.0 / sqrt(sig);
-    double chi2 = 0.0;
-    for (int it = P->skip; it < P->iters; ++it) {
-        double dd = O->hist_I[it] - res;
-        chi2 += dd * dd / O->hist_S2[it];
-    }
-    O->chi2_dof = K > 1 ? chi2 / (K - 1) : 0.0;
     if (O->x_edges) memcpy(O->x_edges, x_edges, sizeof(double) * (size_t)(d * E));
     if (O->nh) memcpy(O->nh, nh, sizeof(int) * (size_t)C);
 

@@ -129,6 +129,22 @@ def test_closed_forms_of_library():
     q, _ = si.nquad(lambda x, y: (1 + 0.5 * x + 1.5 * y) ** -3, [[0, 1], [0, 1]])
     assert I.corner_peak(2, [0.5, 1.5]).exact == pytest.approx(q, rel=1e-10)
     assert I.oscillatory(1, 0.0, [1.0]).exact == pytest.approx(math.sin(1.0), rel=1e-14)
+    # the reference library's closed forms (src/commons/functions.cu) against
+    # 2-D quadrature of their definitions
+    quad2 = {
+        "linear": lambda x, y: x + y,
+        "roosarnold": lambda x, y: (abs(4 * x - 2) + abs(4 * y - 2)) / 2,
+        "morokoffcalfisch": lambda x, y: (x * y) ** 0.5 * 1.5 ** 2,
+        "expon": lambda x, y: math.exp(-(x * x + y * y)),
+        "cosin": lambda x, y: math.cos(x) * math.cos(y),
+        "dirichlet": lambda x, y: x ** 4 * y ** 4 / (math.gamma(5) ** 2 / math.gamma(10)),
+        "sinexp": lambda x, y: math.sin(x) + math.exp(y),
+        "one": lambda x, y: 1.0,
+    }
+    for name, f in quad2.items():
+        q, _ = si.nquad(f, [[0, 1], [0, 1]], opts={"points": [0.5]})
+        assert I.by_name(name, 2).exact == pytest.approx(q, rel=1e-9), name
+    assert I.hellekalek(2).exact == 0.0
 
 
 def test_run_len_must_be_power_of_two():
@@ -194,3 +210,22 @@ def test_roofline_slot_counts():
     assert slots_per_eval(6, "gaussian_mixture") == 302
     assert slots_per_replayed_sample(6, "gaussian_mixture", 32) == 454
     assert valu_peak(256, 2.4e9) == pytest.approx(3.93216e13)
+
+
+def test_combine_rule():
+    """cuvegas_amd.combine: the reference's inverse-variance mean of the
+    iterations >= skip (cuda_wrapper.cu:758-766) with chi2/dof, checked on
+    hand-computed values; iterations of zero variance are exact and give
+    their mean with error 0."""
+    import math
+    from cuvegas_amd import combine
+    I = [9.0, 1.0, 2.0, 4.0]
+    S2 = [1.0, 1.0, 4.0, 4.0]
+    res, err, chi2 = combine(I, S2, 1)
+    # weights 1, 1/4, 1/4: mean (1 + 0.5 + 1) / 1.5, error 1 / sqrt(1.5)
+    assert res == pytest.approx(2.5 / 1.5, rel=1e-15)
+    assert err == pytest.approx(1.0 / math.sqrt(1.5), rel=1e-15)
+    m = 2.5 / 1.5
+    assert chi2 == pytest.approx(((1 - m) ** 2 + (2 - m) ** 2 / 4 + (4 - m) ** 2 / 4) / 2, rel=1e-14)
+    assert combine([5.0, 3.0, 3.5, 7.0], [1.0, 0.0, 0.0, 2.0], 1) == (3.25, 0.0, 0.0)
+

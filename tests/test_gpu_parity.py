@@ -818,3 +818,30 @@ def test_bin_copies_default_shape_bit_exact(cv, orc, prec):
     assert np.array_equal(JF, oJF) and np.array_equal(JF2, oJF2)
     st.close()
     st2.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_integrand_library_against_closed_forms(cv, prec):
+    """Every integrand of the library -- the reference's own test set
+    (src/commons/functions.cu:5-111: linear, roosarnold, morokoffcalfisch,
+    hellekalek, ridge, expon, cosin, dirichlet, gaussian, sinexp, one) and the
+    benchmark families -- integrated on the GPU (3-D; ridge in 4-D, where its
+    closed form is known): within 5 sigma of the closed form (the larger of
+    the reference's and the unbiased error), the error positive, and the
+    constant integrand exact (its map stays uniform)."""
+    from cuvegas_amd import integrands as I
+    N, iters = 1_000_000, 10
+    for name in sorted(I.LIBRARY):
+        d = 4 if name == "ridge" else 3
+        itg = I.by_name(name, d)
+        _say("library", name, prec)
+        st = cv.Integrator(itg.source, d, N, n_intervals=512, max_it=iters, skip=5, seed=3, precision=prec)
+        st.run(iters)
+        r = st.result()
+        st.close()
+        if name == "one":
+            assert abs(r["result"] - 1.0) < 1e-12 and r["error"] < 1e-10, r
+            continue
+        err = max(r["error"], r["error_unbiased"])
+        assert err > 0 and math.isfinite(r["result"]), (name, r)
+        assert abs(r["result"] - itg.exact) < 5 * err, (name, r["result"], itg.exact, err)

@@ -314,6 +314,27 @@ def test_run_polynomial_fp32_and_domain(orc):
     assert abs(r["result"] - exact) < 4 * r["error"] + 1e-12
 
 
+def test_run_constant_integrand_is_exact(orc):
+    """f = 1 in the FP32 form: J f is exactly 1 under the uniform map, so the
+    first iterations have zero variance (later ones move the map: with D = 0
+    every cube gets the minimum 2 samples and some bins stay empty).  With an
+    exact iteration among those averaged, the combine returns the exact
+    iterations' mean with error 0 (the reference's inverse-variance sums would
+    give inf / inf = NaN); without one, the usual weighted mean."""
+    d, n, ns, N, iters = 3, 32, 4, 4000, 6
+    p, keep = orc.make_params(d, n, ns, N, iters, orc.F_ONE, precision=1, skip=2)
+    r = orc.run(p, keep, ns ** d)
+    S2 = r["hist_S2"]
+    assert S2[0] == 0.0 and S2[2] == 0.0 and np.all(r["hist_I"][S2 == 0.0] == 1.0)
+    assert r["result"] == 1.0 and r["error"] == 0.0 and r["chi2_dof"] == 0.0
+    if np.all(S2[4:] > 0):
+        p, keep = orc.make_params(d, n, ns, N, iters, orc.F_ONE, precision=1, skip=4)
+        r = orc.run(p, keep, ns ** d)
+        w = 1.0 / S2[4:]
+        assert r["result"] == pytest.approx(np.sum(r["hist_I"][4:] * w) / np.sum(w), rel=1e-14)
+        assert np.isfinite(r["error"]) and r["error"] > 0
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_run_closed_form_oscillatory(orc, prec, seed):
