@@ -33,13 +33,15 @@ def _log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def run_form(precision, itg, n_eval_it, warmup, steps, n_strat, nbins, seed, use_timing=True, options=None):
+def run_form(precision, itg, n_eval_it, warmup, steps, n_strat, nbins, seed, use_timing=True, options=None,
+             run_len=0):
     import numpy as np
     from cuvegas_amd import Integrator
     iters = warmup + steps
     t0 = time.perf_counter()
     st = Integrator(itg.source, itg.n_dim, n_eval_it, n_intervals=nbins, n_strat=n_strat,
-                    max_it=iters, skip=warmup, seed=seed, precision=precision, options=options)
+                    max_it=iters, skip=warmup, seed=seed, precision=precision, options=options,
+                    run_len=run_len)
     setup_s = time.perf_counter() - t0
     st.set_timing(use_timing)
     st.run(warmup, use_graph=False)
@@ -402,7 +404,8 @@ def single_bench(args, workload, warmup, steps, with_cpu=True, neval=None):
     for prec in (["fp64", "fp32"] if args.only is None else [args.only]):
         _log(workload, "running", prec)
         forms[prec] = run_form(prec, itg, n_eval_it, warmup, steps, n_strat, nbins, seed,
-                               options=getattr(args, "fill_options", None) or None)
+                               options=getattr(args, "fill_options", None) or None,
+                               run_len=getattr(args, "run_len", 0) or 0)
         _log(workload, prec, json.dumps(forms[prec]))
     head = forms.get("fp32") or forms["fp64"]
 
@@ -458,6 +461,8 @@ def main():
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
     ap.add_argument("--fill-options", default="",
                     help="dg8/osc16/gauss4: sampling-program options (e.g. 'bin_copies=1') for A/B runs")
+    ap.add_argument("--run-len", type=int, default=0,
+                    help="dg8/osc16/gauss4: samples per lane run (power of two; default: the library's 16)")
     ap.add_argument("--batch-config", default="",
                     help="batch512: 'block,run_len,task_windows' instead of autotuning")
     ap.add_argument("--skip-sequential", action="store_true",

@@ -141,16 +141,14 @@ __device__ inline bool vg_last_block(unsigned* cnt, int* flag) {
 // block partial of q = 0: sum I_h, 1: sum sigma_h^2 / n_h, 2: sum d_h,
 // 3: sum sigma_h^2 / (n_h - 1) (no reference counterpart).
 // ---------------------------------------------------------------------------
-// Results of cube h from complete per-cube sums (the window folds ran before).
-__device__ inline void vg_cube_result_sums(long long h, int C, double V, double beta,
-                                           const int* __restrict__ nh, const double2* __restrict__ JS,
-                                           double* __restrict__ dh, double& sr, double& ss, double& dd,
-                                           double& su) {
-    sr = 0.0; ss = 0.0; dd = 0.0; su = 0.0;
-    if (h >= C) return;
-    const int neval = nh[h];
-    const double2 js = JS[h];
-    const double j1 = js.x, j2 = js.y;
+// I_h, sigma_h^2 / n_h, d_h and sigma_h^2 / (n_h - 1) of a cube with sums
+// j1 = sum J f, j2 = sum (J f)^2 over neval >= 2 samples, in the reference's
+// expression order (cuda_wrapper.cu:33-47).  (A branch that wrote the
+// divisions by n_h = 2 as products by 0.5 -- the same bits -- left the results
+// launch unchanged, 125 against 123 us at 16.8 M cubes: it is bound by memory
+// and latency, not by the divide sequences.)
+__device__ inline void vg_cube_terms(double j1, double j2, int neval, double V, double beta, double& sr,
+                                     double& ss, double& dd, double& su) {
     const double Ih = j1 / neval * V;
     const double Sig2 = (j2 / neval * V * V) - (Ih * Ih);
     sr = Ih;
@@ -160,6 +158,17 @@ __device__ inline void vg_cube_result_sums(long long h, int C, double V, double 
     double t = V * V * j2 / neval - u * u;
     if (t < 0) t = 0;
     dd = vg_pow(t, beta);
+}
+
+// Results of cube h from complete per-cube sums (the window folds ran before).
+__device__ inline void vg_cube_result_sums(long long h, int C, double V, double beta,
+                                           const int* __restrict__ nh, const double2* __restrict__ JS,
+                                           double* __restrict__ dh, double& sr, double& ss, double& dd,
+                                           double& su) {
+    sr = 0.0; ss = 0.0; dd = 0.0; su = 0.0;
+    if (h >= C) return;
+    const double2 js = JS[h];
+    vg_cube_terms(js.x, js.y, nh[h], V, beta, sr, ss, dd, su);
     dh[h] = dd;
 }
 
@@ -950,17 +959,8 @@ vg_bresults_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ b
         const long long h = base + 256 * j;
         double sr = 0.0, ss = 0.0, dd = 0.0, su = 0.0;
         if (h < I.C) {
-            const int neval = nh[I.cube_off + h];
-            const double j1 = JF[I.cube_off + h], j2 = JF2[I.cube_off + h];
-            const double Ih = j1 / neval * I.V;
-            const double Sig2 = (j2 / neval * I.V * I.V) - (Ih * Ih);
-            sr = Ih;
-            ss = Sig2 / neval;
-            su = Sig2 / (neval - 1);
-            const double u = I.V * j1 / neval;
-            double t = I.V * I.V * j2 / neval - u * u;
-            if (t < 0) t = 0;
-            dd = vg_pow(t, beta);
+            vg_cube_terms(JF[I.cube_off + h], JF2[I.cube_off + h], nh[I.cube_off + h], I.V, beta, sr, ss, dd,
+                          su);
             dh[I.cube_off + h] = dd;
         }
         r[j][0] = sr; r[j][1] = ss; r[j][2] = dd; r[j][3] = su;
