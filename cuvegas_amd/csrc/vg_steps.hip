@@ -160,29 +160,36 @@ __device__ inline void vg_cube_terms(double j1, double j2, int neval, double V, 
     dd = vg_pow(t, beta);
 }
 
-// Results of cube h from complete per-cube sums (the window folds ran before).
-__device__ inline void vg_cube_result_sums(long long h, int C, double V, double beta,
-                                           const int* __restrict__ nh, const double2* __restrict__ JS,
-                                           double* __restrict__ dh, double& sr, double& ss, double& dd,
-                                           double& su) {
-    sr = 0.0; ss = 0.0; dd = 0.0; su = 0.0;
-    if (h >= C) return;
-    const double2 js = JS[h];
-    vg_cube_terms(js.x, js.y, nh[h], V, beta, sr, ss, dd, su);
-    dh[h] = dd;
-}
-
 // Results of the 1024 cubes of block b: d_h, and the block partials of
 // (sum I_h, sum sigma_h^2 / n_h, sum d_h, sum sigma_h^2 / (n_h - 1)) in
-// part[VG_NRES b + q].  256 threads; red: VG_NRES * 256 doubles of LDS.
+// part[VG_NRES b + q].  256 threads, 4 cubes each; red: VG_NRES * 256 doubles
+// of LDS.  The per-cube sums (complete: the window folds ran before) of all 4
+// cubes are requested before any is evaluated, so a thread's loads overlap
+// instead of waiting behind the previous cube's divides and pow.
 __device__ inline void vg_results_block(long long b, int C, double V, double beta,
                                         const int* __restrict__ nh, const double2* __restrict__ JS,
                                         double* __restrict__ dh, double* __restrict__ part, double* red) {
     const long long base = b * VG_TREE + threadIdx.x;
+    int ne[4];
+    double2 js[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const long long h = base + 256 * j;
+        ne[j] = 2;
+        js[j] = make_double2(0.0, 0.0);
+        if (h < C) { ne[j] = nh[h]; js[j] = JS[h]; }
+    }
     double r[4][VG_NRES];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-        vg_cube_result_sums(base + 256 * j, C, V, beta, nh, JS, dh, r[j][0], r[j][1], r[j][2], r[j][3]);
+    for (int j = 0; j < 4; ++j) {
+        const long long h = base + 256 * j;
+        double sr = 0.0, ss = 0.0, dd = 0.0, su = 0.0;
+        if (h < C) {
+            vg_cube_terms(js[j].x, js[j].y, ne[j], V, beta, sr, ss, dd, su);
+            dh[h] = dd;
+        }
+        r[j][0] = sr; r[j][1] = ss; r[j][2] = dd; r[j][3] = su;
+    }
     double tot[VG_NRES];
     vg_tree1024_256q<VG_NRES>(r, red, tot);
     if (threadIdx.x == 0) {
@@ -953,14 +960,22 @@ vg_bresults_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ b
     const long long bb = blockIdx.x - n_refine;
     const vg_bitem I = items[blk_item[bb]];
     const long long base = (bb - I.blk_off) * VG_TREE + threadIdx.x;
+    // the sums of all 4 cubes are requested before any is evaluated
+    int ne[4];
+    double j1[4], j2[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const long long h = base + 256 * j;
+        ne[j] = 2; j1[j] = 0.0; j2[j] = 0.0;
+        if (h < I.C) { ne[j] = nh[I.cube_off + h]; j1[j] = JF[I.cube_off + h]; j2[j] = JF2[I.cube_off + h]; }
+    }
     double r[4][VG_NRES];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const long long h = base + 256 * j;
         double sr = 0.0, ss = 0.0, dd = 0.0, su = 0.0;
         if (h < I.C) {
-            vg_cube_terms(JF[I.cube_off + h], JF2[I.cube_off + h], nh[I.cube_off + h], I.V, beta, sr, ss, dd,
-                          su);
+            vg_cube_terms(j1[j], j2[j], ne[j], I.V, beta, sr, ss, dd, su);
             dh[I.cube_off + h] = dd;
         }
         r[j][0] = sr; r[j][1] = ss; r[j][2] = dd; r[j][3] = su;
