@@ -269,7 +269,8 @@ def mix6_bench(args, neval=None):
     for prec in (["fp64", "fp32"] if args.only is None else [args.only]):
         _log("mixture", prec)
         st = Integrator(itg.source, d, n_eval_it, n_intervals=nbins, max_it=iters, skip=skip, seed=0,
-                        precision=prec, theta=theta, options="grad")
+                        precision=prec, theta=theta,
+                        options=("grad " + (getattr(args, "fill_options", "") or "")).strip())
         st.set_record(True)
         st.set_timing(True)
         st.run(iters, use_graph=False)
@@ -460,7 +461,7 @@ def main():
                     help="dg8: leave out the osc16, mix6 and batch512 workloads that the default line carries")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
     ap.add_argument("--fill-options", default="",
-                    help="dg8/osc16/gauss4: sampling-program options (e.g. 'bin_copies=1') for A/B runs")
+                    help="dg8/osc16/gauss4/mix6: sampling-program options (e.g. 'bin_copies=1') for A/B runs")
     ap.add_argument("--run-len", type=int, default=0,
                     help="dg8/osc16/gauss4: samples per lane run (power of two; default: the library's 16)")
     ap.add_argument("--batch-config", default="",

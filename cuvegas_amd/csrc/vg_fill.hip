@@ -624,8 +624,16 @@ __device__ inline void vg_wave_tsum(double* tb, int lane, double a0, const doubl
 // consecutive waves of the layout and writes its accumulator to part[q][unit]
 // (value q of 2 + 2 NT, unit-contiguous); vg_grad_reduce sums the units in
 // index order.  The largest cubes come first, so the hardware starts the
-// longest units first.  Two waves per SIMD (at most 256 VGPRs).
-extern "C" __global__ void __launch_bounds__(64, 2)
+// longest units first.  The launch bound asks for VG_GRAD_WPS waves per SIMD.
+// With 1 the compiler may use up to 512 registers: the FP64 kernel then keeps
+// its 286 without scratch spills (one wave per SIMD, the same replay time as
+// two waves with 34 spilled), and the FP32 kernel is scheduled into 236 (two
+// waves per SIMD still fit), its replay 14% faster than under the bound of 2
+// waves (6-D mixture, 32 parameters: 0.78 against 0.90 ms).
+#ifndef VG_GRAD_WPS
+#define VG_GRAD_WPS 1
+#endif
+extern "C" __global__ void __launch_bounds__(64, VG_GRAD_WPS)
 vg_grad(const vg_map_t* __restrict__ gmap, const int* __restrict__ nh, const long long* __restrict__ nh_sum,
         const int* __restrict__ order, const int* __restrict__ lane_start, const int* __restrict__ wave_first,
         int C, int slots_per_unit, unsigned it, double V, const double* __restrict__ domain, double domain_vol,
