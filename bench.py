@@ -33,6 +33,69 @@ def _log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def _time_to_target(I_k, S2, iter_ms, warmup, exact, target=1e-4, S2chi=None):
+    """Device time (ms, iterations from the first) and index of the first
+    iteration at which the running inverse-variance mean of the iterations >=
+    warmup is within `target` relative of the closed form with chi2/dof < 2
+    (chi2 from the variances S2chi, S2 by default); (None, None) if none is."""
+    S2chi = S2 if S2chi is None else S2chi
+    cum = 0.0
+    for k in range(len(I_k)):
+        cum += iter_ms[k]
+        if k < warmup:
+            continue
+        sl = slice(warmup, k + 1)
+        w = 1.0 / S2[sl]
+        mean = float((I_k[sl] * w).sum() / w.sum())
+        K = k + 1 - warmup
+        chi2 = float((((I_k[sl] - mean) ** 2) / S2chi[sl]).sum() / (K - 1)) if K > 1 else 0.0
+        if abs(mean - exact) / abs(exact) <= target and chi2 < 2.0:
+            return cum, k
+    return None, None
+
+
+def time_to_target_seeds(itg, n_eval_it, n_strat, nbins, warmup, iters, seeds):
+    """Time to 1e-4 of both forms over several seeds (one path per seed and
+    form; the single path of the headline run says little, as the two forms'
+    estimates scatter independently and an iteration's sigma is about the
+    target itself): for the seeds that reach it within `iters` iterations,
+    the mean and median device time and the mean iteration index, and how
+    many do -- with chi2/dof < 2 from the reference's per-iteration variances
+    (the metric's definition) and from the unbiased ones."""
+    import numpy as np
+    from cuvegas_amd import Integrator
+    out = {}
+    for prec in ("fp64", "fp32"):
+        got = {"reference": ([], []), "unbiased": ([], [])}
+        prog = None
+        for seed in seeds:
+            st = Integrator(itg.source, itg.n_dim, n_eval_it, n_intervals=nbins, n_strat=n_strat,
+                            max_it=iters, skip=warmup, seed=seed, precision=prec, program=prog)
+            prog = st.program
+            st.set_timing(True)
+            st.run(iters, use_graph=False)
+            I_k, S2, _ = st.history()
+            S2u = st.history_unbiased()
+            _, iter_ms = st.timing()
+            st.close()
+            for key, chi in (("reference", S2), ("unbiased", S2u)):
+                t, k = _time_to_target(I_k, S2, iter_ms, warmup, itg.exact, S2chi=chi)
+                if t is not None:
+                    got[key][0].append(t)
+                    got[key][1].append(k)
+        out[prec] = {key: {"mean_ms": float(np.mean(t)) if t else None,
+                           "median_ms": float(np.median(t)) if t else None,
+                           "mean_iteration": float(np.mean(k)) if k else None,
+                           "reached": len(t)}
+                     for key, (t, k) in got.items()}
+    a, b = out["fp32"]["reference"]["mean_ms"], out["fp64"]["reference"]["mean_ms"]
+    out["fp64_over_fp32_mean"] = b / a if a and b else None
+    out["iterations"] = iters
+    out["seeds"] = [min(seeds), max(seeds)]
+    out["n_seeds"] = len(seeds)
+    return out
+
+
 def run_form(precision, itg, n_eval_it, warmup, steps, n_strat, nbins, seed, use_timing=True, options=None,
              run_len=0):
     import numpy as np
@@ -54,20 +117,7 @@ def run_form(precision, itg, n_eval_it, warmup, steps, n_strat, nbins, seed, use
     st.close()
     timed_evals = int(ne[warmup:].sum())
     timed_ms = float(iter_ms[warmup:].sum())
-    # running weighted mean (iterations >= skip) -> time to 1e-4
-    t_to, it_to, cum = None, None, 0.0
-    for k in range(iters):
-        cum += iter_ms[k]
-        if k < warmup:
-            continue
-        sl = slice(warmup, k + 1)
-        w = 1.0 / S2[sl]
-        mean = float((I_k[sl] * w).sum() / w.sum())
-        K = k + 1 - warmup
-        chi2 = float((((I_k[sl] - mean) ** 2) * w).sum() / (K - 1)) if K > 1 else 0.0
-        if abs(mean - itg.exact) / abs(itg.exact) <= 1e-4 and chi2 < 2.0:
-            t_to, it_to = cum, k
-            break
+    t_to, it_to = _time_to_target(I_k, S2, iter_ms, warmup, itg.exact)
     return {
         "precision": precision,
         "evals_per_s": timed_evals / (timed_ms * 1e-3),
@@ -499,6 +549,11 @@ def main():
         return
     out = single_bench(args, args.workload, args.warmup, args.steps, with_cpu=not args.no_cpu, neval=args.neval)
     if args.workload == "dg8" and not args.no_extras:
+        # time to 1e-4 of both forms over 64 more seeds (5 + 15 iterations each)
+        from cuvegas_amd import integrands as I
+        _log("dg8 time to 1e-4 over seeds")
+        out["time_to_1e-4_seeds"] = time_to_target_seeds(
+            I.double_gaussian(8, 0.33, 0.67, 0.05), int(args.neval or 1e8), 8, 1024, 5, 20, range(1, 65))
         # the other configurations of the hot path, each at its own size and
         # iteration count, in the same line (their own timed regions)
         extras = {}
