@@ -845,3 +845,36 @@ def test_integrand_library_against_closed_forms(cv, prec):
         err = max(r["error"], r["error_unbiased"])
         assert err > 0 and math.isfinite(r["result"]), (name, r)
         assert abs(r["result"] - itg.exact) < 5 * err, (name, r["result"], itg.exact, err)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_unbiased_variance_is_calibrated(cv, prec):
+    """Per-iteration pulls (I_k - I) / sigma_k over 16 seeds x 10 averaged
+    iterations of a 4-D Gaussian whose cubes mostly hold the minimum 2 samples
+    (30 strata: 810,000 cubes for 1e6 samples): with the unbiased per-cube
+    variances the mean pull^2 is 1 within its scatter (160 iterations: about
+    +-0.11), while the reference's plug-in variance / n_h (cuda_wrapper.cu:33-47)
+    halves the variance of every 2-sample cube, so its mean pull^2 is well
+    above 1.  A dropped (n_h - 1) or a misplaced n_h in either estimator moves
+    these means by about a factor of 2."""
+    from cuvegas_amd import integrands as I
+    itg = I.gaussian_peak(4, 0.5, 0.1)
+    p_ref, p_unb, prog = [], [], None
+    for seed in range(16):
+        _say("calibration", prec, seed)
+        st = cv.Integrator(itg.source, 4, 1_000_000, n_intervals=256, n_strat=30, max_it=12, skip=2, seed=seed,
+                           precision=prec, program=prog)
+        prog = st.program
+        st.run(12)
+        I_k, S2, _ = st.history()
+        S2u = st.history_unbiased()
+        nh = st.get_nh()
+        st.close()
+        assert np.mean(nh == 2) > 0.5
+        p_ref.extend(((I_k[2:] - itg.exact) ** 2 / S2[2:]).tolist())
+        p_unb.extend(((I_k[2:] - itg.exact) ** 2 / S2u[2:]).tolist())
+    m_unb, m_ref = float(np.mean(p_unb)), float(np.mean(p_ref))
+    _say("calibration-means", prec, round(m_unb, 3), round(m_ref, 3))
+    assert 0.7 < m_unb < 1.4, (m_unb, m_ref)
+    assert m_ref > 1.3 * m_unb, (m_unb, m_ref)
+
