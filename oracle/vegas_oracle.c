@@ -36,7 +36,13 @@
  *   orc_refine        fixed point under uniform weights, monotone edges,
  *                     piecewise-constant weights -> analytic inverse CDF,
  *                     alpha = 0 -> uniform damping
- *   orc_run           closed-form integrals (pull < 4 sigma, chi2/dof < 2)
+ *   orc_run           closed-form integrals (pull < 4 sigma, chi2/dof < 2) of
+ *                     every integrand it restates: polynomial, linear with a
+ *                     domain, Gaussian (one and per-dimension centres), double
+ *                     Gaussian, oscillatory, corner peak, mixture; f = 1 exact
+ *   orc_grad_replay   value equal to the forward's I_k on the same samples;
+ *                     gradient equal to central differences of the replayed
+ *                     value; within 4 sigma of the erf closed-form gradient
  */
 #include <math.h>
 #include <omp.h>

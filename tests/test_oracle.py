@@ -405,3 +405,54 @@ def test_run_closed_form_gaussian_per_dimension_centres(orc, prec):
     assert abs(r["result"] / exact - 1) < 2e-3 and r["chi2_dof"] < 3.0
     shared = (0.5 * (math.erf(0.5 / (sigma * math.sqrt(2))) * 2)) ** d  # every centre at 0.5
     assert abs(shared - exact) > 20 * r["error"]
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_run_closed_form_mixture(orc, prec):
+    """3-D four-component Gaussian mixture (weights, widths and centres drawn
+    as for the autograd workload, seed 5): the oracle integrator, both forms,
+    within 4 sigma of the erf closed form, chi2/dof < 3.  A width taken for a
+    centre, a component weight dropped or a wrong normalisation moves the
+    value by far more than 4 sigma."""
+    from cuvegas_amd import integrands as I
+    d, n, ns, N, iters = 3, 128, 6, 40000, 10
+    th = I.mixture_theta(d, seed=5)
+    exact, _ = I.mixture_exact(d, th)
+    p, keep = orc.make_params(d, n, ns, N, iters, orc.F_MIXTURE, th, seed=1, precision=prec)
+    r = orc.run(p, keep, ns ** d, snapshots=False)
+    assert abs(r["result"] - exact) < 4 * r["error"], (r["result"], exact, r["error"])
+    assert r["chi2_dof"] < 3.0
+    assert r["error"] < 2e-3 * exact
+
+
+def test_grad_replay_against_differences_and_closed_form(orc):
+    """The oracle's replay of a frozen iteration of the 3-D mixture (uniform
+    map and allocation of iteration 0): its value equals the forward's I_0
+    (same samples); d value / d theta equals central differences of the
+    replayed value itself (the samples do not move with theta, so the frozen
+    estimator is smooth in theta) to 1e-5 of the largest component; and each
+    component is within 4 of its own standard errors of the closed-form
+    gradient.  A gradient term with a wrong sign, factor or index fails the
+    differences."""
+    from cuvegas_amd import integrands as I
+    d, n, ns, N = 3, 64, 5, 50000
+    th = np.array(I.mixture_theta(d, seed=5))
+    p, keep = orc.make_params(d, n, ns, N, 1, orc.F_MIXTURE, th.tolist(), skip=0, seed=2)
+    r = orc.run(p, keep, ns ** d, snapshots=False)
+    C = ns ** d
+    x0 = np.tile(np.linspace(0.0, 1.0, n + 1), (d, 1))
+    nh0 = np.full(C, max(2, N // C), dtype=np.int32)
+    I0, S20, g, gv = orc.grad_replay(p, keep, x0, nh0, 0, th)
+    assert abs(I0 - r["hist_I"][0]) <= 1e-12 * abs(I0) and S20 > 0
+    gmax = np.max(np.abs(g))
+    for q in range(th.size):
+        h = 1e-6 * max(abs(th[q]), 1e-3)
+        tp, tm = th.copy(), th.copy()
+        tp[q] += h
+        tm[q] -= h
+        fd = (orc.grad_replay(p, keep, x0, nh0, 0, tp)[0] - orc.grad_replay(p, keep, x0, nh0, 0, tm)[0]) / (2 * h)
+        assert abs(fd - g[q]) <= 1e-5 * gmax, (q, fd, g[q])
+    _, eg = I.mixture_exact(d, th.tolist())
+    pull = (g - np.array(eg)) / np.sqrt(gv)
+    assert np.all(np.abs(pull) < 4.0), pull
+
