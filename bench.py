@@ -300,6 +300,56 @@ def batch_bench(args, sequential=True):
     return out
 
 
+def cpu_config_bench(args):
+    """The CPU-runnable configuration at its own size: 4-D normalised Gaussian
+    (sigma = 0.1), 1024 bins, n_strat 22 (234,256 cubes), 1e6 evaluations per
+    iteration, 10 iterations (5 averaged), FP64, seed 0.  The whole 10-iteration
+    job on the GPU (FP64 form, one graph replay, after a first run that loads
+    the program) and in the CPU oracle (the FP64 restatement of the reference
+    algorithm, on every core of the process's OpenMP budget and on one core),
+    each with its error against the closed form.  At this size a GPU iteration
+    is a few launches' latency, not throughput."""
+    from cuvegas_amd import Integrator
+    from cuvegas_amd import integrands as I
+    d, nbins, n_strat, N, iters, skip = 4, 1024, 22, 1_000_000, 10, 5
+    itg = I.gaussian_peak(d, 0.5, 0.1)
+    runs = []
+    prog = None
+    for _ in range(2):
+        st = Integrator(itg.source, d, N, n_intervals=nbins, n_strat=n_strat, max_it=iters, skip=skip,
+                        seed=0, precision="fp64", program=prog)
+        prog = st.program
+        st.run(iters, use_graph=True)
+        runs.append(st.result())
+        st.close()
+    r = runs[-1]
+    gpu = {"loop_ms": r["loop_ms"], "evals": int(r["tot_nevals"]),
+           "evals_per_s": r["tot_nevals"] / (r["loop_ms"] * 1e-3), "result": r["result"], "error": r["error"],
+           "pull": (r["result"] - itg.exact) / r["error"]}
+    out = {"workload": f"4-D Gaussian sigma=0.1, nbins={nbins}, n_strat={n_strat} ({n_strat ** d:,} cubes), "
+                       f"{N:.0e} evals/iteration, {iters} iterations ({skip} skipped), FP64, seed 0",
+           "exact": itg.exact, "gpu_fp64": gpu}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        affinity = len(os.sched_getaffinity(0))
+        budget = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        cores = min(affinity, budget) if budget > 0 else affinity
+        for threads in (cores, 1):
+            p, keep = orc.make_params(d, nbins, n_strat, N, iters, orc.F_GAUSS, list(itg.params), skip=skip,
+                                      seed=0, threads=threads)
+            t0 = time.perf_counter()
+            o = orc.run(p, keep, n_strat ** d, snapshots=False)
+            dt = time.perf_counter() - t0
+            out[f"cpu_oracle_{threads}_threads" if threads > 1 else "cpu_oracle_1_thread"] = {
+                "wall_ms": dt * 1e3, "evals": int(o["tot_nevals"]), "evals_per_s": o["tot_nevals"] / dt,
+                "result": o["result"], "error": o["error"], "pull": (o["result"] - itg.exact) / o["error"]}
+        out["cores"] = cores
+        out["gpu_over_cpu_all_cores"] = out[f"cpu_oracle_{cores}_threads" if cores > 1 else
+                                            "cpu_oracle_1_thread"]["wall_ms"] / gpu["loop_ms"]
+    return out
+
+
 def mix6_bench(args, neval=None):
     """6-D four-component Gaussian mixture (32 parameters from the committed
     fixture, numpy default_rng(2)): 5e7 evaluations over 10 iterations (5e6 per
@@ -562,6 +612,8 @@ def main():
                                               "fp32_over_fp64", "fp32_vs_fp64", "roofline")}
         extras["mix6"] = mix6_bench(args)
         extras["batch512"] = batch_bench(args)
+        _log("cpu configuration")
+        extras["cpu_config"] = cpu_config_bench(args)
         out["configs"] = extras
     print(json.dumps(out), flush=True)
 
