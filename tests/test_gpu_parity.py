@@ -976,3 +976,35 @@ def test_oscillatory_16d_iterations_match_oracle(cv, orc):
     assert np.max(np.abs(S2 / o["hist_S2"] - 1)) < 1e-12, (S2, o["hist_S2"])
     assert np.mean(nh != o["nh"]) < 1e-4
 
+
+def test_mixture_forward_iterations_match_oracle(cv, orc):
+    """The 6-D four-component mixture of the autograd configuration at its full
+    forward size (1024 bins, 10 strata: 10^6 cubes, 5e6 evaluations per
+    iteration, theta from the committed fixture), FP64 (the project's exp on
+    both sides, so per-sample values are the same bits), four iterations on
+    the GPU and in the oracle: equal evaluation counts, I_k and sigma_k^2
+    within 1e-12 relative, the allocation equal in all but a handful of
+    cubes."""
+    import json
+    import os
+    from cuvegas_amd import integrands as I
+    with open(os.path.join(os.path.dirname(__file__), "golden", "workloads.json")) as f:
+        theta = json.load(f)["mixture_6d"]["theta"]
+    d, n, ns, N, iters = 6, 1024, 10, 5_000_000, 4
+    itg = I.gaussian_mixture_param(d, theta)
+    _say("mixture-forward-iterations")
+    st = cv.Integrator(itg.source, d, N, n_intervals=n, n_strat=ns, max_it=iters, skip=0, seed=0,
+                       precision="fp64", theta=theta)
+    st.run(iters)
+    I_k, S2, ne = st.history()
+    nh = st.get_nh()
+    st.close()
+    p, keep = orc.make_params(d, n, ns, N, iters, orc.F_MIXTURE, list(theta), skip=0, seed=0)
+    o = orc.run(p, keep, ns ** d, snapshots=False)
+    _say("mixture-differences", np.max(np.abs(I_k / o["hist_I"] - 1)), np.max(np.abs(S2 / o["hist_S2"] - 1)),
+         int(np.sum(nh != o["nh"])))
+    assert np.array_equal(ne, o["hist_n"])
+    assert np.max(np.abs(I_k / o["hist_I"] - 1)) < 1e-12, (I_k, o["hist_I"])
+    assert np.max(np.abs(S2 / o["hist_S2"] - 1)) < 1e-12, (S2, o["hist_S2"])
+    assert np.mean(nh != o["nh"]) < 1e-4
+
