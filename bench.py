@@ -206,6 +206,34 @@ def _workloads():
         return json.load(f)
 
 
+def _batch_fill_roofline(prog, items, theta, fams, iters, skip):
+    """Roofline of the batched sampling kernel: an eager run of the same batch
+    with HIP events around each iteration's one sampling launch; achieved =
+    the algorithmic VALU slots of the iteration's evaluations (each item's
+    slots per evaluation for its family and dimension, cuvegas_amd.roofline)
+    over the launch's duration, against the issue peak.  The kernel's own
+    overheads (runtime strata division, per-task dimension switch) are not
+    counted as work."""
+    import numpy as np
+    from cuvegas_amd.batch import Batch
+    from cuvegas_amd.roofline import slots_per_eval, valu_peak
+    fam_name = {"gaussian": "gaussian_peak", "oscillatory": "oscillatory", "corner": "corner_peak"}
+    S = np.array([slots_per_eval(d, fam_name[f]) for (d, _, _), f in zip(items, fams)], dtype=np.float64)
+    b = Batch(prog, items, theta, max_it=iters, skip=skip, seed=0)
+    b.set_timing(True)
+    b.run(iters, use_graph=False)
+    fill_ms, iter_ms = b.timing()
+    ne = np.array([b.history(i)[2] for i in range(len(items))], dtype=np.float64)  # [item, iteration]
+    b.close()
+    slots = (S[:, None] * ne).sum(axis=0)  # per iteration
+    peak = valu_peak(*_device_cu_clock())
+    achieved = float(slots.sum() / (fill_ms.sum() * 1e-3))
+    return {"bound": "valu", "kernel": "vg_bfill", "achieved": achieved / 1e12, "peak": peak / 1e12,
+            "unit": "Tslot/s", "frac": achieved / peak, "fill_ms_avg": float(fill_ms.mean()),
+            "fill_share_of_iteration": float(fill_ms.sum() / iter_ms.sum()),
+            "slots_per_eval_mean": float(slots.sum() / ne.sum())}
+
+
 def batch_bench(args, sequential=True):
     """512 independent integrals from the committed fixture (d ~ U{2..12},
     Gaussian / oscillatory / corner peak, budgets log-uniform in [1e4, 1e7] per
@@ -264,6 +292,7 @@ def batch_bench(args, sequential=True):
                                                        np.maximum(np.abs(exact), 1e-300)))},
         "autotune": tune["table"],
     }
+    out["roofline"] = _batch_fill_roofline(prog, items, theta, fams, iters, skip)
     if not sequential:
         return out
     # Sequential baseline: the same integrals one after another through the same

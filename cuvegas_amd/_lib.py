@@ -142,6 +142,8 @@ SIGNATURES = {
     "cvg_batch_run": (C.c_int, [_P, C.c_int, C.c_int]),
     "cvg_batch_results": (C.c_int, [_P, _PD, _PD, _PD, _PLL, _PD, _PD, _PD]),
     "cvg_batch_history": (C.c_int, [_P, C.c_int, _PD, _PD, _PLL, C.c_int]),
+    "cvg_batch_set_timing": (C.c_int, [_P, C.c_int]),
+    "cvg_batch_get_timing": (C.c_int, [_P, _PD, _PD, C.c_int]),
     "cvg_batch_item_info": (C.c_int, [_P, C.c_int, _PI, _PI, _PLL]),
     "cvg_batch_get_map": (C.c_int, [_P, C.c_int, _PD]),
     "cvg_batch_get_nh": (C.c_int, [_P, C.c_int, _PI]),

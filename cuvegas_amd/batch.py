@@ -81,6 +81,17 @@ class Batch:
         return {"result": res, "error": err, "chi2_dof": chi2, "nevals": ne, "loop_ms": ms.value,
                 "error_unbiased": erru, "chi2_dof_unbiased": chi2u}
 
+    def set_timing(self, enable: bool = True) -> None:
+        """Time the sampling launch and the whole iteration of eager runs
+        (run(..., use_graph=False)) with HIP events on the batch's stream."""
+        check(lib.cvg_batch_set_timing(self._h, 1 if enable else 0))
+
+    def timing(self):
+        """(fill_ms, iter_ms) of every timed iteration so far."""
+        f, t = np.zeros(self.max_it), np.zeros(self.max_it)
+        k = check(lib.cvg_batch_get_timing(self._h, _dptr(f), _dptr(t), self.max_it))
+        return f[:k], t[:k]
+
     def history(self, item: int):
         I, S2 = np.zeros(self.max_it), np.zeros(self.max_it)
         ne = np.zeros(self.max_it, dtype=np.int64)

@@ -1483,12 +1483,18 @@ struct cvg_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // optional per-iteration timing (eager launches): sampling launch and
+    // whole iteration, as for a single integral
+    int timing = 0;
+    std::vector<hipEvent_t> tev;
+    std::vector<double> fill_ms, iter_ms;
 
     ~cvg_batch() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         void* bufs[] = {fitems, bitems, task_item, task_w0, blk_item, rd_item, rd_dim, nh, nh_sum, JF, JF2,
                         dh, run_cube, win_first, win_last, part, dsum, bsum, boff, s_item, it_p, err,
                         hist, x_all, map64, map32, weights, counts, theta, fold_list, fold_count, big_list};
@@ -1533,10 +1539,14 @@ struct cvg_batch {
         return CVG_OK;
     }
 
-    int iteration() {
+    int iteration(hipEvent_t e_start = nullptr, hipEvent_t e_fill = nullptr, hipEvent_t e_end = nullptr) {
+        if (e_start) VG_HIP(hipEventRecord(e_start, stream));
         int rc = fill();
         if (rc) return rc;
-        return steps(1);
+        if (e_fill) VG_HIP(hipEventRecord(e_fill, stream));
+        if ((rc = steps(1))) return rc;
+        if (e_end) VG_HIP(hipEventRecord(e_end, stream));
+        return CVG_OK;
     }
 };
 
@@ -1815,10 +1825,20 @@ int cvg_batch_run(cvg_batch* b, int n_iter, int use_graph) {
         b->graph = g;
         VG_HIP(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
     }
+    const bool timed = b->timing && !use_graph;
+    if (timed) {
+        while ((int)b->tev.size() < 3 * n_iter) {
+            hipEvent_t e;
+            VG_HIP(hipEventCreate(&e));
+            b->tev.push_back(e);
+        }
+    }
     VG_HIP(hipEventRecord(b->ev0, b->stream));
     for (int i = 0; i < n_iter; ++i) {
         if (use_graph) {
             VG_HIP(hipGraphLaunch(b->graph_exec, b->stream));
+        } else if (timed) {
+            if ((rc = b->iteration(b->tev[3 * i], b->tev[3 * i + 1], b->tev[3 * i + 2]))) return rc;
         } else if ((rc = b->iteration())) {
             return rc;
         }
@@ -1829,11 +1849,36 @@ int cvg_batch_run(cvg_batch* b, int n_iter, int use_graph) {
     VG_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     b->loop_ms += ms;
     b->iterations_done += n_iter;
+    if (timed) {
+        for (int i = 0; i < n_iter; ++i) {
+            float a = 0.0f, c = 0.0f;
+            VG_HIP(hipEventElapsedTime(&a, b->tev[3 * i], b->tev[3 * i + 1]));
+            VG_HIP(hipEventElapsedTime(&c, b->tev[3 * i], b->tev[3 * i + 2]));
+            b->fill_ms.push_back(a);
+            b->iter_ms.push_back(c);
+        }
+    }
     int h_err = 0;
     VG_HIP(hipMemcpy(&h_err, b->err, sizeof(int), hipMemcpyDeviceToHost));
     if (h_err & VG_ERR_CAPACITY) return fail(CVG_ERR_CAPACITY, "an item's allocation exceeded its buffers");
     if (h_err & VG_ERR_HIST) return fail(CVG_ERR_CAPACITY, "more iterations than max_it");
     return CVG_OK;
+}
+
+int cvg_batch_set_timing(cvg_batch* b, int enable) {
+    if (!b) return fail(CVG_ERR_INVALID, "batch is NULL");
+    b->timing = enable ? 1 : 0;
+    return CVG_OK;
+}
+
+int cvg_batch_get_timing(cvg_batch* b, double* fill_ms, double* iter_ms, int cap) {
+    if (!b) return fail(CVG_ERR_INVALID, "batch is NULL");
+    const int n = (int)b->fill_ms.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        if (fill_ms) fill_ms[i] = b->fill_ms[i];
+        if (iter_ms) iter_ms[i] = b->iter_ms[i];
+    }
+    return n;
 }
 
 int cvg_batch_history(cvg_batch* b, int item, double* I, double* sigma2, long long* nevals, int cap) {

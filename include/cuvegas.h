@@ -263,6 +263,12 @@ int cvg_batch_run(cvg_batch* b, int n_iter, int use_graph);
 int cvg_batch_results(cvg_batch* b, double* result, double* error, double* chi2_dof,
                       long long* nevals, double* loop_ms, double* error_unbiased,
                       double* chi2_dof_unbiased);
+/* Per-iteration device timing of eager batch runs (use_graph = 0), as for a
+ * single integral: HIP events around the one sampling launch of the batch and
+ * around the whole iteration, on the batch's stream; get_timing returns the
+ * number of timed iterations (no reference counterpart). */
+int cvg_batch_set_timing(cvg_batch* b, int enable);
+int cvg_batch_get_timing(cvg_batch* b, double* fill_ms, double* iter_ms, int cap);
 int cvg_batch_history(cvg_batch* b, int item, double* I, double* sigma2, long long* nevals, int cap);
 int cvg_batch_item_info(cvg_batch* b, int item, int* n_dim, int* n_strat, long long* n_cubes);
 int cvg_batch_get_map(cvg_batch* b, int item, double* x_edges);

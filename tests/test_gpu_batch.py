@@ -253,3 +253,32 @@ def test_batch512_full_configuration(orc, prec):
     assert ok.sum() >= 400
     assert np.sqrt(np.mean(pull[ok] ** 2)) < 1.6, np.sqrt(np.mean(pull[ok] ** 2))
     assert np.mean(np.abs(pull[ok]) < 3) > 0.95
+
+
+def test_batch_timing_of_eager_runs():
+    """cvg_batch_set_timing / get_timing: an eager run records one (fill,
+    iteration) pair per iteration with 0 < fill <= iteration; graph replays
+    record none, and the results are those of an untimed run."""
+    from cuvegas_amd.batch import FAMILY_SOURCE, Batch, BatchProgram
+    items, theta, exact, fams = _fixture_items()
+    items = items[:24]
+    prog = BatchProgram(FAMILY_SOURCE, 12, 256, "fp32", run_len=16)
+    _say("batch-timing")
+    a = Batch(prog, items, theta, max_it=6, skip=2, seed=0)
+    a.set_timing(True)
+    a.run(3, use_graph=False)
+    f, t = a.timing()
+    assert len(f) == 3 and np.all(f > 0) and np.all(f <= t)
+    a.run(3, use_graph=True)
+    f2, _ = a.timing()
+    assert len(f2) == 3
+    ra = a.results()
+    a.close()
+    b = Batch(prog, items, theta, max_it=6, skip=2, seed=0)
+    b.run(3, use_graph=False)
+    b.run(3, use_graph=True)
+    rb = b.results()
+    b.close()
+    assert np.array_equal(ra["nevals"], rb["nevals"])
+    assert np.allclose(ra["result"], rb["result"], rtol=1e-9, atol=0)
+
