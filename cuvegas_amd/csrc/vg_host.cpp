@@ -626,10 +626,15 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
     // Copies of the LDS bins in the dimensions whose stratum digit a window's
     // cubes share (from the first dimension i with n_strat^i >= 256: a window
     // of 64 K samples holds fewer cubes than that digit's period once the
-    // allocation has adapted) when a stratum spans few bins (n / n_strat <= 32):
-    // the lanes of a wave then add into the few bins of one stratum, and lane l
-    // adds into copy l mod copies.  As many copies (8, 4 or 2) as fit in LDS.
-    // 4-D Gaussian, 1024 bins, 70 strata: dimensions 2 and 3, 4 copies.
+    // allocation has adapted) when a stratum spans few bins: the lanes of a
+    // wave then add into the bins of one stratum, and lane l adds into copy
+    // l mod copies.  As many copies (8, 4 or 2) as fit in LDS.  4-D Gaussian,
+    // 1024 bins, 70 strata (14.6 bins per stratum): dimensions 2 and 3, 4
+    // copies, both forms.  Up to 128 bins per stratum in the FP32 form, which
+    // the LDS co-bounds: 8-D double Gaussian, 1024 bins, 8 strata: dimensions
+    // 3-7, 2 copies, fill 1.5% faster in five paired same-box runs; the FP64
+    // form, held by its VALU work, ran 2% slower with them, so it takes copies
+    // only up to 32 bins per stratum.
     auto copies_lds = [&](int copies, int dim0) {
         return (long long)(copies - 1) * ((long long)(prog->d - dim0) * prog->nbins + 17) * (prog->bin32 ? 8 : 12);
     };
@@ -637,7 +642,8 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
     const bool copies_ok = prog->lds_bins && prog->bin_dims == prog->d;
     prog->bin_copies = 1;
     prog->copy_dim0 = prog->d;
-    if (copies_ok && prog->nbins <= 32LL * prog->nstrat) {
+    const long long span_max = prog->precision == CVG_FP32C ? 128 : 32;
+    if (copies_ok && prog->nbins <= span_max * prog->nstrat) {
         long long period = 1;
         int i0 = 0;
         while (i0 < prog->d && period < 256) { period *= prog->nstrat; ++i0; }
