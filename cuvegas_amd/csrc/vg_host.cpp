@@ -630,11 +630,12 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
     // wave then add into the bins of one stratum, and lane l adds into copy
     // l mod copies.  As many copies (8, 4 or 2) as fit in LDS.  4-D Gaussian,
     // 1024 bins, 70 strata (14.6 bins per stratum): dimensions 2 and 3, 4
-    // copies, both forms.  Up to 128 bins per stratum in the FP32 form, which
+    // copies, both forms.  Up to 256 bins per stratum in the FP32 form, which
     // the LDS co-bounds: 8-D double Gaussian, 1024 bins, 8 strata: dimensions
-    // 3-7, 2 copies, fill 1.5% faster in five paired same-box runs; the FP64
-    // form, held by its VALU work, ran 2% slower with them, so it takes copies
-    // only up to 32 bins per stratum.
+    // 3-7, 2 copies, fill 0.4-1.5% faster in paired same-box runs; 16-D
+    // oscillatory, 512 bins, 2 strata: dimensions 8-15, 2 copies, 1.9% faster.
+    // The FP64 form, held by its VALU work, ran 2% slower with them at the
+    // 8-D shape, so it takes copies only up to 32 bins per stratum.
     auto copies_lds = [&](int copies, int dim0) {
         return (long long)(copies - 1) * ((long long)(prog->d - dim0) * prog->nbins + 17) * (prog->bin32 ? 8 : 12);
     };
@@ -642,7 +643,7 @@ int cvg_program_create(const char* integrand_src, const cvg_params* shape, const
     const bool copies_ok = prog->lds_bins && prog->bin_dims == prog->d;
     prog->bin_copies = 1;
     prog->copy_dim0 = prog->d;
-    const long long span_max = prog->precision == CVG_FP32C ? 128 : 32;
+    const long long span_max = prog->precision == CVG_FP32C ? 256 : 32;
     if (copies_ok && prog->nbins <= span_max * prog->nstrat) {
         long long period = 1;
         int i0 = 0;
