@@ -145,8 +145,10 @@ __device__ inline bool vg_last_block(unsigned* cnt, int* flag) {
 // j1 = sum J f, j2 = sum (J f)^2 over neval >= 2 samples, in the reference's
 // expression order (cuda_wrapper.cu:33-47).  (A branch that wrote the
 // divisions by n_h = 2 as products by 0.5 -- the same bits -- left the results
-// launch unchanged, 125 against 123 us at 16.8 M cubes: it is bound by memory
-// and latency, not by the divide sequences.)
+// launch unchanged, 125 against 123 us at 16.8 M cubes, and so did the same
+// branch taken only by waves whose cubes all hold 2 samples (112-118 against
+// 115-119 us after adaptation, where most waves take it): it is bound by
+// memory and latency, not by the divide sequences.)
 __device__ inline void vg_cube_terms(double j1, double j2, int neval, double V, double beta, double& sr,
                                      double& ss, double& dd, double& su) {
     const double Ih = j1 / neval * V;
