@@ -492,7 +492,7 @@ def single_bench(args, workload, warmup, steps, with_cpu=True, neval=None):
     sample.  Returns the result dict (the bench line's fields)."""
     from cuvegas_amd import integrands as I
     from cuvegas_amd import lib
-    from cuvegas_amd.roofline import slots_per_eval, valu_peak
+    from cuvegas_amd.roofline import device_rates, lds_roofline, slots_per_eval, valu_peak
     import ctypes as C
 
     seed = 0
@@ -547,6 +547,11 @@ def single_bench(args, workload, warmup, steps, with_cpu=True, neval=None):
             "frac": achieved / peak, "slots_per_eval": S,
             "traffic": _traffic_from_profile(workload) if head is forms.get("fp32") else None,
             "fill_ms_avg": head["fill_ms_avg"], "evals_per_fill": head["evals_per_fill_avg"]}
+    # the same launch against the LDS: d bin updates per evaluation at the rate the
+    # device's LDS sustains them at random addresses (measured in this run)
+    rates = device_rates()
+    roof["lds"] = lds_roofline(head["evals_per_fill_avg"], d, head["fill_ms_avg"] * 1e-3, n_cu.value,
+                               clock_hz, rates["lds_pair_random"])
     if roof["traffic"]:
         # the same launch against the HBM roofline: measured bytes per launch / its duration
         from cuvegas_amd.roofline import MI355X_HBM_BPS
@@ -574,7 +579,8 @@ def single_bench(args, workload, warmup, steps, with_cpu=True, neval=None):
         "fp32_vs_fp64": _form_difference(forms),
         "roofline": roof,
         "cpu_baseline": cpu,
-        "device": {"arch": arch.value.decode(), "n_cu": n_cu.value, "clock_hz": clock_hz},
+        "device": {"arch": arch.value.decode(), "n_cu": n_cu.value, "clock_hz": clock_hz,
+                   "cycles_per_wave64_instruction": rates},
     }
 
 

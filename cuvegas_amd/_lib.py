@@ -99,6 +99,7 @@ _PLL = C.POINTER(C.c_longlong)
 SIGNATURES = {
     "cvg_last_error": (C.c_char_p, []),
     "cvg_device_info": (C.c_int, [_PI, _PI, C.c_char_p, C.c_int]),
+    "cvg_device_rates": (C.c_int, [_PD, C.c_int]),
     "cvg_program_create": (C.c_int, [C.c_char_p, C.POINTER(cvg_params), C.c_char_p, C.POINTER(_P)]),
     "cvg_program_log": (C.c_char_p, [_P]),
     "cvg_program_code_size": (C.c_longlong, [_P]),

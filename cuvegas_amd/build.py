@@ -4,8 +4,8 @@ Steps:
   1. embed vg_fill.hip, vg_philox.h and vg_math.h as C string literals (csrc/vg_embed.inc),
      so the library carries the sampling-kernel source it hands to hipRTC;
   2. compile the ahead-of-time step kernels (vg_steps.hip), the backward replay
-     entry point that allocates nothing (vg_noalloc.hip) and the host driver
-     (vg_host.cpp);
+     entry point that allocates nothing (vg_noalloc.hip), the probes of the
+     device's instruction costs (vg_rates.hip) and the host driver (vg_host.cpp);
   3. link the shared library against libhiprtc.
 
 Usage: python cuvegas_amd/build.py   (run as a file: importing the package needs the library)
@@ -66,13 +66,15 @@ def build(verbose: bool = True) -> str:
     steps_o = os.path.join(BUILD, "vg_steps.o")
     host_o = os.path.join(BUILD, "vg_host.o")
     noalloc_o = os.path.join(BUILD, "vg_noalloc.o")
+    rates_o = os.path.join(BUILD, "vg_rates.o")
     _run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, "vg_steps.hip"), "-o", steps_o])
     _run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, "vg_noalloc.hip"), "-o", noalloc_o])
+    _run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, "vg_rates.hip"), "-o", rates_o])
     _run([HIPCC, "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
           "-Wno-unused-result", "-c", "-x", "c++", os.path.join(CSRC, "vg_host.cpp"),
           "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-o", host_o])
     tmp = OUT + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", steps_o, noalloc_o, host_o,
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", steps_o, noalloc_o, rates_o, host_o,
           "-o", tmp, "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, OUT)
     return OUT

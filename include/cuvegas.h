@@ -109,6 +109,24 @@ const char* cvg_last_error(void);
 /* Device facts used for roofline figures. */
 int cvg_device_info(int* n_cu, int* clock_khz, char* arch, int arch_len);
 
+/* Measured issue costs of the sampling kernel's instructions on the current
+ * device, in cycles of the nominal clock per wave64 instruction (CU cycles for
+ * the LDS, SIMD cycles for the VALU): the ceilings its roofline is read
+ * against.  No counterpart in the reference (it reports no roofline).  About
+ * 30 ms of probe kernels on a private stream; allocates 4 MB for their output.
+ * n >= CVG_RATE_COUNT; indices: */
+#define CVG_RATE_LDS_F64_RANDOM 0 /* ds_add_f64, 64 lanes at random slots of 1024 */
+#define CVG_RATE_LDS_U32_RANDOM 1 /* ds_add_u32, the same slots */
+#define CVG_RATE_LDS_PAIR_RANDOM 2 /* one of each per slot (the bin update), per instruction */
+#define CVG_RATE_LDS_F64_SEQ 3    /* ds_add_f64, 64 consecutive slots: no bank conflicts */
+#define CVG_RATE_LDS_U32_SEQ 4    /* ds_add_u32, 64 consecutive slots */
+#define CVG_RATE_VALU_F32_FMA 5   /* v_fma_f32 */
+#define CVG_RATE_VALU_F64_FMA 6   /* v_fma_f64 */
+#define CVG_RATE_VALU_U32_ADD 7   /* v_add_u32 and v_lshrrev_b32, per instruction */
+#define CVG_RATE_VALU_MAD_U64 8   /* v_mad_u64_u32 (Philox's 32x32 -> 64 product) */
+#define CVG_RATE_COUNT 9
+int cvg_device_rates(double* cycles, int n);
+
 /* Compile an integrand source (HIP C++) for the given shape with hipRTC.
  * shape supplies n_dim, n_intervals, n_strat, precision, run_len, has_domain.
  * options: space-separated launch knobs and extra hipRTC options, or NULL.

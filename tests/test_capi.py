@@ -212,6 +212,25 @@ def test_roofline_slot_counts():
     assert valu_peak(256, 2.4e9) == pytest.approx(3.93216e13)
 
 
+def test_lds_roofline_and_rate_arguments():
+    """The LDS roofline of a sampling launch: d bin updates (a ds_add_f64 and
+    a ds_add_u32) per evaluation against the CUs' rate for the pair; 1.33e8
+    evaluations of an 8-D integrand in 1.25 ms at 16.7 cycles per instruction
+    on 256 CUs at 2.4 GHz is 8.52e11 of 1.177e12 updates/s.  cvg_device_rates
+    refuses an output array shorter than CVG_RATE_COUNT before touching a
+    device."""
+    from cuvegas_amd._lib import CVG_ERR_INVALID, lib
+    from cuvegas_amd.roofline import RATE_NAMES, lds_roofline
+    r = lds_roofline(1.33e8, 8, 1.25e-3, 256, 2.4e9, 16.7)
+    assert r["achieved"] == pytest.approx(1.33e8 * 8 / 1.25e-3 / 1e9)
+    assert r["peak"] == pytest.approx(256 * 2.4e9 * 64 / 33.4 / 1e9)
+    assert r["frac"] == pytest.approx(r["achieved"] / r["peak"])
+    assert len(RATE_NAMES) == 9
+    buf = (C.c_double * 8)()
+    assert lib.cvg_device_rates(buf, 8) == CVG_ERR_INVALID
+    assert b"CVG_RATE_COUNT" in lib.cvg_last_error()
+
+
 def test_combine_rule():
     """cuvegas_amd.combine: the reference's inverse-variance mean of the
     iterations >= skip (cuda_wrapper.cu:758-766) with chi2/dof, checked on

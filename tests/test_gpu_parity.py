@@ -1008,3 +1008,18 @@ def test_mixture_forward_iterations_match_oracle(cv, orc):
     assert np.max(np.abs(S2 / o["hist_S2"] - 1)) < 1e-12, (S2, o["hist_S2"])
     assert np.mean(nh != o["nh"]) < 1e-4
 
+
+
+def test_device_rates_plausible(cv):
+    """cvg_device_rates on the GPU: every cost is a finite number of cycles
+    per wave64 instruction in a sane range, and atomics at random LDS slots
+    cost more than at 64 consecutive slots (bank conflicts)."""
+    from cuvegas_amd.roofline import device_rates
+    _say("device rates")
+    r = device_rates()
+    _say(r)
+    for name, v in r.items():
+        assert math.isfinite(v) and 0.5 < v < 2000.0, (name, v)
+    assert r["lds_f64_random"] > r["lds_f64_seq"]
+    assert r["lds_u32_random"] >= r["lds_u32_seq"] * 0.9
+    assert r["lds_pair_random"] > r["lds_u32_seq"]
