@@ -6,7 +6,9 @@ Steps:
   2. compile the ahead-of-time step kernels (vg_steps.hip), the backward replay
      entry point that allocates nothing (vg_noalloc.hip), the probes of the
      device's instruction costs (vg_rates.hip) and the host driver (vg_host.cpp);
-  3. link the shared library against libhiprtc.
+  3. link the shared library against libhiprtc;
+  4. build the standalone program cuvegas_main (vg_main.cpp, with the integrand
+     library vg_functions.h embedded) against the shared library.
 
 Usage: python cuvegas_amd/build.py   (run as a file: importing the package needs the library)
 """
@@ -19,6 +21,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcuvegas_amd.so")
+MAIN = os.path.join(HERE, "cuvegas_main")
 BUILD = os.path.join(HERE, "_build")
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -77,6 +80,12 @@ def build(verbose: bool = True) -> str:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", steps_o, noalloc_o, rates_o, host_o,
           "-o", tmp, "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, OUT)
+    # 4. the standalone program (the reference's src/main.cu caller) over the C ABI
+    _embed([("kVgFunctionsSrc", os.path.join(CSRC, "vg_functions.h"))], os.path.join(CSRC, "vg_main_embed.inc"))
+    tmp = MAIN + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "vg_main.cpp"), "-o", tmp,
+          f"-L{HERE}", "-lcuvegas_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, MAIN)
     return OUT
 
 

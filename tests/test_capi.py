@@ -231,6 +231,46 @@ def test_lds_roofline_and_rate_arguments():
     assert b"CVG_RATE_COUNT" in lib.cvg_last_error()
 
 
+def _run_main(*args, cwd=None):
+    import subprocess
+    exe = os.path.join(ROOT, "cuvegas_amd", "cuvegas_main")
+    return subprocess.run([exe, *args], capture_output=True, text=True, timeout=120, cwd=cwd)
+
+
+def test_standalone_main_reads_reference_config():
+    """cuvegas_main (the reference's standalone src/main.cu caller) reads a
+    configuration header in the reference's format: the settings from their
+    initialisers, the reference's sizing (n_eval_it = tot_eval / max_it,
+    n_strat = (int)(n_eval_it / 4)^(1/d): 1e7 / 20 in 4-D gives 18 strata,
+    104,976 cubes) and the integrand, which hipRTC compiles with the library
+    standing in for commons/functions.cuh (no GPU needed)."""
+    r = _run_main(os.path.join(ROOT, "tests", "golden", "config_ridge4.cuh"), "--compile-only")
+    assert r.returncode == 0, r.stderr
+    lines = dict(l.split(": ", 1) for l in r.stdout.splitlines() if ": " in l)
+    assert lines["Strat"] == "18" and lines["Cubes"] == "104976" and lines["Intervals"] == "1024"
+    assert lines["Evaluations per iteration"] == "500000" and lines["Iterations"] == "20 (skip 5)"
+    assert lines["Alpha"] == "0.5" and lines["Beta"] == "0.75" and lines["Seed"] == "123"
+
+
+def test_standalone_main_refuses_bad_configs(tmp_path):
+    """An include other than the integrand library or the C maths/I-O headers,
+    a missing n_dim, unusable settings and an integrand hipRTC rejects are
+    reported on stderr with a non-zero status."""
+    cases = {
+        "inc.cuh": '#include "mystuff.h"\nconst int n_dim = 2;\n__host__ __device__ double integrand(double* x) { return 1.0; }\n',
+        "nodim.cuh": "__host__ __device__ double integrand(double* x) { return 1.0; }\n",
+        "skip.cuh": "int max_it = 5;\nint skip = 5;\nconst int n_dim = 2;\n__host__ __device__ double integrand(double* x) { return 1.0; }\n",
+        "bad.cuh": "const int n_dim = 2;\n__host__ __device__ double integrand(double* x) { return nope(x); }\n",
+    }
+    for name, text in cases.items():
+        p = tmp_path / name
+        p.write_text(text)
+        r = _run_main(str(p), "--compile-only")
+        assert r.returncode != 0 and r.stderr.strip(), (name, r.stdout)
+    assert "unsupported include" in _run_main(str(tmp_path / "inc.cuh"), "--compile-only").stderr
+    assert "nope" in _run_main(str(tmp_path / "bad.cuh"), "--compile-only").stderr
+
+
 def test_combine_rule():
     """cuvegas_amd.combine: the reference's inverse-variance mean of the
     iterations >= skip (cuda_wrapper.cu:758-766) with chi2/dof, checked on

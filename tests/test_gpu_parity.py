@@ -1023,3 +1023,33 @@ def test_device_rates_plausible(cv):
     assert r["lds_f64_random"] > r["lds_f64_seq"]
     assert r["lds_u32_random"] >= r["lds_u32_seq"] * 0.9
     assert r["lds_pair_random"] > r["lds_u32_seq"]
+
+
+def test_standalone_main_runs_reference_config():
+    """cuvegas_main on a configuration in the reference's header format (its
+    default run: 4-D ridge, tot_eval 1e7 over 20 iterations, 5 skipped, 1024
+    bins, seed 123) prints the reference program's lines in its order
+    (src/main.cu:292,428-430,791-797) and a result within 5 sigma (and 0.5%)
+    of the closed form 0.851317758241297887; the graph replay and the eager
+    loop give the same numbers."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "cuvegas_amd", "cuvegas_main")
+    cfg = os.path.join(root, "tests", "golden", "config_ridge4.cuh")
+    outs = []
+    for extra in ([], ["--eager"]):
+        _say("cuvegas_main", cfg, *extra)
+        r = subprocess.run([exe, cfg, *extra], capture_output=True, text=True, timeout=110)
+        assert r.returncode == 0, r.stderr
+        outs.append(r.stdout)
+    lines = outs[0].splitlines()
+    want = ["GPUs: ", "Strat: ", "Cubes: ", "Intervals: ", "Result: ", "Error: ", "Total evals: ",
+            "Time elapsed ", "Iteration avg time ", "Nocontext ", "Context creation "]
+    assert len(lines) >= len(want) and all(l.startswith(w) for l, w in zip(lines, want)), lines
+    vals = [dict(l.split(": ", 1) for l in o.splitlines() if ": " in l) for o in outs]
+    res, err = float(vals[0]["Result"]), float(vals[0]["Error"])
+    exact = 0.851317758241297887
+    assert abs(res - exact) < 5 * err and abs(res / exact - 1) < 5e-3, (res, err)
+    assert int(vals[0]["Total evals"]) > 9_000_000
+    assert vals[0]["Result"] == vals[1]["Result"] and vals[0]["Error"] == vals[1]["Error"]
