@@ -271,6 +271,68 @@ def test_standalone_main_refuses_bad_configs(tmp_path):
     assert "nope" in _run_main(str(tmp_path / "bad.cuh"), "--compile-only").stderr
 
 
+def test_standalone_integrand_library_closed_forms(tmp_path):
+    """vg_functions.h (the reference's integrand library as the standalone
+    program compiles it, functions.cu:5-111) built as host code and
+    integrated by the midpoint rule against each function's closed form in
+    1-D (every function but sinexp) and 2-D.  A wrong constant, sign or
+    operand shows up as a missed integral."""
+    import math
+    import subprocess
+    src = tmp_path / "fn.cpp"
+    src.write_text("""
+#define __host__
+#define __device__
+#include <math.h>
+#include "vg_functions.h"
+typedef double (*fn)(double*, int);
+static const fn table[] = {linear, roosarnold, morokoffcalfisch, hellekalek, ridge, expon, cosin,
+                           dirichlet, gaussian, sinexp, one};
+extern "C" double vg_quad(int k, int d, int m) {  // midpoint rule, m points per dimension
+    double x[8], s = 0.0;
+    long long n = 1;
+    for (int i = 0; i < d; ++i) n *= m;
+    for (long long c = 0; c < n; ++c) {
+        long long t = c;
+        for (int i = 0; i < d; ++i) { x[i] = ((double)(t % m) + 0.5) / m; t /= m; }
+        s += table[k](x, d);
+    }
+    return s / (double)n;
+}
+""")
+    so = tmp_path / "libfn.so"
+    subprocess.run(["g++", "-O2", "-shared", "-fPIC", "-I", os.path.join(ROOT, "cuvegas_amd", "csrc"),
+                    str(src), "-o", str(so)], check=True)
+    q = C.CDLL(str(so)).vg_quad
+    q.restype, q.argtypes = C.c_double, [C.c_int, C.c_int, C.c_int]
+    names = ["linear", "roosarnold", "morokoffcalfisch", "hellekalek", "ridge", "expon", "cosin",
+             "dirichlet", "gaussian", "sinexp", "one"]
+    erfm = lambda mu, sg: 0.5 * (math.erf((1 - mu) / (sg * math.sqrt(2))) + math.erf(mu / (sg * math.sqrt(2))))
+
+    def ridge_exact(d):
+        return (100 / math.pi) ** 2 / 1000 * sum(
+            (math.sqrt(math.pi / 100) * 0.5 * (math.erf(10 * (1 - k / 999)) + math.erf(10 * k / 999))) ** d
+            for k in range(1000))
+
+    exact = {
+        "linear": lambda d: d / 2, "roosarnold": lambda d: 1.0, "morokoffcalfisch": lambda d: 1.0,
+        "hellekalek": lambda d: 0.0, "ridge": ridge_exact,
+        "expon": lambda d: (math.sqrt(math.pi) / 2 * math.erf(1)) ** d, "cosin": lambda d: math.sin(1) ** d,
+        "dirichlet": lambda d: 0.2 ** d * math.gamma(5 * d) / math.gamma(5) ** d,
+        "gaussian": lambda d: erfm(0.5, 0.01) ** d, "sinexp": lambda d: (1 - math.cos(1)) + (math.e - 1),
+        "one": lambda d: 1.0}
+    for k, name in enumerate(names):
+        for d, m in ((1, 20000 if name == "ridge" else 200000), (2, 300 if name == "ridge" else 1000)):
+            if name == "sinexp" and d == 1:
+                continue
+            got, want = q(k, d, m), exact[name](d)
+            tol = 2e-3 if name in ("morokoffcalfisch", "gaussian") and d == 2 else 1e-4
+            assert abs(got - want) <= tol * max(1.0, abs(want)), (name, d, got, want)
+    # the closed form used above gives the reference's default run its value
+    # (4-D ridge, 0.851317758241297887)
+    assert ridge_exact(4) == pytest.approx(0.851317758241297887, rel=1e-13)
+
+
 def test_combine_rule():
     """cuvegas_amd.combine: the reference's inverse-variance mean of the
     iterations >= skip (cuda_wrapper.cu:758-766) with chi2/dof, checked on
