@@ -202,6 +202,44 @@ def test_sampling_loop_variants_bit_exact(cv, orc, prec, options, K):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("chunk", [0, 2, 8])
+@pytest.mark.parametrize("K", [1, 2, 4, 8, 16])
+def test_preloaded_run_bounds_bit_exact(cv, orc, prec, chunk, K):
+    """The pipelined loop with each run's cube boundaries read before the
+    loop and the closed cubes' sums stored between chunks of steps (chunk 0:
+    boundaries read and sums stored as the loop reaches them) gives the
+    oracle's per-cube sums bit for bit, for every run length it takes (1 to
+    16 samples): cubes of 2 samples, so a run crosses its most boundaries,
+    next to cubes spanning windows, and a partial last run."""
+    from cuvegas_amd import integrands as I
+    d, n, ns, N = 3, 32, 5, 100000
+    itg = I.poly_peak(d)
+    opts = "pipe=1 " + ("-DVG_RUN_BOUNDS=0" if chunk == 0 else f"-DVG_RUN_BOUNDS=1 -DVG_RB_CHUNK={chunk}")
+    _say("run-bounds", prec, opts, K)
+    p, keep = _orc_params(orc, itg, d, n, ns, N, 1, 0 if prec == "fp64" else 1, K=K)
+    st = _integrator(cv, itg, d, n, ns, N, 2, prec, K=K, options=opts)
+    rng = np.random.default_rng(7)
+    nh0 = np.where(rng.random(st.n_cubes) < 0.1, rng.integers(300, 2000, st.n_cubes), 2)
+    nh0[-1] = 3  # an odd last cube: a partial last run
+    st.set_nh(nh0)
+    S = int(nh0.sum())
+    x0 = st.get_map()
+    _, ids, _, ojf = orc.sample_dump(p, keep, x0, nh0, 0, S)
+    oJF, oJF2 = orc.cube_sums(nh0, ojf, ojf * ojf, K)
+    st.fill()
+    _, c = st.get_bins()
+    oc = np.stack([np.bincount(ids[:, i], minlength=n) for i in range(d)]).astype(np.uint32)
+    assert np.array_equal(c, oc)
+    # per-cube sums need the window folds of a full iteration: fresh state, same program
+    st2 = cv.Integrator(itg.source, d, N, n_intervals=n, n_strat=ns, max_it=2, skip=1, seed=0,
+                        precision=prec, run_len=K, program=st.program)
+    st2.set_nh(nh0)
+    st2.run(1, use_graph=False)
+    JF, JF2 = st2.get_cube_sums()
+    assert np.array_equal(JF, oJF) and np.array_equal(JF2, oJF2)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("shape", [(14, 1024, 2, 60_000), (16, 1024, 2, 120_000)])
 def test_bins_in_dimension_groups(cv, orc, prec, shape):
     """Bins larger than LDS (d x 1024 x 12 B > 160 KB): the fill runs once per
