@@ -252,7 +252,13 @@ def batch_bench(args, sequential=True):
         _log("autotune")
         tune = autotune_batch(FAMILY_SOURCE, items, theta, max_dim=12, n_intervals=256, precision="fp32",
                               use_cache=False, verbose=True)
-    prog = program(FAMILY_SOURCE, 12, 256, "fp32", tune["block"], tune["run_len"], tune["task_windows"])
+    extra = (getattr(args, "fill_options", "") or "").strip()
+    if extra:  # same-box A/B of program options: a program of its own
+        from cuvegas_amd.batch import BatchProgram
+        prog = BatchProgram(FAMILY_SOURCE, 12, 256, "fp32", run_len=tune["run_len"],
+                            options=f"block={tune['block']} task_windows={tune['task_windows']} {extra}")
+    else:
+        prog = program(FAMILY_SOURCE, 12, 256, "fp32", tune["block"], tune["run_len"], tune["task_windows"])
     _log("batched run", tune["block"], tune["run_len"], tune["task_windows"])
     b = Batch(prog, items, theta, max_it=iters, skip=skip, seed=0)
     t0 = time.perf_counter()
@@ -596,7 +602,7 @@ def main():
                     help="dg8: leave out the osc16, mix6 and batch512 workloads that the default line carries")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
     ap.add_argument("--fill-options", default="",
-                    help="dg8/osc16/gauss4/mix6: sampling-program options (e.g. 'bin_copies=1') for A/B runs")
+                    help="sampling-program options (e.g. 'bin_copies=1') for A/B runs")
     ap.add_argument("--run-len", type=int, default=0,
                     help="dg8/osc16/gauss4: samples per lane run (power of two; default: the library's 16)")
     ap.add_argument("--batch-config", default="",
