@@ -120,11 +120,14 @@ constexpr long long NCUBES = vg_ipow(NS, D);
 // gathers: loads and stores share one counter, so a boundary load or a sum
 // store inside the loop makes every wait for a gather wait for it as well.
 // On by default in the FP32 form: 4-D Gaussian (1e8 samples, 24 M cubes of
-// about 5 samples) fill 0.789 -> 0.750 ms, 8-D double Gaussian unchanged
+// about 5 samples) fill 0.789 -> 0.756 ms, 8-D double Gaussian unchanged
 // within 1% (same-box runs).  The FP64 form, held at 168 registers by its
 // launch bound, ran 2% slower at the 8-D shape with it and keeps the loads
-// and stores in the loop.  Chunks of 2 steps (one sum held) did at least as
-// well as chunks of 4 or 8.
+// and stores in the loop.  The chunk is the whole run by default: the sums of
+// a run's cubes then leave in one burst at its end, so the 128-byte lines of
+// per-cube sums are written whole instead of over the run's lifetime (the
+// fill's written bytes at the 8-D headline 1.54x -> 1.11x the per-cube sums,
+// 4-D Gaussian 1.67x -> 1.05x, against chunks of 2 steps) at the same speed.
 #ifndef VG_RUN_BOUNDS
 #define VG_RUN_BOUNDS (VG_PIPE && !VG_PAIR && VG_K <= 16 && VG_FP32)
 #endif
@@ -132,7 +135,7 @@ constexpr long long NCUBES = vg_ipow(NS, D);
 #error "VG_RUN_BOUNDS needs the pipelined loop (pipe=1) and runs of at most 16 samples"
 #endif
 #ifndef VG_RB_CHUNK
-#define VG_RB_CHUNK 2
+#define VG_RB_CHUNK 16
 #endif
 constexpr int RB_N = K >= 2 ? K / 2 : 1;                         // boundaries a run can reach
 constexpr int RB_CHUNK = K < VG_RB_CHUNK ? K : VG_RB_CHUNK;      // steps per chunk

@@ -202,12 +202,13 @@ def test_sampling_loop_variants_bit_exact(cv, orc, prec, options, K):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("chunk", [0, 2, 8])
+@pytest.mark.parametrize("chunk", [0, 2, 16])
 @pytest.mark.parametrize("K", [1, 2, 4, 8, 16])
 def test_preloaded_run_bounds_bit_exact(cv, orc, prec, chunk, K):
     """The pipelined loop with each run's cube boundaries read before the
-    loop and the closed cubes' sums stored between chunks of steps (chunk 0:
-    boundaries read and sums stored as the loop reaches them) gives the
+    loop and the closed cubes' sums stored between chunks of steps (chunk 16:
+    the whole run, the default; chunk 0: boundaries read and sums stored as
+    the loop reaches them) gives the
     oracle's per-cube sums bit for bit, for every run length it takes (1 to
     16 samples): cubes of 2 samples, so a run crosses its most boundaries,
     next to cubes spanning windows, and a partial last run."""
