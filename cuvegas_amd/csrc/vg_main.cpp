@@ -61,9 +61,11 @@ std::string strip_comments(const std::string& s) {
 
 bool ident_char(char c) { return std::isalnum((unsigned char)c) || c == '_'; }
 
-// Value of the initialiser "key = <number>;" (the first declaration of key as
-// a whole word followed by '='); false if there is none.
-bool find_setting(const std::string& text, const char* key, double* out) {
+// Value of the initialiser "key = <number>;" (the first occurrence of key as a
+// whole word followed by '='): 1 with *out set, 0 if there is none, -1 if the
+// initialiser is not a plain number (an expression, which this program does
+// not evaluate).
+int find_setting(const std::string& text, const char* key, double* out) {
     const size_t n = std::strlen(key);
     for (size_t p = text.find(key); p != std::string::npos; p = text.find(key, p + 1)) {
         if (p > 0 && ident_char(text[p - 1])) continue;
@@ -74,15 +76,15 @@ bool find_setting(const std::string& text, const char* key, double* out) {
         const char* start = text.c_str() + q + 1;
         char* end = nullptr;
         const double v = std::strtod(start, &end);
-        if (end == start) return false;
+        if (end == start) return -1;
         while (*end && std::isspace((unsigned char)*end)) ++end;
         while (*end == 'l' || *end == 'L' || *end == 'u' || *end == 'U' || *end == 'f' || *end == 'F') ++end;
         while (*end && std::isspace((unsigned char)*end)) ++end;
-        if (*end != ';') return false;
+        if (*end != ';') return -1;
         *out = v;
-        return true;
+        return 1;
     }
-    return false;
+    return 0;
 }
 
 // The configuration as hipRTC source: an include of the integrand library is
@@ -160,14 +162,20 @@ int main(int argc, char** argv) {
     // settings, with the values of the reference's own header where one is missing
     double tot_eval = 1e7, max_it = 20, skip = 5, n_intervals = 1024, alpha = 0.5, beta = 0.75, n_dim = 0,
            max_batch = 0;
-    find_setting(text, "tot_eval", &tot_eval);
-    find_setting(text, "max_it", &max_it);
-    find_setting(text, "skip", &skip);
-    find_setting(text, "n_intervals", &n_intervals);
-    find_setting(text, "alpha", &alpha);
-    find_setting(text, "beta", &beta);
-    find_setting(text, "max_batch_size", &max_batch);  // read and reported: every sample of an iteration is one launch here
-    if (!find_setting(text, "n_dim", &n_dim) || n_dim < 1 || n_dim > 64 || n_dim != std::floor(n_dim)) {
+    struct {
+        const char* key;
+        double* val;
+    } settings[] = {{"tot_eval", &tot_eval}, {"max_it", &max_it}, {"skip", &skip},
+                    {"n_intervals", &n_intervals}, {"alpha", &alpha}, {"beta", &beta},
+                    {"max_batch_size", &max_batch},  // reported only: an iteration is one launch here
+                    {"n_dim", &n_dim}};
+    for (const auto& st : settings) {
+        if (find_setting(text, st.key, st.val) < 0) {
+            std::fprintf(stderr, "%s: the initialiser of %s is not a plain number\n", path, st.key);
+            return 2;
+        }
+    }
+    if (n_dim < 1 || n_dim > 64 || n_dim != std::floor(n_dim)) {
         std::fprintf(stderr, "%s: no usable 'const int n_dim = <1..64>;'\n", path);
         return 2;
     }

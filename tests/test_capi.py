@@ -254,13 +254,15 @@ def test_standalone_main_reads_reference_config():
 
 def test_standalone_main_refuses_bad_configs(tmp_path):
     """An include other than the integrand library or the C maths/I-O headers,
-    a missing n_dim, unusable settings and an integrand hipRTC rejects are
+    a missing n_dim, unusable settings, a setting initialised by an
+    expression (not evaluated here) and an integrand hipRTC rejects are
     reported on stderr with a non-zero status."""
     cases = {
         "inc.cuh": '#include "mystuff.h"\nconst int n_dim = 2;\n__host__ __device__ double integrand(double* x) { return 1.0; }\n',
         "nodim.cuh": "__host__ __device__ double integrand(double* x) { return 1.0; }\n",
         "skip.cuh": "int max_it = 5;\nint skip = 5;\nconst int n_dim = 2;\n__host__ __device__ double integrand(double* x) { return 1.0; }\n",
         "bad.cuh": "const int n_dim = 2;\n__host__ __device__ double integrand(double* x) { return nope(x); }\n",
+        "expr.cuh": "long int tot_eval = 10 * 1000000;\nconst int n_dim = 2;\n__host__ __device__ double integrand(double* x) { return 1.0; }\n",
     }
     for name, text in cases.items():
         p = tmp_path / name
@@ -269,6 +271,7 @@ def test_standalone_main_refuses_bad_configs(tmp_path):
         assert r.returncode != 0 and r.stderr.strip(), (name, r.stdout)
     assert "unsupported include" in _run_main(str(tmp_path / "inc.cuh"), "--compile-only").stderr
     assert "nope" in _run_main(str(tmp_path / "bad.cuh"), "--compile-only").stderr
+    assert "tot_eval is not a plain number" in _run_main(str(tmp_path / "expr.cuh"), "--compile-only").stderr
 
 
 def test_standalone_integrand_library_closed_forms(tmp_path):
