@@ -995,6 +995,17 @@ __device__ inline void vg_beval(double& jf, unsigned* ids, long long s, const un
     jf = (double)(f * jac);
 }
 
+// Copies of the block's LDS bins (VG_BCOPIES): lane l adds into copy l mod
+// VG_BCOPIES, copies VG_BXS entries apart (an odd count, so that one bin's
+// copies sit in different banks), summed in copy order at the task's flush.
+// With 256 bins per dimension the 64 lanes of a wave meet in one bin often
+// (and in all of them when a window's cubes share a stratum); the copies
+// divide those collisions.
+#ifndef VG_BCOPIES
+#define VG_BCOPIES 1
+#endif
+constexpr int VG_BXS = MD * NB + 17;
+
 // Windows per wave per task: a task covers VG_TASKW * (VG_BLOCK / 64) windows.
 #ifndef VG_TASKW
 #define VG_TASKW 8
@@ -1012,6 +1023,7 @@ __device__ inline void vg_btask(const vg_item& I, int item, long long w0, long l
                                 unsigned* lc) {
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
+    const int cofs = VG_BCOPIES > 1 ? (lane % VG_BCOPIES) * VG_BXS : 0;  // this lane's bin copy
     for (int tw = 0; tw < VG_TASKW; ++tw) {
         const long long w = w0 + wid + (long long)tw * WPB;
         if (w >= nwin) continue;
@@ -1052,8 +1064,8 @@ __device__ inline void vg_btask(const vg_item& I, int item, long long w0, long l
                 const double jq = jf * jf;
 #pragma unroll
                 for (int i = 0; i < DD; ++i) {
-                    atomicAdd(&lw[i * NB + ids[i]], jq);
-                    atomicAdd(&lc[i * NB + ids[i]], 1u);
+                    atomicAdd(&lw[cofs + i * NB + ids[i]], jq);
+                    atomicAdd(&lc[cofs + i * NB + ids[i]], 1u);
                 }
                 a1 = a1 + jf;
                 a2 = a2 + jq;
@@ -1102,8 +1114,8 @@ vg_bfill(const vg_item* __restrict__ items, const int* __restrict__ task_item,
          double2* __restrict__ win_first, double2* __restrict__ win_last,
          double* __restrict__ weights, unsigned* __restrict__ counts,
          const vg_real* __restrict__ theta_all, unsigned seed) {
-    __shared__ double lw[MD * NB];
-    __shared__ unsigned lc[MD * NB];
+    __shared__ double lw[(VG_BCOPIES - 1) * VG_BXS + MD * NB];
+    __shared__ unsigned lc[(VG_BCOPIES - 1) * VG_BXS + MD * NB];
     const unsigned it = (unsigned)*it_p;
     for (int task = blockIdx.x; task < ntasks; task += gridDim.x) {
         const int item = task_item[task];
@@ -1118,7 +1130,10 @@ vg_bfill(const vg_item* __restrict__ items, const int* __restrict__ task_item,
         const vg_real* theta = theta_all + I.theta_off;
         double* jf1 = JF + I.cube_off;
         double* jf2 = JF2 + I.cube_off;
-        for (int i = threadIdx.x; i < d * NB; i += VG_BLOCK) { lw[i] = 0.0; lc[i] = 0u; }
+        for (int i = threadIdx.x; i < d * NB; i += VG_BLOCK) {
+#pragma unroll
+            for (int cp = 0; cp < VG_BCOPIES; ++cp) { lw[cp * VG_BXS + i] = 0.0; lc[cp * VG_BXS + i] = 0u; }
+        }
         __syncthreads();
         switch (d) {
 #define VG_BTASK_CASE(n)                                                                         \
@@ -1136,9 +1151,15 @@ vg_bfill(const vg_item* __restrict__ items, const int* __restrict__ task_item,
         }
         __syncthreads();
         for (int i = threadIdx.x; i < d * NB; i += VG_BLOCK) {
-            const unsigned c = lc[i];
+            unsigned c = lc[i];
+            double wsum = lw[i];
+#pragma unroll
+            for (int cp = 1; cp < VG_BCOPIES; ++cp) {  // the other copies, in copy order
+                c += lc[cp * VG_BXS + i];
+                wsum = wsum + lw[cp * VG_BXS + i];
+            }
             if (c) {
-                unsafeAtomicAdd(&weights[I.map_off + i], lw[i]);
+                unsafeAtomicAdd(&weights[I.map_off + i], wsum);
                 atomicAdd(&counts[I.map_off + i], c);
             }
         }

@@ -156,6 +156,21 @@ def test_run_len_must_be_power_of_two():
         assert rc == CVG_ERR_INVALID
 
 
+@pytest.mark.parametrize("opts,max_dim,nbins", [("bin_copies=3", 12, 256), ("bin_copies=0", 4, 64),
+                                                ("bin_copies=4", 12, 1024), ("block=96", 4, 64)])
+def test_batch_launch_options_are_checked(opts, max_dim, nbins):
+    """The batched program refuses a bin-copy count other than 1, 2 or 4, copies
+    whose bins do not fit in LDS (4 x 12 dimensions x 1024 bins x 12 bytes), and
+    a block that is not a multiple of 64, before it compiles anything."""
+    from cuvegas_amd._lib import CVG_ERR_INVALID, lib
+    h = C.c_void_p()
+    rc = lib.cvg_batch_program_create(b"__device__ vg_real integrand(vg_real* x){return x[0];}", max_dim,
+                                      C.byref(_params(n_dim=max_dim, n_intervals=nbins)), opts.encode(),
+                                      C.byref(h))
+    assert rc == CVG_ERR_INVALID
+    assert not h.value
+
+
 def test_program_cache_reuses_compiled_programs():
     """integrate() reuses the hipRTC program of an identical (source, shape,
     options) and compiles a new one when any of them changes; the cache is

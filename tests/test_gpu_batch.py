@@ -103,16 +103,17 @@ def test_batch_graph_matches_eager():
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_batch_launch_shapes_agree(prec):
     """One block per task (the default) and a resident grid striding over the
-    tasks, at two block sizes and task lengths: identical per-cube sums,
-    allocation and first-iteration estimates, maps within 1e-12 (the bin
-    atomics' order differs)."""
+    tasks, at three block sizes and task lengths, with 1, 2 and 4 copies of
+    the LDS bins: identical per-cube sums, allocation and first-iteration
+    estimates, maps within 1e-12 (the bin atomics' order differs)."""
     from cuvegas_amd.batch import FAMILY_SOURCE, Batch, BatchProgram, serving_workload
     _say("batch-launch-shapes", prec)
     items, theta, _, _ = serving_workload(24, seed=9)
     items = [(d, min(N, 200_000), off) for d, N, off in items]
     ref = None
-    for opts in ("block=256 task_windows=4", "block=256 task_windows=4 fill_grid=0",
-                 "block=512 task_windows=2 fill_grid=0", "block=1024 task_windows=1"):
+    for opts in ("block=256 task_windows=4 bin_copies=1", "block=256 task_windows=4 fill_grid=0",
+                 "block=512 task_windows=2 fill_grid=0", "block=512 task_windows=2 bin_copies=1",
+                 "block=1024 task_windows=1", "block=1024 task_windows=1 bin_copies=4"):
         prog = BatchProgram(FAMILY_SOURCE, 12, 128, prec, options=opts)
         b = Batch(prog, items, theta, max_it=2, skip=0, seed=4)
         b.run(1, use_graph=False)
