@@ -267,7 +267,10 @@ typedef struct cvg_batch_item {
 /* shape supplies n_intervals (<= 1024), precision and run_len; options may
  * hold block=N (threads per block), task_windows=N (windows per wave in one
  * task), fill_grid=0|1 (1, the default: one block per task; 0: a resident
- * grid striding over the tasks) and hipRTC options. */
+ * grid striding over the tasks), bin_copies=1|2|4 (copies of the block's LDS
+ * bins, lane l adding into copy l mod copies; default 2 when the blocks that
+ * fill a CU's 16 waves still fit in LDS with them, else 1; CVG_ERR_INVALID
+ * when they do not fit) and hipRTC options. */
 int cvg_batch_program_create(const char* integrand_src, int max_dim, const cvg_params* shape,
                              const char* options, cvg_program** out);
 /* common supplies n_intervals, precision, run_len, max_it, skip, alpha, beta, seed. */
