@@ -896,7 +896,7 @@ vg_grad_reduce(const double* __restrict__ part, int n_units, double* __restrict_
 // ---------------------------------------------------------------------------
 // Item descriptor; the host writes the same 80-byte layout (vg_host.cpp).
 struct vg_item {
-    long long cube_off;  // first cube of the item (nh, JF, JF2, dh)
+    long long cube_off;  // first cube of the item (nh, per-cube sums, dh)
     long long nhs_off;   // first entry of the item's nh_sum (C+1 entries)
     long long win_off;   // first window (win_first / win_last); runs start at 64 * win_off
     double V;            // cube volume n_strat^-d
@@ -1017,7 +1017,7 @@ template <int DD>
 __device__ inline void vg_btask(const vg_item& I, int item, long long w0, long long S, long long nwin,
                                 unsigned it, const vg_map_t* __restrict__ map,
                                 const long long* __restrict__ nhs, const int* __restrict__ run_cube,
-                                double* __restrict__ jf1, double* __restrict__ jf2,
+                                double2* __restrict__ js,
                                 double2* __restrict__ win_first, double2* __restrict__ win_last,
                                 const vg_real* __restrict__ theta, unsigned seed, double* lw,
                                 unsigned* lc) {
@@ -1050,7 +1050,7 @@ __device__ inline void vg_btask(const vg_item& I, int item, long long w0, long l
             if (s < re) {
                 if (s == cend) {
                     if (npieces == 0 && sb) { hasF = true; hF = h; f1 = a1; f2 = a2; }
-                    else { jf1[h] = a1; jf2[h] = a2; }
+                    else js[h] = make_double2(a1, a2);
                     ++npieces;
                     ++h;
                     vg_bdigits_next<DD>(kn, I);
@@ -1089,11 +1089,11 @@ __device__ inline void vg_btask(const vg_item& I, int item, long long w0, long l
                 double t1 = f1, t2 = f2;
                 if (lane > 0) { t1 = q1 + t1; t2 = q2 + t2; }
                 if (startF < wstart) win_first[gw] = make_double2(t1, t2);
-                else { jf1[hF] = t1; jf2[hF] = t2; }
+                else js[hF] = make_double2(t1, t2);
             }
             if (!cont) {
                 if (cstart < wstart) win_first[gw] = make_double2(s1, s2);
-                else { jf1[h] = s1; jf2[h] = s2; }
+                else js[h] = make_double2(s1, s2);
             } else if (lane == 63) {
                 win_last[gw] = make_double2(s1, s2);
                 if (cstart < wstart) win_first[gw] = make_double2(s1, s2);
@@ -1110,7 +1110,7 @@ vg_bfill(const vg_item* __restrict__ items, const int* __restrict__ task_item,
          const int* __restrict__ task_w0, int ntasks, const vg_map_t* __restrict__ gmap,
          const long long* __restrict__ nh_sum, const int* __restrict__ run_cube,
          const long long* __restrict__ s_item, const int* __restrict__ it_p,
-         double* __restrict__ JF, double* __restrict__ JF2,
+         double2* __restrict__ JS,
          double2* __restrict__ win_first, double2* __restrict__ win_last,
          double* __restrict__ weights, unsigned* __restrict__ counts,
          const vg_real* __restrict__ theta_all, unsigned seed) {
@@ -1128,8 +1128,7 @@ vg_bfill(const vg_item* __restrict__ items, const int* __restrict__ task_item,
         const vg_map_t* map = gmap + I.map_off;
         const long long* nhs = nh_sum + I.nhs_off;
         const vg_real* theta = theta_all + I.theta_off;
-        double* jf1 = JF + I.cube_off;
-        double* jf2 = JF2 + I.cube_off;
+        double2* js = JS + I.cube_off;
         for (int i = threadIdx.x; i < d * NB; i += VG_BLOCK) {
 #pragma unroll
             for (int cp = 0; cp < VG_BCOPIES; ++cp) { lw[cp * VG_BXS + i] = 0.0; lc[cp * VG_BXS + i] = 0u; }
@@ -1139,8 +1138,8 @@ vg_bfill(const vg_item* __restrict__ items, const int* __restrict__ task_item,
 #define VG_BTASK_CASE(n)                                                                         \
     case n:                                                                                      \
         if constexpr (n <= MD)                                                                   \
-            vg_btask<n>(I, item, w0, S, nwin, it, map, nhs, run_cube, jf1, jf2, win_first,       \
-                        win_last, theta, seed, lw, lc);                                          \
+            vg_btask<n>(I, item, w0, S, nwin, it, map, nhs, run_cube, js, win_first, win_last,   \
+                        theta, seed, lw, lc);                                                    \
         break;
             VG_BTASK_CASE(1) VG_BTASK_CASE(2) VG_BTASK_CASE(3) VG_BTASK_CASE(4)
             VG_BTASK_CASE(5) VG_BTASK_CASE(6) VG_BTASK_CASE(7) VG_BTASK_CASE(8)

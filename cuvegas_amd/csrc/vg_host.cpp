@@ -1467,7 +1467,8 @@ struct cvg_batch {
     int *task_item = nullptr, *task_w0 = nullptr, *blk_item = nullptr, *rd_item = nullptr, *rd_dim = nullptr;
     int* nh = nullptr;
     long long* nh_sum = nullptr;
-    double *JF = nullptr, *JF2 = nullptr, *dh = nullptr;
+    double2* JS = nullptr;  // per-cube (sum J f, sum (J f)^2)
+    double* dh = nullptr;
     int* run_cube = nullptr;
     double2 *win_first = nullptr, *win_last = nullptr;
     double *part = nullptr, *dsum = nullptr;
@@ -1502,7 +1503,7 @@ struct cvg_batch {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
-        void* bufs[] = {fitems, bitems, task_item, task_w0, blk_item, rd_item, rd_dim, nh, nh_sum, JF, JF2,
+        void* bufs[] = {fitems, bitems, task_item, task_w0, blk_item, rd_item, rd_dim, nh, nh_sum, JS,
                         dh, run_cube, win_first, win_last, part, dsum, bsum, boff, s_item, it_p, err,
                         hist, x_all, map64, map32, weights, counts, theta, fold_list, fold_count, big_list};
         for (void* b : bufs)
@@ -1518,7 +1519,7 @@ struct cvg_batch {
                                           x_all, weights, counts, map64, map32));
         VG_HIP(vg_launch_batch_iteration_steps(
             stream, bitems, n_items, tot_blk, blk_item, n_refine, rd_item, rd_dim, p.n_intervals, p.alpha,
-            p.beta, k_shift, p.max_it, nh, nh_sum, JF, JF2, win_first, win_last, dh, part, dsum, bsum, boff,
+            p.beta, k_shift, p.max_it, nh, nh_sum, JS, win_first, win_last, dh, part, dsum, bsum, boff,
             s_item, run_cube, it_p, hist, x_all, weights, counts, map64, map32, err, phase, fold_list,
             fold_count, fold_cap, task_item, task_w0, n_tasks, (prog->block / 64) * prog->task_w, big_list,
             big_cap));
@@ -1538,9 +1539,8 @@ struct cvg_batch {
         int nt = n_tasks;
         void* args[] = {(void*)&fitems,  (void*)&task_item, (void*)&task_w0,  (void*)&nt,
                         (void*)&map,     (void*)&nh_sum,    (void*)&run_cube, (void*)&s_item,
-                        (void*)&it_p,    (void*)&JF,        (void*)&JF2,      (void*)&win_first,
-                        (void*)&win_last, (void*)&weights,  (void*)&counts,   (void*)&theta,
-                        (void*)&seed};
+                        (void*)&it_p,    (void*)&JS,        (void*)&win_first, (void*)&win_last,
+                        (void*)&weights, (void*)&counts,    (void*)&theta,     (void*)&seed};
         VG_HIP(hipModuleLaunchKernel(prog->bfill, grid, 1, 1, (unsigned)prog->block, 1, 1, 0, stream, args,
                                      nullptr));
         return CVG_OK;
@@ -1783,8 +1783,7 @@ int cvg_batch_create(cvg_program* prog, const cvg_params* common, const cvg_batc
     if ((rc = upload(s, &b->rd_dim, rd_dim))) return rc;
     if ((rc = dalloc(&b->nh, b->tot_cubes))) return rc;
     if ((rc = dalloc(&b->nh_sum, b->tot_nhs))) return rc;
-    if ((rc = dalloc(&b->JF, b->tot_cubes))) return rc;
-    if ((rc = dalloc(&b->JF2, b->tot_cubes))) return rc;
+    if ((rc = dalloc(&b->JS, b->tot_cubes))) return rc;
     if ((rc = dalloc(&b->dh, b->tot_cubes))) return rc;
     if ((rc = dalloc(&b->run_cube, b->tot_win * 64))) return rc;
     if ((rc = dalloc(&b->win_first, b->tot_win))) return rc;
@@ -1983,8 +1982,9 @@ int cvg_batch_get_cube_sums(cvg_batch* b, int item, double* jf, double* jf2) {
     if (!b || !jf || !jf2 || item < 0 || item >= b->n_items) return fail(CVG_ERR_INVALID, "bad argument");
     const vg_bitem& g = b->hitems[(size_t)item];
     VG_HIP(hipStreamSynchronize(b->stream));
-    VG_HIP(hipMemcpy(jf, b->JF + g.cube_off, sizeof(double) * g.C, hipMemcpyDeviceToHost));
-    VG_HIP(hipMemcpy(jf2, b->JF2 + g.cube_off, sizeof(double) * g.C, hipMemcpyDeviceToHost));
+    std::vector<double2> js((size_t)g.C);
+    VG_HIP(hipMemcpy(js.data(), b->JS + g.cube_off, sizeof(double2) * g.C, hipMemcpyDeviceToHost));
+    for (long long h = 0; h < g.C; ++h) { jf[h] = js[(size_t)h].x; jf2[h] = js[(size_t)h].y; }
     return CVG_OK;
 }
 

@@ -80,7 +80,7 @@ hipError_t vg_launch_batch_iteration_steps(hipStream_t st, const vg_bitem* items
                                            long long nblk, const int* blk_item, int n_refine,
                                            const int* rd_item, const int* rd_dim, int n,
                                            double alpha, double beta, int k_shift, int hist_cap,
-                                           int* nh, long long* nh_sum, double* JF, double* JF2,
+                                           int* nh, long long* nh_sum, double2* JS,
                                            const double2* win_first, const double2* win_last,
                                            double* dh, double* part, double* dsum, long long* bsum,
                                            long long* boff, long long* s_item, int* run_cube,

@@ -249,7 +249,7 @@ __device__ void vg_fold_tree(long long wa, long long m, const double2* __restric
 // cubes spanning 2..VG_FOLD_MAX windows, found at window starts -- the cube
 // holding the first sample of window w (run_cube[64 w]) crosses from window
 // w - 1 when it starts there; its partials are left-folded in window order,
-// win_last[w - 1] + win_first[w] + ... + win_first[wb], into JF / JF2.
+// win_last[w - 1] + win_first[w] + ... + win_first[wb], into JS.
 #define VG_BIGFOLD_BLOCKS 128
 __global__ void __launch_bounds__(256)
 vg_bigfold_kernel(const int* __restrict__ list, const int* __restrict__ count, int cap,
@@ -943,8 +943,8 @@ hipError_t vg_launch_snapshot(hipStream_t st, const int* it_p, int skip, int cap
 // (4n + 1 + n/1024 + 2 doubles).
 __global__ void __launch_bounds__(256)
 vg_bresults_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ blk_item,
-                   const int* __restrict__ nh, const double* __restrict__ JF,
-                   const double* __restrict__ JF2, double beta, double* __restrict__ dh,
+                   const int* __restrict__ nh, const double2* __restrict__ JS, double beta,
+                   double* __restrict__ dh,
                    double* __restrict__ part, int n_refine, const int* __restrict__ rd_item,
                    const int* __restrict__ rd_dim, int n, double alpha, double* __restrict__ x_all,
                    double* __restrict__ weights, unsigned* __restrict__ counts,
@@ -969,7 +969,12 @@ vg_bresults_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ b
     for (int j = 0; j < 4; ++j) {
         const long long h = base + 256 * j;
         ne[j] = 2; j1[j] = 0.0; j2[j] = 0.0;
-        if (h < I.C) { ne[j] = nh[I.cube_off + h]; j1[j] = JF[I.cube_off + h]; j2[j] = JF2[I.cube_off + h]; }
+        if (h < I.C) {
+            ne[j] = nh[I.cube_off + h];
+            const double2 v = JS[I.cube_off + h];
+            j1[j] = v.x;
+            j2[j] = v.y;
+        }
     }
     double r[4][VG_NRES];
 #pragma unroll
@@ -1108,15 +1113,14 @@ vg_bscan_blocks_kernel(const vg_bitem* __restrict__ items, const long long* __re
 // found at window starts over all items' windows (walking the sampling
 // kernel's task list, (item, first window) per task_windows windows): the cube
 // holding the first sample of window w crosses from window w - 1 when it
-// starts there; its partials are left-folded in window order into JF / JF2.
+// starts there; its partials are left-folded in window order into JS.
 __global__ void __launch_bounds__(256)
 vg_bbigfold_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ task_item,
                    const int* __restrict__ task_w0, int n_tasks, int task_windows,
                    const long long* __restrict__ list, const int* __restrict__ count, int cap,
                    const long long* __restrict__ nh_sum, const int* __restrict__ run_cube_all,
                    const long long* __restrict__ s_item, const double2* __restrict__ win_first,
-                   const double2* __restrict__ win_last, int ws_shift, double* __restrict__ JF,
-                   double* __restrict__ JF2) {
+                   const double2* __restrict__ win_last, int ws_shift, double2* __restrict__ JS) {
     __shared__ double red[256];
     __shared__ double part[2 * 2048];
     if (blockIdx.x >= VG_BIGFOLD_BLOCKS) {
@@ -1144,8 +1148,7 @@ vg_bbigfold_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ t
                 j1 = j1 + v.x;
                 j2 = j2 + v.y;
             }
-            JF[I.cube_off + h] = j1;
-            JF2[I.cube_off + h] = j2;
+            JS[I.cube_off + h] = make_double2(j1, j2);
         }
         return;
     }
@@ -1158,7 +1161,7 @@ vg_bbigfold_kernel(const vg_bitem* __restrict__ items, const int* __restrict__ t
         const long long wa = (st >> ws_shift) + I.win_off, wb = ((en - 1) >> ws_shift) + I.win_off;
         double o1, o2;
         vg_fold_tree(wa, wb - wa + 1, win_first, win_last, part, red, o1, o2);
-        if (threadIdx.x == 0) { JF[I.cube_off + h] = o1; JF2[I.cube_off + h] = o2; }
+        if (threadIdx.x == 0) JS[I.cube_off + h] = make_double2(o1, o2);
     }
 }
 
@@ -1289,7 +1292,7 @@ hipError_t vg_launch_batch_iteration_steps(hipStream_t st, const vg_bitem* items
                                            long long nblk, const int* blk_item, int n_refine,
                                            const int* rd_item, const int* rd_dim, int n,
                                            double alpha, double beta, int k_shift, int hist_cap,
-                                           int* nh, long long* nh_sum, double* JF, double* JF2,
+                                           int* nh, long long* nh_sum, double2* JS,
                                            const double2* win_first, const double2* win_last,
                                            double* dh, double* part, double* dsum, long long* bsum,
                                            long long* boff, long long* s_item, int* run_cube,
@@ -1307,12 +1310,12 @@ hipError_t vg_launch_batch_iteration_steps(hipStream_t st, const vg_bitem* items
         hipLaunchKernelGGL(vg_bbigfold_kernel, dim3((unsigned)(VG_BIGFOLD_BLOCKS + wblocks)), dim3(256), 0, st,
                            items, task_item,
                            task_w0, n_tasks, task_windows, fold_list, fold_count, fold_cap, nh_sum, run_cube,
-                           s_item, win_first, win_last, 6 + k_shift, JF, JF2);
+                           s_item, win_first, win_last, 6 + k_shift, JS);
         // the refinement blocks' workspace, or the results blocks' tree buffer
         size_t lds = sizeof(double) * (size_t)(4LL * n + 1 + n / VG_TREE + 2);
         if (lds < sizeof(double) * VG_NRES * 256) lds = sizeof(double) * VG_NRES * 256;
         hipLaunchKernelGGL(vg_bresults_kernel, dim3((unsigned)(nblk + n_refine)), dim3(256), lds, st, items,
-                           blk_item, nh, JF, JF2, beta, dh, part, n_refine, rd_item, rd_dim, n, alpha, x_all,
+                           blk_item, nh, JS, beta, dh, part, n_refine, rd_item, rd_dim, n, alpha, x_all,
                            weights, counts, map64, map32);
         hipLaunchKernelGGL(vg_bfinalize_kernel, dim3(n_items), dim3(256), 0, st, items, part, s_item,
                            it_p, hist_cap, hist, dsum, err);
