@@ -26,11 +26,13 @@ BATCH_SHAPES = [
     # max_dim, n_intervals, items (d, n_eval_it, theta offset)
     (6, 64, [(2, 20_000, 0), (5, 60_000, 0), (3, 3_000, 0), (4, 250_000, 0), (1, 5_000, 0), (6, 40_000, 0)]),
     (16, 256, [(16, 50_000, 0), (9, 30_000, 0), (12, 20_000, 0)]),
+    # a bin count that is not a power of two (map widths not pre-scaled)
+    (5, 100, [(3, 30_000, 0), (5, 80_000, 0), (1, 4_000, 0), (4, 12_000, 0)]),
 ]
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("shape", BATCH_SHAPES, ids=["maxd6", "maxd16"])
+@pytest.mark.parametrize("shape", BATCH_SHAPES, ids=["maxd6", "maxd16", "maxd5_nbins100"])
 def test_batch_items_match_single_integrals(prec, shape):
     from cuvegas_amd import Integrator
     from cuvegas_amd.batch import Batch, BatchProgram
