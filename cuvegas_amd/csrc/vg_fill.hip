@@ -10,7 +10,8 @@
 //                sample -> cube map comes from the nh prefix sums, so there is
 //                no evals[] array) and sums each cube's samples in order;
 //       level 1  the 64 runs of a window (one wave) combine the pieces of a
-//                cube that crosses runs with a segmented Hillis-Steele scan;
+//                cube that crosses runs with a segmented scan of DPP moves
+//                (vg_seg_scan64);
 //       level 2  a cube that crosses windows leaves its window partials in
 //                win_first / win_last; the results kernel folds them;
 //   * per-dimension bin sums of (J f)^2 and hit counts, privatised in LDS and
@@ -64,6 +65,47 @@ __device__ inline float vg_expr(float x) {
     const float e = __builtin_amdgcn_exp2f(hi);
     return __builtin_fmaf(e, lo * 0x1.62e430p-1f, e);
 }
+
+// Segmented inclusive scan across the 64 lanes of a wave, for keys that do
+// not decrease with the lane (each key's lanes are contiguous): s1, s2 become
+// the sums of the lane's segment up to and including itself.  DPP moves only
+// (VALU; no LDS traffic behind the bin atomics, no waits on them):
+//   1-4: row_shr 1, 2, 4, 8 inside each row of 16 lanes (lane l takes lane
+//        l - off of its row when that lane has the same key);
+//   5:   row_bcast 15, rows 1 and 3: lane 15 of the row before;
+//   6:   row_bcast 31, rows 2 and 3: lane 31.
+// After step 4 a lane holds its segment's sum within its row; step 5 adds the
+// previous row's part to rows 1 and 3, and then lane 31 holds its segment's
+// sum over rows 0-1, which step 6 adds to rows 2 and 3.  Each addition is
+// (earlier lanes) + (this lane), the order the oracle restates.
+template <int CTRL, int ROWS>
+__device__ inline double vg_dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int CTRL, int ROWS>
+__device__ inline void vg_seg_step(int key, double& s1, double& s2) {
+    // lanes the step does not write keep -2, which no key equals (keys >= -1)
+    const int kp = __builtin_amdgcn_update_dpp(-2, key, CTRL, ROWS, 0xf, false);
+    const double p1 = vg_dpp_f64<CTRL, ROWS>(s1);
+    const double p2 = vg_dpp_f64<CTRL, ROWS>(s2);
+    if (kp == key) {
+        s1 = p1 + s1;
+        s2 = p2 + s2;
+    }
+}
+__device__ inline void vg_seg_scan64(int key, double& s1, double& s2) {
+    vg_seg_step<0x111, 0xf>(key, s1, s2);  // row_shr:1
+    vg_seg_step<0x112, 0xf>(key, s1, s2);  // row_shr:2
+    vg_seg_step<0x114, 0xf>(key, s1, s2);  // row_shr:4
+    vg_seg_step<0x118, 0xf>(key, s1, s2);  // row_shr:8
+    vg_seg_step<0x142, 0xa>(key, s1, s2);  // row_bcast:15, rows 1 and 3
+    vg_seg_step<0x143, 0xc>(key, s1, s2);  // row_bcast:31, rows 2 and 3
+}
+// The value of lane l - 1 (wave_shr:1; lane 0 gets 0).
+__device__ inline double vg_lane_before(double v) { return vg_dpp_f64<0x138, 0xf>(v); }
 
 #include "vg_user.h"
 
@@ -515,16 +557,10 @@ vg_fill(const vg_map_t* __restrict__ gmap,      // [D][NB] (edge, width)
 
         // level 1: segmented scan of each run's last piece across the window
         const bool cont = cend > re;
-        int key = active ? h : -1;
+        const int key = active ? h : -1;
         double s1 = active ? a1 : 0.0, s2 = active ? a2 : 0.0;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int kp = __shfl_up(key, off, 64);
-            const double p1 = __shfl_up(s1, off, 64);
-            const double p2 = __shfl_up(s2, off, 64);
-            if (lane >= off && kp == key) { s1 = p1 + s1; s2 = p2 + s2; }
-        }
-        const double q1 = __shfl_up(s1, 1, 64), q2 = __shfl_up(s2, 1, 64);
+        vg_seg_scan64(key, s1, s2);
+        const double q1 = vg_lane_before(s1), q2 = vg_lane_before(s2);
         const long long wstart = w * WS;
         if (active && write_sums) {
             if (hasF) {
@@ -1072,16 +1108,10 @@ __device__ inline void vg_btask(const vg_item& I, int item, long long w0, long l
             }
         }
         const bool cont = cend > re;
-        int key = active ? h : -1;
+        const int key = active ? h : -1;
         double s1 = active ? a1 : 0.0, s2 = active ? a2 : 0.0;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int kp = __shfl_up(key, off, 64);
-            const double p1 = __shfl_up(s1, off, 64);
-            const double p2 = __shfl_up(s2, off, 64);
-            if (lane >= off && kp == key) { s1 = p1 + s1; s2 = p2 + s2; }
-        }
-        const double q1 = __shfl_up(s1, 1, 64), q2 = __shfl_up(s2, 1, 64);
+        vg_seg_scan64(key, s1, s2);
+        const double q1 = vg_lane_before(s1), q2 = vg_lane_before(s2);
         const long long wstart = w * WS;
         const long long gw = I.win_off + w;
         if (active) {

@@ -17,7 +17,8 @@
  * Where the reference is order-nondeterministic (CUB reductions, atomics), this
  * file fixes one order, and the GPU build implements the same order:
  *   - per-cube sums: sequential within a run of K consecutive samples,
- *     segmented Hillis-Steele scan across the 64 runs of a window, sequential
+ *     segmented scan across the 64 runs of a window in the steps of the GPU's
+ *     DPP scan (within groups of 16 runs, then across groups), sequential
  *     fold across windows (orc_cube_sums);
  *   - reductions over cubes or bins: tree of 1024-wide strided pairwise sums,
  *     repeated over the partials (orc_tree_sum);
@@ -211,9 +212,12 @@ ORC_EXPORT double orc_tree_sum(const double* a, long long m) {
 /*  level 0: within a run, consecutive samples of one cube are summed in     */
 /*           order starting from 0.0 ("pieces");                             */
 /*  level 1: within a window, each run's last piece enters a segmented       */
-/*           Hillis-Steele inclusive scan over the 64 runs (offsets 1..32,   */
-/*           new = left + right when the keys match); a cube ending inside a */
-/*           run as that run's first piece takes scan(previous run) + piece; */
+/*           inclusive scan over the 64 runs in the GPU's wave order: steps  */
+/*           of 1, 2, 4, 8 runs inside each group of 16 runs, then run 15 of */
+/*           the group before into groups 1 and 3, then run 31 into groups 2 */
+/*           and 3 (new = left + right when the keys match); a cube ending   */
+/*           inside a run as that run's first piece takes scan(previous run) */
+/*           + piece;                                                        */
 /*  level 2: a cube spanning windows folds its window partials left to right.*/
 /* The GPU fill kernel implements the same three levels.                     */
 /* ------------------------------------------------------------------------ */
@@ -237,11 +241,22 @@ static void window_finish(cube_acc* A, long long w) {
         s1[l] = A->active[l] ? A->l1[l] : 0.0;
         s2[l] = A->active[l] ? A->l2[l] : 0.0;
     }
-    for (int off = 1; off < ORC_WAVE; off <<= 1) {
+    /* six steps; step t takes lane src(t, l) (or none, -1) into lane l */
+    for (int t = 0; t < 6; ++t) {
         for (int l = 0; l < ORC_WAVE; ++l) {
-            if (l >= off && key[l - off] == key[l]) {
-                n1[l] = s1[l - off] + s1[l];
-                n2[l] = s2[l - off] + s2[l];
+            const int row = l >> 4;
+            int src = -1;
+            if (t < 4) {
+                const int off = 1 << t;                       /* row_shr 1, 2, 4, 8 */
+                if ((l & 15) >= off) src = l - off;
+            } else if (t == 4) {
+                if (row == 1 || row == 3) src = 16 * row - 1; /* row_bcast 15 */
+            } else {
+                if (row >= 2) src = 31;                       /* row_bcast 31 */
+            }
+            if (src >= 0 && key[src] == key[l]) {
+                n1[l] = s1[src] + s1[l];
+                n2[l] = s2[src] + s2[l];
             } else {
                 n1[l] = s1[l];
                 n2[l] = s2[l];
