@@ -1066,37 +1066,39 @@ __device__ inline void vg_btask(const vg_item& I, int item, long long w0, long l
         const long long r = w * 64 + lane;
         const long long rs = r * K;
         const bool active = rs < S;
-        const long long re = active ? (rs + K < S ? rs + K : S) : rs;
-        int h = 0;
-        long long cstart = 0, cend = 0;
+        // the run's samples and cube ends as offsets from rs (32-bit; a cube
+        // end past the run is clamped to K + 1)
+        const int n = active ? (int)(rs + K < S ? K : S - rs) : 0;
+        const long long wstart = w * WS;
+        int h = 0, ce = 0;
+        bool sb = false, f_before = false;  // first cube started before the run / the window
         if (active) {
             h = run_cube[I.win_off * 64 + r];
-            cstart = nhs[h];
-            cend = nhs[h + 1];
+            const long long cstart = nhs[h], cend = nhs[h + 1];
+            ce = (int)(cend - rs < K + 1 ? cend - rs : K + 1);
+            sb = cstart < rs;
+            f_before = cstart < wstart;
         }
         unsigned kn[DD];
         vg_bdigits<DD>(kn, h, I);
-        const long long startF = cstart;
-        const bool sb = active && cstart < rs;
         bool hasF = false;
         int hF = 0, npieces = 0;
         double f1 = 0.0, f2 = 0.0, a1 = 0.0, a2 = 0.0;
         for (int j = 0; j < K; ++j) {
-            const long long s = rs + j;
-            if (s < re) {
-                if (s == cend) {
+            if (j < n) {
+                if (j == ce) {
                     if (npieces == 0 && sb) { hasF = true; hF = h; f1 = a1; f2 = a2; }
                     else js[h] = make_double2(a1, a2);
                     ++npieces;
                     ++h;
                     vg_bdigits_next<DD>(kn, I);
-                    cstart = cend;
-                    cend = nhs[h + 1];
+                    const long long cend = nhs[h + 1];
+                    ce = (int)(cend - rs < K + 1 ? cend - rs : K + 1);
                     a1 = 0.0; a2 = 0.0;
                 }
                 double jf;
                 unsigned ids[DD];
-                vg_beval<DD>(jf, ids, s, kn, it, seed, (unsigned)item, I, map, theta);
+                vg_beval<DD>(jf, ids, rs + j, kn, it, seed, (unsigned)item, I, map, theta);
                 const double jq = jf * jf;
 #pragma unroll
                 for (int i = 0; i < DD; ++i) {
@@ -1107,26 +1109,28 @@ __device__ inline void vg_btask(const vg_item& I, int item, long long w0, long l
                 a2 = a2 + jq;
             }
         }
-        const bool cont = cend > re;
+        // the current cube runs past the run; it started before the window
+        // only if it is the run's first cube and that one did
+        const bool cont = ce > n;
+        const bool c_before = npieces == 0 && f_before;
         const int key = active ? h : -1;
         double s1 = active ? a1 : 0.0, s2 = active ? a2 : 0.0;
         vg_seg_scan64(key, s1, s2);
         const double q1 = vg_lane_before(s1), q2 = vg_lane_before(s2);
-        const long long wstart = w * WS;
         const long long gw = I.win_off + w;
         if (active) {
             if (hasF) {
                 double t1 = f1, t2 = f2;
                 if (lane > 0) { t1 = q1 + t1; t2 = q2 + t2; }
-                if (startF < wstart) win_first[gw] = make_double2(t1, t2);
+                if (f_before) win_first[gw] = make_double2(t1, t2);
                 else js[hF] = make_double2(t1, t2);
             }
             if (!cont) {
-                if (cstart < wstart) win_first[gw] = make_double2(s1, s2);
+                if (c_before) win_first[gw] = make_double2(s1, s2);
                 else js[h] = make_double2(s1, s2);
             } else if (lane == 63) {
                 win_last[gw] = make_double2(s1, s2);
-                if (cstart < wstart) win_first[gw] = make_double2(s1, s2);
+                if (c_before) win_first[gw] = make_double2(s1, s2);
             }
         }
     }
